@@ -1,0 +1,265 @@
+// api.hip -- C ABI entry points of libqs3x (model, sectors, H.v, timing).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <utility>
+
+#include "qs3x_internal.h"
+
+namespace qs3x {
+
+static thread_local std::string g_err;
+static int g_device = 0;
+static hipStream_t g_stream = nullptr;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int device() { return g_device; }
+
+hipStream_t stream() {
+  if (!g_stream) {
+    QS3X_HIP(hipSetDevice(g_device));
+    QS3X_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  }
+  return g_stream;
+}
+
+template <class F>
+static int guarded(F &&f) {
+  try {
+    f();
+    return 0;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return 1;
+  } catch (const std::exception &e) {
+    set_error(std::string("qs3x: ") + e.what());
+    return 2;
+  } catch (...) {
+    set_error("qs3x: unknown error");
+    return 3;
+  }
+}
+
+}  // namespace qs3x
+
+using namespace qs3x;
+
+extern "C" {
+
+const char *qs3x_last_error(void) { return g_err.c_str(); }
+
+int qs3x_set_device(int32_t dev) {
+  return guarded([&] {
+    int n = 0;
+    QS3X_HIP(hipGetDeviceCount(&n));
+    QS3X_REQUIRE(dev >= 0 && dev < n, "qs3x_set_device: no such device");
+    if (g_stream && dev != g_device) {
+      (void)hipStreamDestroy(g_stream);
+      g_stream = nullptr;
+    }
+    g_device = dev;
+    QS3X_HIP(hipSetDevice(dev));
+  });
+}
+
+int qs3x_model_create(int32_t NOS, int32_t LX, int32_t LY, int32_t LZ, int32_t NOxxz,
+                      const int32_t *p_xxz, const double *jxy, const double *jz,
+                      qs3x_model **out) {
+  return guarded([&] {
+    QS3X_REQUIRE(out, "qs3x_model_create: out is NULL");
+    QS3X_REQUIRE(NOS > 0 && LX > 0 && LY > 0 && LZ > 0 && (int64_t)LX * LY * LZ == NOS,
+                 "qs3x_model_create: NOS must equal LX*LY*LZ");
+    QS3X_REQUIRE(NOS < 65536, "qs3x_model_create: NOS must be < 65536");
+    QS3X_REQUIRE(NOxxz >= 0 && (NOxxz == 0 || (p_xxz && jxy && jz)), "qs3x_model_create: bonds");
+    QS3X_HIP(hipSetDevice(g_device));
+    auto *m = new qs3x_model();
+    m->NOS = NOS; m->LX = LX; m->LY = LY; m->LZ = LZ; m->NOxxz = NOxxz;
+    m->chain = (LY == 1 && LZ == 1);
+    m->p1.resize(NOxxz); m->p2.resize(NOxxz);
+    m->jxy.assign(jxy, jxy + NOxxz);
+    m->jz.assign(jz, jz + NOxxz);
+    // read_ip (input_param.f90:92-103): swap so that p1 < p2; Jz/4 summed for
+    // the ferromagnetic energy (eigen_solver.f90:341-347).
+    double ezz = 0.0;
+    std::map<std::pair<double, double>, int> types;
+    std::vector<std::vector<int32_t>> adj(NOS + 1);
+    for (int b = 0; b < NOxxz; ++b) {
+      int a = p_xxz[2 * b], c = p_xxz[2 * b + 1];
+      if (!(a >= 1 && a <= NOS && c >= 1 && c <= NOS)) {
+        delete m;
+        throw Error{"qs3x_model_create: bond site out of range"};
+      }
+      if (a > c) std::swap(a, c);
+      m->p1[b] = a; m->p2[b] = c;
+      ezz += jz[b] / 4.0;
+      auto key = std::make_pair(jxy[b], jz[b]);
+      auto it = types.find(key);
+      int bt;
+      if (it == types.end()) {
+        bt = (int)m->bt_jxy2.size();
+        types[key] = bt;
+        m->bt_jxy2.push_back(jxy[b] * 0.5);
+        m->bt_jz4.push_back(jz[b] * 0.25);
+      } else {
+        bt = it->second;
+      }
+      if (bt >= 32768) {
+        delete m;
+        throw Error{"qs3x_model_create: more than 32767 distinct (Jxy, Jz) pairs"};
+      }
+      if (a == c) continue;  // a self-bond is never antiparallel: diagonal only
+      adj[a].push_back(c | (bt << 16));
+      adj[c].push_back(a | (bt << 16));
+    }
+    m->ezz = ezz;
+    m->adj_ptr.assign(NOS + 2, 0);
+    for (int s = 1; s <= NOS; ++s) {
+      m->adj_ptr[s + 1] = m->adj_ptr[s] + (int)adj[s].size();
+      m->max_deg = std::max<int>(m->max_deg, (int)adj[s].size());
+      for (int32_t e : adj[s]) m->adj.push_back(e);
+    }
+    m->adj_ptr[0] = 0;
+    m->d_adj_ptr.alloc(m->adj_ptr.size());
+    m->d_adj_ptr.upload(m->adj_ptr.data(), m->adj_ptr.size());
+    m->d_adj.alloc(std::max<size_t>(1, m->adj.size()));
+    if (!m->adj.empty()) m->d_adj.upload(m->adj.data(), m->adj.size());
+    std::vector<double> bt(2 * std::max<size_t>(1, m->bt_jxy2.size()), 0.0);
+    for (size_t q = 0; q < m->bt_jxy2.size(); ++q) {
+      bt[2 * q] = m->bt_jxy2[q];
+      bt[2 * q + 1] = m->bt_jz4[q];
+    }
+    m->d_bt.alloc(bt.size());
+    m->d_bt.upload(bt.data(), bt.size());
+    QS3X_HIP(hipStreamSynchronize(stream()));
+    *out = m;
+  });
+}
+
+int qs3x_model_destroy(qs3x_model *m) {
+  return guarded([&] { delete m; });
+}
+
+int qs3x_fm_energy(const qs3x_model *m, double *e) {
+  return guarded([&] {
+    QS3X_REQUIRE(m && e, "qs3x_fm_energy: NULL argument");
+    // calcu_FM_energy sums Jint(:,3) then divides by 4
+    double s = 0.0;
+    for (double z : m->jz) s += z;
+    *e = s / 4.0;
+  });
+}
+
+double qs3x_wavevector(int32_t K, int32_t L) {
+  const double PI = std::acos(-1.0);
+  return 2.0 * PI * (double)K / (double)L;
+}
+
+int qs3x_sector_build(qs3x_model *m, int32_t nod, double rkx, double rky, double rkz,
+                      qs3x_sector **out, int64_t *dim) {
+  return guarded([&] {
+    QS3X_REQUIRE(m && out, "qs3x_sector_build: NULL argument");
+    auto *s = new qs3x_sector();
+    s->model = m;
+    s->nod = nod;
+    s->rk[0] = rkx; s->rk[1] = rky; s->rk[2] = rkz;
+    try {
+      build_sector_device(s);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+    if (dim) *dim = s->dim;
+  });
+}
+
+int qs3x_sector_from_lists(qs3x_model *m, int32_t nod, int64_t dim, const int64_t *list_s,
+                           const double *list_r, const double *explist, qs3x_sector **out) {
+  return guarded([&] {
+    QS3X_REQUIRE(m && out && (dim == 0 || (list_s && list_r)), "qs3x_sector_from_lists: NULL argument");
+    auto *s = new qs3x_sector();
+    s->model = m;
+    s->nod = nod;
+    s->dim = dim;
+    try {
+      sector_from_lists(s, list_s, list_r, explist);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+  });
+}
+
+int qs3x_sector_destroy(qs3x_sector *s) {
+  return guarded([&] { delete s; });
+}
+
+int qs3x_sector_dim(const qs3x_sector *s, int64_t *dim) {
+  return guarded([&] {
+    QS3X_REQUIRE(s && dim, "qs3x_sector_dim: NULL argument");
+    *dim = s->dim;
+  });
+}
+
+int qs3x_sector_lists(const qs3x_sector *s, int64_t *list_s, double *list_r) {
+  return guarded([&] {
+    QS3X_REQUIRE(s, "qs3x_sector_lists: NULL sector");
+    if (list_s && s->dim) s->d_list_s.download(list_s, s->dim);
+    if (list_r && s->dim) s->d_list_r.download(list_r, s->dim);
+  });
+}
+
+int qs3x_sector_nnz(qs3x_sector *s, int64_t *nnz) {
+  return guarded([&] {
+    QS3X_REQUIRE(s && nnz, "qs3x_sector_nnz: NULL argument");
+    if (s->nnz < 0) s->nnz = count_nnz(s);
+    *nnz = s->nnz;
+  });
+}
+
+int qs3x_ham_to_vec_wave_vector(qs3x_sector *s, double *v0, const double *v1) {
+  return guarded([&] {
+    QS3X_REQUIRE(s && v0 && v1, "qs3x_ham_to_vec_wave_vector: NULL argument");
+    const size_t n = (size_t)2 * s->dim;
+    DevBuf<double> a(std::max<size_t>(n, 2)), b(std::max<size_t>(n, 2));
+    if (n) a.upload(v1, n);
+    launch_hv(s, QS3X_FP64, b.p, a.p, nullptr, nullptr, nullptr, stream());
+    if (n) b.download(v0, n);
+  });
+}
+
+int qs3x_hv_dev(qs3x_sector *s, int32_t prec, void *v0_dev, const void *v1_dev, void *st) {
+  return guarded([&] {
+    QS3X_REQUIRE(s && v0_dev && v1_dev, "qs3x_hv_dev: NULL argument");
+    QS3X_REQUIRE(prec == QS3X_FP64 || prec == QS3X_FP32, "qs3x_hv_dev: bad precision");
+    launch_hv(s, prec, v0_dev, v1_dev, nullptr, nullptr, nullptr,
+              st ? (hipStream_t)st : stream());
+  });
+}
+
+int qs3x_time_hv(qs3x_sector *s, int32_t prec, int32_t n_rep, double *ms) {
+  return guarded([&] {
+    QS3X_REQUIRE(s && ms && n_rep > 0, "qs3x_time_hv: bad argument");
+    const size_t es = (prec == QS3X_FP64) ? 16 : 8;
+    DevBuf<unsigned char> a(std::max<size_t>(1, es * s->dim)), b(std::max<size_t>(1, es * s->dim));
+    QS3X_HIP(hipMemsetAsync(a.p, 0, es * s->dim, stream()));
+    hipEvent_t e0, e1;
+    QS3X_HIP(hipEventCreate(&e0));
+    QS3X_HIP(hipEventCreate(&e1));
+    launch_hv(s, prec, b.p, a.p, nullptr, nullptr, nullptr, stream());
+    QS3X_HIP(hipEventRecord(e0, stream()));
+    for (int r = 0; r < n_rep; ++r) launch_hv(s, prec, b.p, a.p, nullptr, nullptr, nullptr, stream());
+    QS3X_HIP(hipEventRecord(e1, stream()));
+    QS3X_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    QS3X_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)t / n_rep;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,36 @@
+// blas.h -- launchers of the fused vector kernels (blas.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace qs3x {
+
+constexpr int kVecThreads = 256;
+constexpr int kMaxGroup = 8;    // basis vectors per pass of the multi-dot kernel
+constexpr int kMaxBasis = 512;  // largest Krylov basis (NOM + 1)
+
+int vec_grid(int64_t n);
+void vec_random(int prec, void *v, int64_t n, uint64_t seed, hipStream_t st);
+// dst = s * src with s = sc[is] (is >= 0) or s0; precisions may differ.
+void vec_copy_scale(int pd, void *dst, int ps, const void *src, int64_t n, const double *sc, int is, double s0,
+                    hipStream_t st);
+// block partials of <x, y> = sum conj(x) y; returns the block count.
+int vec_dot(int px, const void *x, int py, const void *y, int64_t n, double *part, hipStream_t st);
+// see k_finalize modes in blas.hip
+void finalize(const double *part, int nb, double *sc, int mode, int i0, int i1, double shift, hipStream_t st);
+int vec_update3(int prec, void *w, const void *u, const void *up, int64_t n, const double *sc, int ia, int isa,
+                int ib, int isb, double *part, hipStream_t st);
+void vec_axpy_acc(int prec, double *psi, const void *u, int64_t n, double z, const double *sc, int is,
+                  hipStream_t st);
+void vec_mdot(int prec, const void *const *V, const double *scales, int m, const void *w, int64_t n, double *part,
+              double *h, hipStream_t st);
+int vec_maxpy(int prec, const void *const *V, const double *scales, int m, const double *h, void *w, int64_t n,
+              double *part, hipStream_t st);
+
+// host eigensolvers (eig.cpp)
+double tridiag_eigenvalue(const double *d, const double *e, int n, int k);
+double tridiag_lowest(const double *d, const double *e, int n);
+void tridiag_vector(const double *d, const double *e, int n, double lam, double *z);
+void sym_eig(int n, double *A, double *w, double *V);
+
+}  // namespace qs3x

@@ -1,0 +1,523 @@
+// hv.hip -- matrix-free H.v in the U(1) x translation basis (pull form).
+//
+// Replaces ham_to_vec_wave_vector (ham2vec_v3.f90:37-72):
+//   v0(i) = sum_b [ antiparallel: -Jz_b/4 v1(i) + (Jxy_b/2) r(id)/r(i) explist(ell) v1(id)
+//                   parallel:     +Jz_b/4 v1(i) ]
+// where id is the row of the representative of the hopped configuration.
+//
+// MI355X layout: one row per lane, rows in ascending rank; a persistent grid
+// stages the bond adjacency, the binomial columns and the norm classes in LDS
+// once per workgroup.  Instead of scanning every bond of the bond file per row
+// (NOxxz x 2 N compares), each down spin visits only its own bonds; the
+// diagonal is e_zz - 2 sum_{antiparallel} Jz/4 with e_zz = sum_b Jz_b/4.
+//
+// Chain representative (LY = LZ = 1): with gaps g_q between cyclically
+// consecutive down spins, rotating down spin j to site 1 gives an image whose
+// largest elements are decided by the gaps preceding j, (g_{j-1}, g_{j-2}, ...).
+// The colex-minimal image is the rotation whose backward gap sequence is
+// lexicographically largest; packing each sequence into a 64-bit key makes the
+// search N integer maxima.  Ties are exactly the stabiliser (its order gives
+// r(id)); ell, the first translation in the reference's loop order reaching
+// the minimum (state_lists.f90:384-416), is L + 1 - (largest tied site), or 0
+// when the configuration already is the representative.  The representative's
+// gaps are read back from the winning key, ranked among configurations that
+// contain site 1, and looked up in a dense rank-addressed index (one 4-byte
+// read) -- replacing findstate's binary search over list_s.
+#include <algorithm>
+
+#include "qs3x_internal.h"
+
+namespace qs3x {
+namespace {
+
+template <class R>
+struct CT;
+template <>
+struct CT<double> {
+  using T = double2;
+};
+template <>
+struct CT<float> {
+  using T = float2;
+};
+
+struct HvArgs {
+  const uint64_t *cfg;
+  const double *list_r;
+  const int32_t *index;
+  const int64_t *list_s;
+  const void *v1;
+  void *v0;
+  int64_t dim;
+  int64_t ntiles;
+  int L, bits, kbits, mode;
+  int LX, LY, LZ, NOS;
+  const int32_t *adj_ptr;
+  const int32_t *adj;
+  int nadj;
+  const double *bt;  // interleaved jxy2, jz4
+  int nbt;
+  const int64_t *binom;  // global [n][k]
+  int kdim;
+  const double *rclass;
+  const double *explist;  // complex
+  double ezz;
+  const double *scale;  // device scalar or null
+  double *dot_partials; // 2 per block or null
+  unsigned long long *nnz_out;
+};
+
+constexpr int kThreads = 256;
+
+// Block reduction of a complex double into out[2*blockIdx.x].
+__device__ void block_reduce_store(double re, double im, double *out) {
+  __shared__ double red[2][kThreads / 64];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    re += __shfl_down(re, d, 64);
+    im += __shfl_down(im, d, 64);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wid] = re;
+    red[1][wid] = im;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0;
+    for (int w = 0; w < kThreads / 64; ++w) {
+      a += red[0][w];
+      b += red[1][w];
+    }
+    out[2 * blockIdx.x] = a;
+    out[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// ------------------------------------------------------------ chain kernel
+template <int N, class R, bool REALPH, bool NORMLIST, bool COUNT>
+__global__ __launch_bounds__(kThreads) void k_hv_chain(HvArgs a) {
+  using C = typename CT<R>::T;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int L = a.L;
+  // LDS: binom [N-1][L-1] int64 | rclass [N+1] R | jxy2 [nbt] R | jz4 [nbt] R |
+  //      adj_ptr [L+2] int | adj [nadj] int
+  int64_t *s_binom = (int64_t *)smem;
+  const int nb = (N > 1) ? (N - 1) * (L - 1) : 0;
+  R *s_rcl = (R *)(s_binom + nb);
+  R *s_jxy2 = s_rcl + (N + 1);
+  R *s_jz4 = s_jxy2 + a.nbt;
+  int *s_ptr = (int *)(s_jz4 + a.nbt);
+  int *s_adj = s_ptr + (L + 2);
+  for (int q = threadIdx.x; q < nb; q += kThreads) {
+    int m = q / (L - 1) + 1, x = q % (L - 1);
+    s_binom[q] = a.binom[(int64_t)x * a.kdim + m];
+  }
+  if (a.rclass)
+    for (int q = threadIdx.x; q <= N; q += kThreads) s_rcl[q] = (R)a.rclass[q];
+  for (int q = threadIdx.x; q < a.nbt; q += kThreads) {
+    s_jxy2[q] = (R)a.bt[2 * q];
+    s_jz4[q] = (R)a.bt[2 * q + 1];
+  }
+  for (int q = threadIdx.x; q < L + 2; q += kThreads) s_ptr[q] = a.adj_ptr[q];
+  for (int q = threadIdx.x; q < a.nadj; q += kThreads) s_adj[q] = a.adj[q];
+  __syncthreads();
+
+  const C *__restrict__ v1 = (const C *)a.v1;
+  C *__restrict__ v0 = (C *)a.v0;
+  const R scale = a.scale ? (R)(*a.scale) : (R)1;
+  const uint64_t fmask = (1ull << a.kbits) - 1ull;
+  const int kb = a.kbits;
+  double dre = 0.0, dim_ = 0.0;
+  unsigned long long nnz = 0;
+
+  for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    const int64_t row = tile * kThreads + threadIdx.x;
+    if (row >= a.dim) continue;
+    const uint64_t w = a.cfg[row];
+    int n[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) n[q] = (int)((w >> (q * a.bits)) & ((1ull << a.bits) - 1ull));
+    const R ri = NORMLIST ? (R)a.list_r[row] : (COUNT ? (R)1 : s_rcl[(int)(w >> 56)]);
+    const R rinv = (R)1 / ri;
+    C x = v1[row];
+    x.x *= scale;
+    x.y *= scale;
+    R anti = 0;
+    R accx = 0, accy = 0;
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+      const int s = n[m];
+      const int e0 = s_ptr[s], e1 = s_ptr[s + 1];
+      for (int e = e0; e < e1; ++e) {
+        const int ent = s_adj[e];
+        const int t = ent & 0xffff, btp = ent >> 16;
+        bool in = false;
+#pragma unroll
+        for (int q = 0; q < N; ++q) in |= (n[q] == t);
+        if (in) continue;
+        anti += s_jz4[btp];
+        // hopped configuration: remove n[m], insert t (sorted)
+        int r[N > 1 ? N - 1 : 1];
+#pragma unroll
+        for (int q = 0; q < N - 1; ++q) r[q] = (q < m) ? n[q] : n[q + 1];
+        int nn[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+          const bool ltq = (q < N - 1) ? (r[q < N - 1 ? q : 0] < t) : false;
+          const bool ltp = (q > 0) ? (r[q > 0 ? q - 1 : 0] < t) : true;
+          nn[q] = ltq ? r[q < N - 1 ? q : 0] : (ltp ? t : r[q > 0 ? q - 1 : 0]);
+        }
+        int g[N];
+#pragma unroll
+        for (int q = 0; q < N - 1; ++q) g[q] = nn[q + 1] - nn[q];
+        g[N - 1] = nn[0] + L - nn[N - 1];
+        uint64_t K[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          uint64_t k = 0;
+#pragma unroll
+          for (int tt = 1; tt < N; ++tt) k = (k << kb) | (uint64_t)g[(j - tt + N) % N];
+          K[j] = k;
+        }
+        uint64_t kmax = K[0];
+#pragma unroll
+        for (int j = 1; j < N; ++j) kmax = K[j] > kmax ? K[j] : kmax;
+        int count = 0, mx = 0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          const bool tie = (K[j] == kmax);
+          count += tie ? 1 : 0;
+          mx = (tie && nn[j] > mx) ? nn[j] : mx;
+        }
+        // anchored colex rank of the representative
+        int64_t c = 0;
+        if (N > 1) {
+          int f[N > 1 ? N - 1 : 1];
+          int sum = 0;
+#pragma unroll
+          for (int tt = 1; tt < N; ++tt) {
+            f[tt - 1] = (int)((kmax >> (kb * (N - 1 - tt))) & fmask);
+            sum += f[tt - 1];
+          }
+          int P = L - sum;  // gap following the anchor
+          c = s_binom[0 * (L - 1) + (P - 1)];
+#pragma unroll
+          for (int mm = 2; mm < N; ++mm) {
+            P += f[N - mm];  // g[j*+mm-1] = field (N - (mm-1))
+            c += s_binom[(mm - 1) * (L - 1) + (P - 1)];
+          }
+        }
+        const int id = a.index[c];
+        if (id < 0) continue;
+        if constexpr (COUNT) {
+          ++nnz;
+          continue;
+        }
+        const R rt = NORMLIST ? (R)a.list_r[id] : s_rcl[count];
+        const R cf = s_jxy2[btp] * rt * rinv;
+        C y = v1[id];
+        if constexpr (REALPH) {
+          accx += cf * y.x;
+          accy += cf * y.y;
+        } else {
+          const bool isrep = (nn[0] == 1) && (K[0] == kmax);
+          const int ell = isrep ? 0 : (L + 1 - mx);
+          const int ei = isrep ? 0 : (ell + 3 * (L + 1));
+          const R pr = (R)a.explist[2 * ei] * cf, pi = (R)a.explist[2 * ei + 1] * cf;
+          accx += pr * y.x - pi * y.y;
+          accy += pr * y.y + pi * y.x;
+        }
+      }
+    }
+    if constexpr (!COUNT) {
+      const R dg = (sizeof(R) == 8 ? (R)a.ezz : (R)0) - (R)2 * anti;
+      C out;
+      out.x = (dg * x.x + accx * scale);
+      out.y = (dg * x.y + accy * scale);
+      v0[row] = out;
+      if (a.dot_partials) {
+        dre += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
+        dim_ += (double)x.x * (double)out.y - (double)x.y * (double)out.x;
+      }
+    }
+  }
+  if constexpr (COUNT) {
+    for (int d = 32; d >= 1; d >>= 1) nnz += __shfl_down(nnz, d, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(a.nnz_out, nnz);
+  } else {
+    if (a.dot_partials) block_reduce_store(dre, dim_, a.dot_partials);
+  }
+}
+
+// ---------------------------------------------------------- general kernel
+// Any LX x LY x LZ lattice: the representative is found by scanning the
+// translation group in the reference's loop order (k, j, i) with a running
+// colex minimum and strict <, exactly as representative_SQ does.
+__device__ __forceinline__ int shift_site(int s, int i, int j, int k, int LX, int LY, int LZ) {
+  int s0 = s - 1;
+  int x = s0 % LX, y = (s0 / LX) % LY, z = s0 / (LX * LY);
+  x = (x + i) % LX;
+  y = (y + j) % LY;
+  z = (z + k) % LZ;
+  return 1 + x + LX * (y + LY * z);
+}
+
+template <class R, bool REALPH, bool COUNT>
+__global__ __launch_bounds__(kThreads) void k_hv_general(HvArgs a, int N) {
+  using C = typename CT<R>::T;
+  const C *__restrict__ v1 = (const C *)a.v1;
+  C *__restrict__ v0 = (C *)a.v0;
+  const R scale = a.scale ? (R)(*a.scale) : (R)1;
+  double dre = 0.0, dim_ = 0.0;
+  unsigned long long nnz = 0;
+  for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    const int64_t row = tile * kThreads + threadIdx.x;
+    if (row >= a.dim) continue;
+    const uint64_t w = a.cfg[row];
+    int n[kMaxNod], nn[kMaxNod], nd[kMaxNod], best[kMaxNod];
+    for (int q = 0; q < N; ++q) n[q] = (int)((w >> (q * a.bits)) & ((1ull << a.bits) - 1ull));
+    const R ri = (R)a.list_r[row];
+    C x = v1[row];
+    x.x *= scale;
+    x.y *= scale;
+    R anti = 0, accx = 0, accy = 0;
+    for (int m = 0; m < N; ++m) {
+      const int s = n[m];
+      for (int e = a.adj_ptr[s]; e < a.adj_ptr[s + 1]; ++e) {
+        const int ent = a.adj[e];
+        const int t = ent & 0xffff, btp = ent >> 16;
+        bool in = false;
+        for (int q = 0; q < N; ++q) in |= (n[q] == t);
+        if (in) continue;
+        anti += (R)a.bt[2 * btp + 1];
+        // hopped configuration, sorted
+        int p = 0;
+        for (int q = 0; q < N; ++q)
+          if (q != m) nn[p++] = n[q];
+        int pos = N - 1;
+        while (pos > 0 && nn[pos - 1] > t) {
+          nn[pos] = nn[pos - 1];
+          --pos;
+        }
+        nn[pos] = t;
+        for (int q = 0; q < N; ++q) best[q] = nn[q];
+        int ei = 0, ei_best = 0;
+        for (int k = 1; k <= a.LZ; ++k)
+          for (int j = 1; j <= a.LY; ++j)
+            for (int i = 1; i <= a.LX; ++i) {
+              for (int q = 0; q < N; ++q) nd[q] = shift_site(nn[q], i, j, k, a.LX, a.LY, a.LZ);
+              for (int q = 1; q < N; ++q) {
+                int v = nd[q], z = q - 1;
+                while (z >= 0 && nd[z] > v) {
+                  nd[z + 1] = nd[z];
+                  --z;
+                }
+                nd[z + 1] = v;
+              }
+              for (int l = N - 1; l >= 0; --l) {
+                if (nd[l] > best[l]) break;
+                if (nd[l] < best[l]) {
+                  ei = i + (a.LX + 1) * (j + (a.LY + 1) * k);
+                  ei_best = 1;
+                  for (int q = 0; q <= l; ++q) best[q] = nd[q];
+                  break;
+                }
+              }
+            }
+        int64_t rank = best[0];
+        for (int i2 = 2; i2 <= N; ++i2) rank += a.binom[(int64_t)(best[i2 - 1] - 1) * a.kdim + i2];
+        int64_t id = -1;
+        if (a.mode == kIndexFullRank) {
+          id = a.index[rank - 1];
+        } else {
+          int64_t lo = 0, hi = a.dim - 1;
+          while (lo <= hi) {
+            int64_t mid = lo + (hi - lo) / 2;
+            int64_t v = a.list_s[mid];
+            if (v < rank) lo = mid + 1;
+            else if (v > rank) hi = mid - 1;
+            else { id = mid; break; }
+          }
+        }
+        if (id < 0) continue;
+        if constexpr (COUNT) {
+          ++nnz;
+          continue;
+        }
+        const R cf = (R)a.bt[2 * btp] * (R)a.list_r[id] / ri;
+        C y = v1[id];
+        if (REALPH || !ei_best) {
+          accx += cf * y.x;
+          accy += cf * y.y;
+        } else {
+          const R pr = (R)a.explist[2 * ei] * cf, pi = (R)a.explist[2 * ei + 1] * cf;
+          accx += pr * y.x - pi * y.y;
+          accy += pr * y.y + pi * y.x;
+        }
+      }
+    }
+    if constexpr (!COUNT) {
+      const R dg = (sizeof(R) == 8 ? (R)a.ezz : (R)0) - (R)2 * anti;
+      C out;
+      out.x = dg * x.x + accx * scale;
+      out.y = dg * x.y + accy * scale;
+      v0[row] = out;
+      if (a.dot_partials) {
+        dre += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
+        dim_ += (double)x.x * (double)out.y - (double)x.y * (double)out.x;
+      }
+    }
+  }
+  if constexpr (COUNT) {
+    for (int d = 32; d >= 1; d >>= 1) nnz += __shfl_down(nnz, d, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(a.nnz_out, nnz);
+  } else {
+    if (a.dot_partials) block_reduce_store(dre, dim_, a.dot_partials);
+  }
+}
+
+// ----------------------------------------------------------------- dispatch
+using KFn = void (*)(HvArgs);
+
+template <int N>
+KFn pick_chain(int prec, bool realph, bool normlist, bool count) {
+  if (count) return k_hv_chain<N, double, true, false, true>;
+  if (prec == QS3X_FP64) {
+    if (normlist)
+      return realph ? k_hv_chain<N, double, true, true, false> : k_hv_chain<N, double, false, true, false>;
+    return realph ? k_hv_chain<N, double, true, false, false> : k_hv_chain<N, double, false, false, false>;
+  }
+  return realph ? k_hv_chain<N, float, true, false, false> : k_hv_chain<N, float, false, false, false>;
+}
+
+KFn chain_kernel(int N, int prec, bool realph, bool normlist, bool count) {
+  switch (N) {
+    case 1: return pick_chain<1>(prec, realph, normlist, count);
+    case 2: return pick_chain<2>(prec, realph, normlist, count);
+    case 3: return pick_chain<3>(prec, realph, normlist, count);
+    case 4: return pick_chain<4>(prec, realph, normlist, count);
+    case 5: return pick_chain<5>(prec, realph, normlist, count);
+    case 6: return pick_chain<6>(prec, realph, normlist, count);
+    case 7: return pick_chain<7>(prec, realph, normlist, count);
+    case 8: return pick_chain<8>(prec, realph, normlist, count);
+    case 9: return pick_chain<9>(prec, realph, normlist, count);
+    case 10: return pick_chain<10>(prec, realph, normlist, count);
+    case 11: return pick_chain<11>(prec, realph, normlist, count);
+    case 12: return pick_chain<12>(prec, realph, normlist, count);
+    default: return nullptr;
+  }
+}
+
+int persistent_grid(int64_t ntiles) {
+  static int cus = 0;
+  if (!cus) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device()) == hipSuccess) cus = p.multiProcessorCount;
+    if (cus <= 0) cus = 256;
+  }
+  int64_t g = (int64_t)cus * 8;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, ntiles));
+}
+
+}  // namespace
+
+static HvArgs make_args(qs3x_sector *s, void *v0, const void *v1, const double *scale,
+                        double *partials) {
+  qs3x_model *m = s->model;
+  HvArgs a{};
+  a.cfg = s->d_cfg.p;
+  a.list_r = s->d_list_r.p;
+  a.index = s->d_index.p;
+  a.list_s = s->d_list_s.p;
+  a.v1 = v1;
+  a.v0 = v0;
+  a.dim = s->dim;
+  a.ntiles = (s->dim + kThreads - 1) / kThreads;
+  a.L = m->NOS;
+  a.bits = s->bits;
+  int kb = 1;
+  while ((1 << kb) <= m->NOS) ++kb;
+  a.kbits = kb;
+  a.mode = s->mode;
+  a.LX = m->LX; a.LY = m->LY; a.LZ = m->LZ; a.NOS = m->NOS;
+  a.adj_ptr = m->d_adj_ptr.p;
+  a.adj = m->d_adj.p;
+  a.nadj = (int)m->adj.size();
+  a.bt = m->d_bt.p;
+  a.nbt = (int)m->bt_jxy2.size();
+  a.binom = s->d_binom.p;
+  a.kdim = s->nbinom_k;
+  a.rclass = s->d_rclass.p;
+  a.explist = s->d_explist.p;
+  a.ezz = m->ezz;
+  a.scale = scale;
+  a.dot_partials = partials;
+  return a;
+}
+
+static size_t chain_lds_bytes(qs3x_sector *s, int prec) {
+  const int N = s->nod, L = s->model->NOS;
+  const size_t rs = (prec == QS3X_FP64) ? 8 : 4;
+  size_t nb = (N > 1) ? (size_t)(N - 1) * (L - 1) : 0;
+  return nb * 8 + rs * (N + 1) + 2 * rs * s->model->bt_jxy2.size() + 4 * (size_t)(L + 2) +
+         4 * s->model->adj.size();
+}
+
+static bool use_chain(qs3x_sector *s, int prec) {
+  return s->chain && s->mode == kIndexAnchored && chain_lds_bytes(s, prec) <= 64 * 1024;
+}
+
+void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double *scale_dev,
+               double *dot_partials, int *nblocks_out, hipStream_t st) {
+  if (s->dim == 0) {
+    if (nblocks_out) *nblocks_out = 0;
+    return;
+  }
+  HvArgs a = make_args(s, v0, v1, scale_dev, dot_partials);
+  const int grid = persistent_grid(a.ntiles);
+  if (nblocks_out) *nblocks_out = grid;
+  const bool normlist = s->d_rclass.p == nullptr;  // sectors from caller lists carry list_r only
+  if (use_chain(s, prec) && !(normlist && prec != QS3X_FP64)) {
+    KFn f = chain_kernel(s->nod, prec, s->real_phase, normlist, false);
+    if (f) {
+      hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, prec), st, a);
+      QS3X_HIP(hipGetLastError());
+      return;
+    }
+  }
+  if (prec == QS3X_FP64) {
+    if (s->real_phase)
+      hipLaunchKernelGGL((k_hv_general<double, true, false>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
+    else
+      hipLaunchKernelGGL((k_hv_general<double, false, false>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
+  } else {
+    if (s->real_phase)
+      hipLaunchKernelGGL((k_hv_general<float, true, false>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
+    else
+      hipLaunchKernelGGL((k_hv_general<float, false, false>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
+  }
+  QS3X_HIP(hipGetLastError());
+}
+
+int64_t count_nnz(qs3x_sector *s) {
+  if (s->dim == 0) return 0;
+  DevBuf<unsigned long long> d_n(1);
+  unsigned long long zero = 0, nz = 0;
+  d_n.upload(&zero, 1);
+  HvArgs a = make_args(s, nullptr, nullptr, nullptr, nullptr);
+  a.nnz_out = d_n.p;
+  const int grid = persistent_grid(a.ntiles);
+  hipStream_t st = stream();
+  const bool normlist = s->d_rclass.p == nullptr;
+  KFn f = use_chain(s, QS3X_FP64) ? chain_kernel(s->nod, QS3X_FP64, true, normlist, true) : nullptr;
+  if (f)
+    hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, QS3X_FP64), st, a);
+  else
+    hipLaunchKernelGGL((k_hv_general<double, true, true>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
+  QS3X_HIP(hipGetLastError());
+  d_n.download(&nz, 1);
+  return (int64_t)nz;
+}
+
+}  // namespace qs3x

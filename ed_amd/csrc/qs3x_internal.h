@@ -1,0 +1,152 @@
+// qs3x_internal.h -- shared definitions for libqs3x (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/qs3x.h"
+
+namespace qs3x {
+
+// ----------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+struct Error {
+  std::string msg;
+};
+#define QS3X_HIP(call)                                                                 \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      throw ::qs3x::Error{std::string(#call) + ": " + hipGetErrorString(e_)};          \
+  } while (0)
+#define QS3X_REQUIRE(cond, msg)                                                        \
+  do {                                                                                 \
+    if (!(cond)) throw ::qs3x::Error{std::string(msg)};                                \
+  } while (0)
+
+hipStream_t stream();  // the library stream (default priority)
+int device();
+
+// Device buffer with RAII.
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  void alloc(size_t count) {
+    release();
+    n = count;
+    if (count) QS3X_HIP(hipMalloc(&p, sizeof(T) * count));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    release();
+    p = o.p; n = o.n; o.p = nullptr; o.n = 0;
+    return *this;
+  }
+  void upload(const T *h, size_t count) {
+    QS3X_HIP(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, stream()));
+  }
+  void download(T *h, size_t count) const {
+    QS3X_HIP(hipMemcpyAsync(h, p, sizeof(T) * count, hipMemcpyDeviceToHost, stream()));
+    QS3X_HIP(hipStreamSynchronize(stream()));
+  }
+};
+
+constexpr int kMaxNod = 16;      // largest N_down any kernel handles
+constexpr int kMaxChainNod = 12; // compile-time specialisations of the chain kernels
+
+}  // namespace qs3x
+
+// ------------------------------------------------------------------ model
+struct qs3x_model {
+  int32_t NOS, LX, LY, LZ, NOxxz;
+  std::vector<int32_t> p1, p2;       // 1-based, p1 <= p2 (reference swap)
+  std::vector<double> jxy, jz;       // raw couplings in bond-file order
+  double ezz = 0.0;                  // sum_b Jz_b / 4
+  bool chain = false;                // LY == LZ == 1
+  // Bond adjacency (self-bonds dropped: never antiparallel).  adj_ptr has
+  // NOS + 2 entries indexed by site (1..NOS); adj[k] = other | (btype << 16).
+  std::vector<int32_t> adj_ptr, adj;
+  std::vector<double> bt_jxy2, bt_jz4;  // per distinct (Jxy, Jz): Jxy/2, Jz/4
+  int32_t max_deg = 0;
+  // device copies
+  qs3x::DevBuf<int32_t> d_adj_ptr, d_adj;
+  qs3x::DevBuf<double> d_bt;  // [2*nbt]: jxy2, jz4 interleaved
+};
+
+// Sector: rows are the representatives in ascending rank.
+//
+// Packed configuration word (d_cfg): down-spin sites n_0 < ... < n_{N-1}
+// (1-based) in consecutive `bits`-wide fields from bit 0, and the norm class
+// in bits 56..63 (chain sectors: the stabiliser order; general sectors: 0).
+enum IndexMode : int32_t {
+  kIndexAnchored = 0,  // chain: dense table over configurations containing site 1
+  kIndexFullRank = 1,  // dense table over all C(NOS, N) ranks
+  kIndexSearch = 2     // binary search in list_s
+};
+
+struct qs3x_sector {
+  qs3x_model *model = nullptr;
+  int32_t nod = 0;
+  double rk[3] = {0, 0, 0};
+  int64_t dim = 0;
+  int32_t bits = 0;        // bits per site field in d_cfg
+  int32_t mode = 0;        // IndexMode
+  bool real_phase = true;  // every explist entry the operator can use is exactly 1
+  bool chain = false;      // chain fast path usable (LY = LZ = 1 and built on device)
+  int64_t index_len = 0;
+  int32_t nbinom_n = 0, nbinom_k = 0;  // binomial table C(n, k), n < nbinom_n, k < nbinom_k
+  qs3x::DevBuf<int64_t> d_list_s;
+  qs3x::DevBuf<double> d_list_r;
+  qs3x::DevBuf<uint64_t> d_cfg;
+  qs3x::DevBuf<int32_t> d_index;
+  qs3x::DevBuf<double> d_explist;   // complex, (LX+1)(LY+1)(LZ+1)
+  qs3x::DevBuf<double> d_rclass;    // chain: r = sqrt(ro) per stabiliser order (0..nod)
+  qs3x::DevBuf<int64_t> d_binom;    // row-major [n][k]
+  std::vector<double> h_rclass;
+  std::vector<double> h_explist;
+  int64_t nnz = -1;
+  // scratch for reductions
+  qs3x::DevBuf<double> d_partials;
+};
+
+namespace qs3x {
+
+// Host helpers (sector.hip)
+int64_t binom_host(int64_t n, int64_t k);  // exact C(n,k) with the reference's recurrence
+void build_sector_device(qs3x_sector *s);
+void sector_from_lists(qs3x_sector *s, const int64_t *list_s, const double *list_r,
+                       const double *explist);
+void compute_explist(const qs3x_model *m, const double rk[3], std::vector<double> &ex);
+
+// H.v launch (hv.hip). v0 = H (scale * v1) [- shift for FP32]; if dot_partials
+// is non-null each block writes sum_i conj(scale*v1_i) * v0_i (complex,
+// 2 doubles) to dot_partials[2*block] and nblocks_out receives the count.
+void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double *scale_dev,
+               double *dot_partials, int *nblocks_out, hipStream_t st);
+int64_t count_nnz(qs3x_sector *s);
+
+}  // namespace qs3x
+
+// ------------------------------------------------------------ device utils
+namespace qs3x {
+namespace dev {
+
+__device__ __forceinline__ int unpack_site(uint64_t cfg, int q, int bits) {
+  return (int)((cfg >> (q * bits)) & ((1ull << bits) - 1ull));
+}
+__device__ __forceinline__ int cfg_class(uint64_t cfg) { return (int)(cfg >> 56); }
+
+}  // namespace dev
+}  // namespace qs3x

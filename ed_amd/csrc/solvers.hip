@@ -1,0 +1,436 @@
+// solvers.hip -- device-resident Lanczos solvers (plain and thick restart).
+//
+// All Krylov vectors stay in HBM; per step the host only enqueues kernels.
+// Lanczos scalars live in a device array (see blas.hip); the host reads them
+// back at convergence checks and restarts, where it solves the small
+// projected eigenproblem (eig.cpp).
+//
+// FP32 mode: Krylov vectors are complex float and the operator applied is
+// H' = H - e_zz (e_zz = sum_b Jz_b/4, the ferromagnetic energy), which keeps
+// the FP32 diagonal O(number of antiparallel bonds) instead of O(L).  Krylov
+// spaces are shift invariant, so only the alphas move: alpha = alpha' + e_zz,
+// added in FP64.  Dots, norms, reorthogonalisation coefficients and the Ritz
+// rotation are FP64.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "blas.h"
+#include "qs3x_internal.h"
+
+using namespace qs3x;
+
+namespace qs3x {
+void ritz_rotate(int prec, void *base, int64_t dim, const double *scales_dev, int nom, int nok,
+                 const double *C_dev, hipStream_t st);
+}
+
+namespace {
+
+template <class F>
+int guarded2(F &&f) {
+  try {
+    f();
+    return 0;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return 1;
+  } catch (const std::exception &e) {
+    set_error(std::string("qs3x: ") + e.what());
+    return 2;
+  }
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+size_t elem_bytes(int prec) { return prec == QS3X_FP64 ? 16 : 8; }
+
+// Loads a start vector (host complex double) or generates the counter-based one.
+void load_start(int prec, void *dst, int64_t dim, const double *start, uint64_t seed, hipStream_t st) {
+  if (start) {
+    DevBuf<double> tmp((size_t)2 * dim);
+    tmp.upload(start, (size_t)2 * dim);
+    vec_copy_scale(prec, dst, QS3X_FP64, tmp.p, dim, nullptr, -1, 1.0, st);
+    QS3X_HIP(hipStreamSynchronize(st));
+  } else {
+    vec_random(prec, dst, dim, seed, st);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// ------------------------------------------------------------ plain Lanczos
+// lancz_eigen_val (lanczos.f90:50-152).
+int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, int64_t itrint, double conv,
+                 const double *start, uint64_t seed, double *ene, double *psi, int64_t *iters) {
+  return guarded2([&] {
+    QS3X_REQUIRE(s && ene, "qs3x_lanczos: NULL argument");
+    QS3X_REQUIRE(prec == QS3X_FP64 || prec == QS3X_FP32, "qs3x_lanczos: bad precision");
+    QS3X_REQUIRE(maxitr >= 1 && itrint >= 1, "qs3x_lanczos: maxitr and itrint must be >= 1");
+    hipStream_t st = stream();
+    const int64_t dim = s->dim;
+    QS3X_REQUIRE(dim >= 1, "qs3x_lanczos: empty sector");
+    const int64_t mitr = std::min<int64_t>(maxitr, dim);
+    const size_t es = elem_bytes(prec);
+    DevBuf<unsigned char> U[3];
+    for (auto &u : U) u.alloc(es * dim);
+    // scalar slots
+    const int64_t A_REC = 0, A_OUT = mitr, BETA = 2 * mitr, SCALE = 3 * mitr + 1;
+    DevBuf<double> sc((size_t)4 * mitr + 4);
+    DevBuf<double> part((size_t)2 * 4096);
+    const double shift = (prec == QS3X_FP32) ? s->model->ezz : 0.0;
+    load_start(prec, U[0].p, dim, start, seed, st);
+    int nb = vec_dot(prec, U[0].p, prec, U[0].p, dim, part.p, st);
+    finalize(part.p, nb, sc.p, 2, (int)(SCALE + mitr + 1), (int)SCALE, 0.0, st);  // scale[0] = 1/|u0|
+    std::vector<double> ha, hb;
+    double E = 0.0, tmp = 0.0;
+    int64_t ite = 0;
+    bool done = false;
+    for (int64_t i = 0; i < mitr && !done; ++i) {
+      void *cur = U[i % 3].p, *prev = U[(i + 2) % 3].p, *w = U[(i + 1) % 3].p;
+      launch_hv(s, prec, w, cur, sc.p + SCALE + i, part.p, &nb, st);
+      finalize(part.p, nb, sc.p, 1, (int)(A_REC + i), (int)(A_OUT + i), shift, st);
+      nb = vec_update3(prec, w, cur, prev, dim, sc.p, (int)(A_REC + i), (int)(SCALE + i),
+                       i > 0 ? (int)(BETA + i - 1) : -1, (int)(SCALE + i - 1), part.p, st);
+      finalize(part.p, nb, sc.p, 2, (int)(BETA + i), (int)(SCALE + i + 1), 0.0, st);
+      ite = i + 1;
+      if (ite == mitr || (ite >= minitr && ite % itrint == 0)) {
+        std::vector<double> all((size_t)4 * mitr + 4);
+        sc.download(all.data(), all.size());
+        ha.assign(all.begin() + A_OUT, all.begin() + A_OUT + ite);
+        hb.assign(all.begin() + BETA, all.begin() + BETA + ite);
+        if (ite == 1) {  // dim == 1 (lanczos.f90:95-101)
+          E = ha[0];
+          done = true;
+          break;
+        }
+        E = tridiag_lowest(ha.data(), hb.data(), (int)ite);
+        const double cv = std::fabs(tmp / E - 1.0);
+        const bool breakdown = !(hb[ite - 1] > 1e-13 * std::fabs(E) + 1e-300);
+        if (ite == maxitr || cv < conv || breakdown) done = true;
+        tmp = E;
+      }
+    }
+    ene[0] = E;
+    ene[1] = 99.0;  // the reference's sentinel for ene(2) (lanczos.f90:23-24)
+    if (iters) *iters = ite;
+    if (psi) {
+      DevBuf<double> p64((size_t)2 * dim);
+      QS3X_HIP(hipMemsetAsync(p64.p, 0, 16 * dim, st));
+      std::vector<double> z(ite);
+      if (ite == 1) z[0] = 1.0;
+      else tridiag_vector(ha.data(), hb.data(), (int)ite, E, z.data());
+      load_start(prec, U[0].p, dim, start, seed, st);
+      vec_axpy_acc(prec, p64.p, U[0].p, dim, z[0], sc.p, (int)SCALE, st);
+      for (int64_t i = 0; i + 1 < ite; ++i) {
+        void *cur = U[i % 3].p, *prev = U[(i + 2) % 3].p, *w = U[(i + 1) % 3].p;
+        launch_hv(s, prec, w, cur, sc.p + SCALE + i, nullptr, nullptr, st);
+        vec_update3(prec, w, cur, prev, dim, sc.p, (int)(A_REC + i), (int)(SCALE + i),
+                    i > 0 ? (int)(BETA + i - 1) : -1, (int)(SCALE + i - 1), nullptr, st);
+        vec_axpy_acc(prec, p64.p, w, dim, z[i + 1], sc.p, (int)(SCALE + i + 1), st);
+      }
+      nb = vec_dot(QS3X_FP64, p64.p, QS3X_FP64, p64.p, dim, part.p, st);
+      finalize(part.p, nb, sc.p, 2, 0, 1, 0.0, st);
+      vec_copy_scale(QS3X_FP64, p64.p, QS3X_FP64, p64.p, dim, sc.p, 1, 1.0, st);
+      p64.download(psi, (size_t)2 * dim);
+    }
+    QS3X_HIP(hipStreamSynchronize(st));
+  });
+}
+
+}  // extern "C"
+
+// -------------------------------------------------------- thick restart
+namespace {
+
+struct TR {
+  qs3x_sector *s;
+  int prec;
+  int NOK, NOM;
+  int64_t dim;
+  double shift;
+  hipStream_t st;
+  DevBuf<unsigned char> basis;  // (NOM+1) x dim, slot j at j*dim
+  DevBuf<double> sc;            // [0,NOM): alpha' ; [NOM,2NOM): beta ; [2NOM, 3NOM+1): scales ; tmp
+  DevBuf<double> part, h, Cdev;
+  DevBuf<const void *> vptr;
+  std::vector<const void *> hptr;
+  int64_t hv_count = 0;
+  int cgs_passes = 2;
+
+  int A() const { return 0; }
+  int Bt() const { return NOM; }
+  int S() const { return 2 * NOM; }
+  int TMP() const { return 3 * NOM + 2; }
+
+  void *slot(int j) { return basis.p + (size_t)j * dim * elem_bytes(prec); }
+
+  void init(qs3x_sector *s_, int prec_, int nok, int nom) {
+    s = s_;
+    prec = prec_;
+    NOK = nok;
+    NOM = nom;
+    dim = s->dim;
+    st = stream();
+    shift = (prec == QS3X_FP32) ? s->model->ezz : 0.0;
+    basis.alloc((size_t)(NOM + 1) * dim * elem_bytes(prec));
+    sc.alloc((size_t)3 * NOM + 8);
+    const int nb = vec_grid(dim);
+    part.alloc((size_t)2 * std::max(nb, 1) * (NOM + 2));
+    h.alloc((size_t)2 * (NOM + 2));
+    Cdev.alloc((size_t)NOM * NOM);
+    hptr.resize(NOM + 1);
+    for (int j = 0; j <= NOM; ++j) hptr[j] = slot(j);
+    vptr.alloc(NOM + 1);
+    vptr.upload(hptr.data(), NOM + 1);
+    QS3X_HIP(hipStreamSynchronize(st));
+  }
+
+  // slot j := v / |v| (scale slot set to 1/|v|, data unchanged)
+  void set_norm_scale(int j) {
+    int nb = vec_dot(prec, slot(j), prec, slot(j), dim, part.p, st);
+    finalize(part.p, nb, sc.p, 2, TMP(), S() + j, 0.0, st);
+  }
+
+  // One Lanczos step from slot i into slot i+1 with CGS2 against slots 0..i.
+  void step(int i) {
+    int nb = 0;
+    launch_hv(s, prec, slot(i + 1), slot(i), sc.p + S() + i, part.p, &nb, st);
+    ++hv_count;
+    finalize(part.p, nb, sc.p, 1, A() + i, TMP(), shift, st);
+    for (int pass = 0; pass < cgs_passes; ++pass) {
+      vec_mdot(prec, (const void *const *)vptr.p, sc.p + S(), i + 1, slot(i + 1), dim, part.p, h.p, st);
+      nb = vec_maxpy(prec, (const void *const *)vptr.p, sc.p + S(), i + 1, h.p, slot(i + 1), dim, part.p, st);
+    }
+    finalize(part.p, nb, sc.p, 2, Bt() + i, S() + i + 1, 0.0, st);
+  }
+
+  // alpha'[i0..i1), beta[i0..i1) to host
+  void read_ab(int i0, int i1, std::vector<double> &a, std::vector<double> &b) {
+    std::vector<double> all(sc.n);
+    sc.download(all.data(), all.size());
+    a.assign(all.begin() + A() + i0, all.begin() + A() + i1);
+    b.assign(all.begin() + Bt() + i0, all.begin() + Bt() + i1);
+  }
+
+  // slots 0..nok-1 := V[0..NOM-1] C[:, 0..nok-1] (C row-major NOM x NOM on host)
+  void rotate(const std::vector<double> &C) {
+    std::vector<double> cc((size_t)NOM * NOK);
+    for (int j = 0; j < NOM; ++j)
+      for (int k = 0; k < NOK; ++k) cc[(size_t)j * NOK + k] = C[(size_t)j * NOM + k];
+    Cdev.upload(cc.data(), cc.size());
+    ritz_rotate(prec, basis.p, dim, sc.p + S(), NOM, NOK, Cdev.p, st);
+    std::vector<double> ones(NOK, 1.0);
+    QS3X_HIP(hipMemcpyAsync(sc.p + S(), ones.data(), sizeof(double) * NOK, hipMemcpyHostToDevice, st));
+    QS3X_HIP(hipStreamSynchronize(st));
+  }
+
+  void copy_slot(int dst, int src) {
+    QS3X_HIP(hipMemcpyAsync(slot(dst), slot(src), dim * elem_bytes(prec), hipMemcpyDeviceToDevice, st));
+    QS3X_HIP(hipMemcpyAsync(sc.p + S() + dst, sc.p + S() + src, sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
+};
+
+// Thick-restart cycles (eigen_solver.f90:91-216).  On entry slot 0 holds the
+// start vector (any norm).  Returns the number of restarts; theta receives the
+// NOM Ritz values (shifted back by e_zz), slots 0..NOK-1 the Ritz vectors.
+struct TRResult {
+  int restarts = 0;
+  bool converged = false;
+  double max_change = 0.0;
+};
+
+TRResult tr_cycles(TR &t, int64_t maxitr, double tol, double abs_floor, std::vector<double> &theta,
+                   std::vector<double> &resid) {
+  const int NOK = t.NOK, NOM = t.NOM;
+  TRResult r;
+  t.set_norm_scale(0);
+  for (int i = 0; i < NOM; ++i) t.step(i);
+  std::vector<double> a, b;
+  t.read_ab(0, NOM, a, b);
+  std::vector<double> T((size_t)NOM * NOM, 0.0), C((size_t)NOM * NOM), w(NOM);
+  for (int i = 0; i < NOM; ++i) {
+    T[(size_t)i * NOM + i] = a[i];
+    if (i + 1 < NOM) T[(size_t)i * NOM + i + 1] = T[(size_t)(i + 1) * NOM + i] = b[i];
+  }
+  double beta_last = b[NOM - 1];
+  sym_eig(NOM, T.data(), w.data(), C.data());
+  std::vector<double> prev(a.begin(), a.begin() + NOK);  // calpha(1:NOK) = alpha (eigen_solver.f90:293-294)
+  int flag = 0;
+  int64_t tritr = 0;
+  for (tritr = 1; tritr <= maxitr; ++tritr) {
+    t.rotate(C);
+    double mc = 0.0;
+    for (int j = 0; j < NOK; ++j) {
+      const double e = w[j] + t.shift, pe = prev[j] + t.shift;
+      double ch = std::fabs(pe / e - 1.0);
+      if (abs_floor > 0) ch = std::min(ch, std::fabs(pe - e) / abs_floor * tol);
+      mc = std::max(mc, ch);
+    }
+    r.max_change = mc;
+    flag = (mc < tol) ? flag + 1 : 0;
+    if (flag == 2) {
+      r.converged = true;
+      break;
+    }
+    if (tritr == maxitr) break;
+    t.copy_slot(NOK, NOM);
+    for (int j = 0; j < NOK; ++j) prev[j] = w[j];
+    std::vector<double> cb(NOK);
+    for (int j = 0; j < NOK; ++j) cb[j] = beta_last * C[(size_t)(NOM - 1) * NOM + j];
+    for (int i = NOK; i < NOM; ++i) t.step(i);
+    t.read_ab(NOK, NOM, a, b);
+    std::fill(T.begin(), T.end(), 0.0);
+    for (int j = 0; j < NOK; ++j) {
+      T[(size_t)j * NOM + j] = w[j];
+      T[(size_t)j * NOM + NOK] = T[(size_t)NOK * NOM + j] = cb[j];
+    }
+    for (int i = NOK; i < NOM; ++i) {
+      T[(size_t)i * NOM + i] = a[i - NOK];
+      if (i + 1 < NOM) T[(size_t)i * NOM + i + 1] = T[(size_t)(i + 1) * NOM + i] = b[i - NOK];
+    }
+    beta_last = b[NOM - 1 - NOK];
+    sym_eig(NOM, T.data(), w.data(), C.data());
+  }
+  r.restarts = (int)std::min<int64_t>(tritr, maxitr);
+  theta.resize(NOM);
+  for (int j = 0; j < NOM; ++j) theta[j] = w[j] + t.shift;
+  resid.resize(NOK);
+  for (int j = 0; j < NOK; ++j) resid[j] = std::fabs(beta_last * C[(size_t)(NOM - 1) * NOM + j]);
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32_t NOM, int64_t maxitr,
+                   double conv, const double *start, uint64_t seed, double *ene, double *psi, double *stats) {
+  return guarded2([&] {
+    QS3X_REQUIRE(s && ene, "qs3x_trlanczos: NULL argument");
+    QS3X_REQUIRE(prec == QS3X_FP64 || prec == QS3X_FP32, "qs3x_trlanczos: bad precision");
+    // eigen_solver.f90:17-22
+    QS3X_REQUIRE(NOE >= 1 && NOE <= NOK, "ne must be eaqual or smaller than NOK!");
+    if (NOM < 2 * NOK) NOM = 2 * NOK;
+    const int64_t dim = s->dim;
+    QS3X_REQUIRE(!(dim < NOE || dim < NOK || dim < NOM), "qs3x_trlanczos: dim < NOM (stop 999)");
+    QS3X_REQUIRE(NOM + 1 <= kMaxBasis, "qs3x_trlanczos: NOM too large");
+    hipStream_t st = stream();
+    const double t0 = now_s();
+    std::vector<double> theta, resid;
+    double t32 = 0.0;
+    int64_t hv32 = 0, hv64 = 0;
+    int restarts = 0;
+    std::vector<double> startv;  // FP64 start for the refinement (host not needed: device copy)
+    TR t64;
+    if (prec == QS3X_FP32) {
+      TR t32s;
+      t32s.init(s, QS3X_FP32, NOK, NOM);
+      load_start(QS3X_FP32, t32s.slot(0), dim, start, seed, st);
+      // FP32 phase: stop at the FP32 resolution of the shifted spectrum
+      const double floor32 = 1e-6;
+      TRResult r = tr_cycles(t32s, maxitr, std::max(conv, 1e-30), floor32, theta, resid);
+      restarts += r.restarts;
+      hv32 = t32s.hv_count;
+      QS3X_HIP(hipStreamSynchronize(st));
+      t32 = now_s() - t0;
+      // FP64 refinement from the FP32 Ritz vectors: start vector = their sum
+      t64.init(s, QS3X_FP64, NOK, NOM);
+      DevBuf<double> acc((size_t)2 * dim);
+      QS3X_HIP(hipMemsetAsync(acc.p, 0, 16 * dim, st));
+      for (int j = 0; j < NOE; ++j) vec_axpy_acc(QS3X_FP32, acc.p, t32s.slot(j), dim, 1.0, t32s.sc.p, t32s.S() + j, st);
+      vec_copy_scale(QS3X_FP64, t64.slot(0), QS3X_FP64, acc.p, dim, nullptr, -1, 1.0, st);
+      QS3X_HIP(hipStreamSynchronize(st));
+    } else {
+      t64.init(s, QS3X_FP64, NOK, NOM);
+      load_start(QS3X_FP64, t64.slot(0), dim, start, seed, st);
+    }
+    TRResult r = tr_cycles(t64, maxitr, conv, 0.0, theta, resid);
+    restarts += r.restarts;
+    hv64 = t64.hv_count;
+    for (int j = 0; j < NOE; ++j) ene[j] = theta[j];
+    if (psi) {
+      for (int j = 0; j < NOE; ++j) {
+        // Ritz vectors are stored normalised to FP64 accuracy after rotation;
+        // normalise explicitly for the host copy.
+        t64.set_norm_scale(j);
+        DevBuf<double> tmp((size_t)2 * dim);
+        vec_copy_scale(QS3X_FP64, tmp.p, QS3X_FP64, t64.slot(j), dim, t64.sc.p, t64.S() + j, 1.0, st);
+        tmp.download(psi + (size_t)2 * dim * j, (size_t)2 * dim);
+      }
+    }
+    QS3X_HIP(hipStreamSynchronize(st));
+    if (stats) {
+      double mr = 0.0;
+      for (int j = 0; j < NOE && j < (int)resid.size(); ++j) mr = std::max(mr, resid[j]);
+      stats[0] = restarts;
+      stats[1] = (double)hv32;
+      stats[2] = (double)hv64;
+      stats[3] = now_s() - t0;
+      stats[4] = t32;
+      stats[5] = mr;
+      stats[6] = stats[7] = 0.0;
+    }
+  });
+}
+
+int qs3x_correlations(qs3x_sector *, const double *, int64_t, const int32_t *, double *, double *) {
+  set_error("qs3x_correlations: not implemented yet");
+  return 9;
+}
+
+int qs3x_local_mag(qs3x_sector *, const double *, double *) {
+  set_error("qs3x_local_mag: not implemented yet");
+  return 9;
+}
+
+int qs3x_apply_sq(qs3x_sector *, qs3x_sector *, int32_t, const double *, const double *, const double *,
+                  double *) {
+  set_error("qs3x_apply_sq: not implemented yet");
+  return 9;
+}
+
+int qs3x_dsf_recurrence(qs3x_sector *, int32_t, int32_t, const double *, int32_t, double, double *, double *,
+                        double *) {
+  set_error("qs3x_dsf_recurrence: not implemented yet");
+  return 9;
+}
+
+// DSF_analysis (source_DSFLan/main.f90:52-84).  Host arithmetic: O(levels x w).
+int qs3x_continued_fraction(int32_t nlev, const double *alpha, const double *beta2, double norm1, double E0,
+                            double eta, int64_t nomega, const double *omega, int32_t reference_quirk,
+                            double *dsf) {
+  return guarded2([&] {
+    QS3X_REQUIRE(nlev >= 1 && alpha && beta2 && omega && dsf, "qs3x_continued_fraction: bad argument");
+    const double pi = std::acos(-1.0);
+    const int M = nlev;
+    for (int64_t o = 0; o < nomega; ++o) {
+      const double w = omega[o];
+      // deepest level M: beta(M+1) = 0 (the file ends after M lines)
+      double kr = w + E0 - alpha[M - 1];
+      double bunbo = kr * kr + eta * eta;
+      double tr = beta2[M - 1] * kr / bunbo;
+      double ti = reference_quirk ? -beta2[M - 1] * eta : -beta2[M - 1] * eta / bunbo;
+      for (int n = M - 1; n >= 2; --n) {
+        kr = w + E0 - alpha[n - 1] - tr;
+        double ki = eta - ti;
+        bunbo = kr * kr + ki * ki;
+        tr = beta2[n - 1] * kr / bunbo;
+        ti = -beta2[n - 1] * ki / bunbo;
+      }
+      kr = w + E0 - alpha[0] - tr;
+      double ki = eta - ti;
+      bunbo = kr * kr + ki * ki;
+      ti = -norm1 * ki / bunbo;
+      dsf[o] = -ti / pi;
+    }
+  });
+}
+
+}  // extern "C"

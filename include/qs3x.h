@@ -1,0 +1,171 @@
+/*
+ * qs3x.h -- C ABI of libqs3x, an MI355X (gfx950) implementation of QS3's
+ * symmetry-adapted matrix-free Lanczos engine (U(1) x translation basis).
+ *
+ * Every function returns an int status: 0 on success, non-zero on error, with
+ * a message available from qs3x_last_error() (thread-local).  Plain C types
+ * only: int32_t / int64_t / double, complex numbers as interleaved
+ * (re, im) double pairs, opaque handles.  Host pointers unless a name says
+ * "_dev"; device pointers are plain HIP device addresses.
+ *
+ * Each entry point names the reference routine it replaces (file:line, paths
+ * relative to the QS-Cube/ED source tree).  The ISO_C_BINDING interfaces a
+ * Fortran driver binds are in ed_amd/fortran/qs3x_iso_c.f90.
+ *
+ * Integer kinds: ranks, dimensions and row indices are int64_t (C(1000,4)
+ * exceeds 2^31); sites, counts and lattice sizes are int32_t.  Ranks are the
+ * reference's 1-based colex ranks (state_lists.f90:131-139); list indices are
+ * 0-based in this ABI and 1-based in the Fortran shims.
+ */
+#ifndef QS3X_H
+#define QS3X_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qs3x_model qs3x_model;
+typedef struct qs3x_sector qs3x_sector;
+
+/* Precision of the vectors an operation works on. */
+enum {
+  QS3X_FP64 = 0, /* complex double vectors, FP64 arithmetic                     */
+  QS3X_FP32 = 1  /* complex float vectors; H applied as H - e_zz (e_zz = sum Jz/4,
+                    added back in FP64), dots and norms accumulated in FP64      */
+};
+
+/* ---------------------------------------------------------------- errors */
+const char *qs3x_last_error(void);
+/* Device the library runs on (HIP ordinal); default 0. */
+int qs3x_set_device(int32_t device);
+
+/* ----------------------------------------------------------------- model */
+/* Lattice and XXZ bond list.  Replaces read_ip's bond handling
+ * (input_param.f90:58-103: p1 < p2 swap, sJint = (Jxy/2, Jz/4)) and the
+ * translation tables mk_shift_{x,y,z}_SQ (state_lists.f90:36-64, main.f90:49-51).
+ * p_xxz: 2*NOxxz site indices (1-based), pairs in bond-file order;
+ * jxy, jz: NOxxz raw couplings (Jint(:,2), Jint(:,3)).  NOS = LX*LY*LZ. */
+int qs3x_model_create(int32_t NOS, int32_t LX, int32_t LY, int32_t LZ, int32_t NOxxz,
+                      const int32_t *p_xxz, const double *jxy, const double *jz,
+                      qs3x_model **out);
+int qs3x_model_destroy(qs3x_model *m);
+/* sum_b Jz_b/4: the energy of the N_down = 0 state.  Replaces calcu_FM_energy
+ * (eigen_solver.f90:341-347). */
+int qs3x_fm_energy(const qs3x_model *m, double *e);
+
+/* ---------------------------------------------------------------- sectors */
+/* Wave vector exactly as read_ip computes it: rk = 2*acos(-1)*K/L
+ * (input_param.f90:76-78). */
+double qs3x_wavevector(int32_t K, int32_t L);
+
+/* Builds the (N_down, rk) sector on the device: translation representatives
+ * (colex-minimal orbit members), their norms and a rank-addressed index.
+ * Replaces allocate_lists_omp_SQ (state_lists.f90:225-292) and, with
+ * rk = rk + q, allocate_lists_omp_SQ_new (state_lists.f90:418-486).
+ * THS (the sector dimension) is returned through dim. */
+int qs3x_sector_build(qs3x_model *m, int32_t nod, double rkx, double rky, double rkz,
+                      qs3x_sector **out, int64_t *dim);
+/* Same sector from caller-supplied lists (e.g. the reference's own list_s /
+ * list_r / explist), for drop-in use behind ham_to_vec_wave_vector: list_s
+ * ascending 1-based ranks, list_r norms, explist (LX+1)(LY+1)(LZ+1) complex
+ * column-major (input_param.f90:88-89). */
+int qs3x_sector_from_lists(qs3x_model *m, int32_t nod, int64_t dim, const int64_t *list_s,
+                           const double *list_r, const double *explist, qs3x_sector **out);
+int qs3x_sector_destroy(qs3x_sector *s);
+int qs3x_sector_dim(const qs3x_sector *s, int64_t *dim);
+/* Copies list_s (ranks) and list_r (norms) to host arrays of length dim
+ * (input_param.f90:33-34 globals list_s, list_r). */
+int qs3x_sector_lists(const qs3x_sector *s, int64_t *list_s, double *list_r);
+/* Number of off-diagonal hop terms with a target inside the sector (nnz of
+ * the matrix-free operator), counted on the device. */
+int qs3x_sector_nnz(qs3x_sector *s, int64_t *nnz);
+
+/* ------------------------------------------------------------------ H . v */
+/* v0 = H v1 with host vectors (complex double, interleaved, length dim).
+ * Replaces ham_to_vec_wave_vector (ham2vec_v3.f90:37-72).  Stages both
+ * vectors over PCIe; solvers below keep everything on the device. */
+int qs3x_ham_to_vec_wave_vector(qs3x_sector *s, double *v0, const double *v1);
+/* Device-pointer H . v on the library stream (hipStream_t passed as void*,
+ * NULL = the library's stream).  prec: QS3X_FP64 -> v0, v1 complex double;
+ * QS3X_FP32 -> complex float with the e_zz shift (v0 = (H - e_zz) v1). */
+int qs3x_hv_dev(qs3x_sector *s, int32_t prec, void *v0_dev, const void *v1_dev, void *stream);
+
+/* --------------------------------------------------------------- solvers */
+/* Plain Lanczos, device resident.  Replaces lanczos_routines /
+ * lancz_eigen_val (lanczos.f90:14-152): three-term recurrence without
+ * reorthogonalisation, lowest Ritz value every itrint steps past minitr,
+ * stop when |E_prev/E - 1| < conv; optional second pass for the eigenvector.
+ * start: host start vector (complex double, length dim) or NULL for the
+ * counter-based generator keyed by seed.  ene[0] = E0; psi (host, length
+ * dim, may be NULL) receives the normalised ground state; iters the number of
+ * Lanczos steps of the first pass. */
+int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, int64_t itrint,
+                 double conv, const double *start, uint64_t seed, double *ene, double *psi,
+                 int64_t *iters);
+
+/* Thick-restart Lanczos with full reorthogonalisation, device resident.
+ * Replaces my_trlanczos_routines_complex / lancz_eigen_val_complex
+ * (eigen_solver.f90:10-261).  NOM is raised to 2*NOK as in the reference.
+ * prec QS3X_FP32 runs the restarts on complex-float Krylov vectors (FP64
+ * dots, reorthogonalisation coefficients and Ritz rotation) and then refines
+ * in FP64 until the FP64 criterion holds.  ene (length NOE) receives the
+ * lowest Ritz values; psi (host, dim x NOE column-major, may be NULL) the
+ * Ritz vectors; stats (may be NULL, 8 doubles): [restarts, hv_fp32, hv_fp64,
+ * seconds_total, seconds_fp32, max_residual, 0, 0]. */
+int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32_t NOM,
+                   int64_t maxitr, double conv, const double *start, uint64_t seed,
+                   double *ene, double *psi, double *stats);
+
+/* ------------------------------------------------------------ observables */
+/* <S^z_i S^z_j> and <S^+_i S^-_j> for npairs ordered pairs (1-based sites,
+ * pairs[2q], pairs[2q+1]) in the state psi (host, complex double, length dim).
+ * Replaces get_cf_2_wave_vector / calcu_cf_trans_2 / mk_list_ij
+ * (get_expectation_values.f90:97-131,234-279; ham2vec_v3.f90:111-174):
+ * values are computed once per translation orbit and replicated.  The pair
+ * list is read as FILECF is (each line "a b" stored as list_expe_ss(1)=b,
+ * list_expe_ss(2)=a, get_expectation_values.f90:25-27) and must hold all NOS^2
+ * pairs, as the reference assumes. */
+int qs3x_correlations(qs3x_sector *s, const double *psi, int64_t npairs, const int32_t *pairs,
+                      double *szsz, double *spsm);
+/* <S^z_j> for all sites.  Replaces get_lm_2_wave_vector / calcu_lm_trans_2
+ * (get_expectation_values.f90:65-95,197-232; ham2vec_v3.f90:74-109). */
+int qs3x_local_mag(qs3x_sector *s, const double *psi, double *expe_sz);
+
+/* f = S^a_q |psi> into the target sector tgt.  Replaces calcu_Sq_a and
+ * search_representative_S{p,m,z}i_state_SQ (get_expectation_values.f90:281-377;
+ * state_lists.f90:488-583), as a pull-form gather (no atomics).
+ * spsmsz 1: S+, 2: S-, 3: Sz; q[3] the transferred wave vector; pos 3*NOS
+ * site positions (list_site_position_*). psi, f: host complex double. */
+int qs3x_apply_sq(qs3x_sector *gs, qs3x_sector *tgt, int32_t spsmsz, const double *q,
+                  const double *pos, const double *psi, double *f);
+
+/* DSF continued-fraction recurrence, device resident.  Replaces
+ * calcu_DSF_wavevector (get_expectation_values.f90:379-605) after the S_q step:
+ * f (host, target-sector vector, not normalised); H - rfield*sztot_new on the
+ * target sector; itr_dsf steps.  alpha[itr_dsf], beta2[itr_dsf] (beta2[0] = 0,
+ * stored like beta_dsf), norm1 = <f|f>.  prec FP32 keeps the three Lanczos
+ * vectors in complex float with FP64 coefficients.  nvec > 1 runs nvec
+ * independent recurrences in one batch: f is dim x nvec, alpha/beta2
+ * itr_dsf x nvec, norm1 nvec. */
+int qs3x_dsf_recurrence(qs3x_sector *tgt, int32_t prec, int32_t nvec, const double *f,
+                        int32_t itr_dsf, double rfield, double *alpha, double *beta2,
+                        double *norm1);
+
+/* S(q, w) = -Im G(w)/pi from alpha / beta2 (beta2[0] = 0).  Replaces
+ * DSF_analysis (source_DSFLan/main.f90:40-94).  z = w + E0 + i eta.
+ * reference_quirk = 1 reproduces DSFLan's deepest level (imaginary part not
+ * divided by the denominator, main.f90:64-65); 0 is the exact fraction. */
+int qs3x_continued_fraction(int32_t nlev, const double *alpha, const double *beta2, double norm1,
+                            double E0, double eta, int64_t nomega, const double *omega,
+                            int32_t reference_quirk, double *dsf);
+
+/* ---------------------------------------------------------- measurement */
+/* Times n_rep device H . v applications (events on the library stream) and
+ * returns the mean milliseconds per application. */
+int qs3x_time_hv(qs3x_sector *s, int32_t prec, int32_t n_rep, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QS3X_H */
