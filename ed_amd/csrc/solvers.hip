@@ -380,28 +380,6 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
   });
 }
 
-int qs3x_correlations(qs3x_sector *, const double *, int64_t, const int32_t *, double *, double *) {
-  set_error("qs3x_correlations: not implemented yet");
-  return 9;
-}
-
-int qs3x_local_mag(qs3x_sector *, const double *, double *) {
-  set_error("qs3x_local_mag: not implemented yet");
-  return 9;
-}
-
-int qs3x_apply_sq(qs3x_sector *, qs3x_sector *, int32_t, const double *, const double *, const double *,
-                  double *) {
-  set_error("qs3x_apply_sq: not implemented yet");
-  return 9;
-}
-
-int qs3x_dsf_recurrence(qs3x_sector *, int32_t, int32_t, const double *, int32_t, double, double *, double *,
-                        double *) {
-  set_error("qs3x_dsf_recurrence: not implemented yet");
-  return 9;
-}
-
 // DSF_analysis (source_DSFLan/main.f90:52-84).  Host arithmetic: O(levels x w).
 int qs3x_continued_fraction(int32_t nlev, const double *alpha, const double *beta2, double norm1, double E0,
                             double eta, int64_t nomega, const double *omega, int32_t reference_quirk,
