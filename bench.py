@@ -1,0 +1,195 @@
+"""bench.py -- headline benchmark of the symmetry-adapted Lanczos engine.
+
+Workload: periodic nearest-neighbour Heisenberg chain, L = 1000, Jxy = Jz = 1,
+N_down = 4, k = 0 (D = 41,417,250 translation representatives).
+
+A "step" is one full-sector H.v in the FP32-mixed path (complex-float
+vectors, operator H - e_zz, FP64 reductions), with inputs resident in HBM.
+value = H.v applications per second.  The same JSON line also reports the
+FP64 H.v rate, the time to the ground-state energy for the plain FP64 Lanczos
+path and for the FP32-mixed thick-restart path (with the error of each
+against the Bethe-ansatz energy 242.000099091345514), the roofline of the
+H.v kernel and the CPU baseline (the oracle's literal restatement of
+ham_to_vec_wave_vector timed on a bounded sample of rows).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "H·v/s & time-to-E0 @1e-9 rel, FP32-mixed vs FP64, L=1000 N↓=4 k=0"
+E0_EXACT = 242.000099091345514  # Bethe ansatz, L=1000, M=4 (real roots I = +-1/2, +-3/2)
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sec, budget_s=12.0):
+    """The oracle's literal H.v (all-bond scan, O(L N) representative search,
+    binary-search findstate) on chunks of rows spread over the sector, with
+    every host core, until about budget_s seconds of wall time are spent.
+    Returns H.v/s extrapolated to the full sector."""
+    from oracle import qs3 as orc
+    L, N = 1000, 4
+    ls, lr = sec.lists()
+    om = orc.Model(L, L, 1, 1, orc.chain_bonds(L, [(1, 1.0, 1.0)]))
+    osec = orc.Sector(N, (0.0, 0.0, 0.0), ls, lr, orc.explist(L, 1, 1, 0.0, 0.0, 0.0))
+    rng = np.random.default_rng(0)
+    v = (rng.uniform(-1, 1, sec.dim) + 1j * rng.uniform(-1, 1, sec.dim)).astype(np.complex128)
+    threads = orc.lib().orc_max_threads()
+    chunk = 4096
+    starts = np.linspace(0, sec.dim - chunk, 64).astype(np.int64)
+    rows = 0
+    t0 = time.perf_counter()
+    for s in starts:
+        om.hv(osec, v, rows=(int(s), int(s) + chunk))
+        rows += chunk
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    rate = rows / dt / sec.dim
+    return {"value": rate, "unit": "H·v/s", "cores": int(threads), "kind": "port",
+            "sample": f"{rows} rows ({len([1 for _ in range(rows // chunk)])} chunks of {chunk}) of the L=1000 "
+                      f"N=4 k=0 sector in {dt:.1f} s, extrapolated to D={sec.dim}"}
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "hv_c3_fp32_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--NOK", type=int, default=4)
+    ap.add_argument("--NOM", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X; multi-GPU runs are not part of this build "
+              "(see DESIGN.md for how the H.v would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from ed_amd import qs3 as q3
+
+    L, N = 1000, 4
+    gm = q3.Model(L, L, 1, 1, q3.chain_bonds(L, [(1, 1.0, 1.0)]))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sec = q3.Sector.build(gm, N, (0.0, 0.0, 0.0))
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    D = sec.dim
+    nnz = sec.nnz()
+    log(f"sector D={D} built in {t_build:.3f} s, nnz={nnz} ({nnz / D:.3f} hops/row)")
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    res = {}
+    for name, prec, dt, sv in (("fp32", q3.FP32, torch.complex64, 8), ("fp64", q3.FP64, torch.complex128, 16)):
+        g = torch.Generator(device=dev)
+        g.manual_seed(1)
+        x = torch.randn(D, dtype=dt, device=dev, generator=g)
+        y = torch.empty_like(x)
+        for _ in range(args.warmup):
+            sec.hv_dev(y.data_ptr(), x.data_ptr(), prec, stream.cuda_stream)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(args.steps):
+            sec.hv_dev(y.data_ptr(), x.data_ptr(), prec, stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms_ev = e0.elapsed_time(e1) / args.steps
+        B = D * (16 + 2 * sv) + nnz * sv
+        res[name] = {"ms": ms_ev, "wall_ms": wall * 1e3 / args.steps, "hv_per_s": args.steps / wall,
+                     "alg_bytes": B, "alg_GBs": B / (ms_ev * 1e-3) / 1e9}
+        log(f"{name}: {ms_ev:.4f} ms/H.v (events), {wall * 1e3 / args.steps:.4f} ms (wall), "
+            f"{res[name]['alg_GBs']:.1f} GB/s algorithmic")
+        del x, y
+        torch.cuda.empty_cache()
+
+    solve = {}
+    if not args.no_solve:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        E64, _, it64 = q3.lanczos_routines(sec, maxitr=5000, minitr=20, itrint=5, lnc_ene_conv=1e-12, prec=q3.FP64,
+                                           seed=1, want_vector=False)
+        torch.cuda.synchronize()
+        t64 = time.perf_counter() - t0
+        log(f"plain FP64 Lanczos: E0={E64:.15f} in {t64:.2f} s, {it64} steps, rel err {abs(E64 / E0_EXACT - 1):.2e}")
+        t0 = time.perf_counter()
+        ene, _, st = q3.my_trlanczos_routines_complex(sec, NOE=1, NOK=args.NOK, NOM=args.NOM, maxitr=400,
+                                                       lnc_ene_conv=1e-12, prec=q3.FP32, seed=1, want_vectors=False)
+        torch.cuda.synchronize()
+        t32 = time.perf_counter() - t0
+        E32 = float(ene[0])
+        log(f"FP32-mixed thick restart: E0={E32:.15f} in {t32:.2f} s, stats {st}, rel err {abs(E32 / E0_EXACT - 1):.2e}")
+        solve = {"time_to_E0_fp64_plain_s": t64, "E0_fp64_plain": E64, "iters_fp64_plain": it64,
+                 "rel_err_fp64_plain": abs(E64 / E0_EXACT - 1),
+                 "time_to_E0_fp32_mixed_s": t32, "E0_fp32_mixed": E32,
+                 "rel_err_fp32_mixed": abs(E32 / E0_EXACT - 1), "fp32_mixed_stats": st,
+                 "speedup_fp32_mixed_vs_fp64_plain": t64 / t32 if t32 > 0 else None}
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(sec)
+            log(f"cpu baseline: {cpu}")
+        except Exception as e:  # the oracle library may be absent on a stripped tree
+            log(f"cpu baseline unavailable: {e}")
+
+    f32 = res["fp32"]
+    traffic = load_traffic()
+    roof = {"bound": "hbm", "achieved": f32["alg_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": f32["alg_GBs"] / HBM_PEAK_GBS,
+            "traffic": (traffic or {}).get("bytes_per_launch"),
+            "alg_bytes_per_launch": f32["alg_bytes"], "kernel": "k_hv_chain<4, float, real-phase>",
+            "avg_launch_ms": f32["ms"]}
+    out = {
+        "metric": METRIC,
+        "value": f32["hv_per_s"],
+        "unit": "H·v/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": f32["wall_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex-fp32 (fp64 reductions)",
+        "data": "synthetic",
+        "config": {"workload": "periodic NN Heisenberg chain L=1000, Jxy=Jz=1, N_down=4, k=0 "
+                               f"(D={D}), one full-sector H.v per step", "dim": D, "nnz": nnz,
+                   "parallelism": "1 GPU"},
+        "hv_per_s_fp64": res["fp64"]["hv_per_s"],
+        "ms_per_hv_fp64": res["fp64"]["ms"],
+        "sector_build_s": t_build,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    out.update(solve)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

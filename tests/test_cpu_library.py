@@ -1,0 +1,98 @@
+"""CPU-side checks of libqs3x: it loads without a GPU, exports every entry
+point include/qs3x.h declares, and its host-only pieces (small eigensolvers,
+continued fraction, start-vector formula) are right."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "ed_amd", "libqs3x.so")
+HDR = os.path.join(ROOT, "include", "qs3x.h")
+
+
+def _declared():
+    txt = open(HDR).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(qs3x_[a-z0-9_]+)\s*\(", txt)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.fail("ed_amd/libqs3x.so is not built (run ed_amd/build.py)")
+    return ctypes.CDLL(LIB)
+
+
+def test_exports_every_declared_symbol(lib):
+    names = _declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_python_mirror_imports():
+    from ed_amd import qs3 as q3
+    assert q3.wavevector(1, 4) == pytest.approx(np.pi / 2)
+    # the exact expression of read_ip: 2 * acos(-1) * K / L
+    assert q3.wavevector(3, 7) == 2.0 * np.arccos(-1.0) * 3.0 / 7.0
+
+
+def test_host_tridiagonal_solver(lib):
+    lib.qs3x_test_tridiag_eigenvalue.restype = ctypes.c_double
+    lib.qs3x_test_tridiag_eigenvalue.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+    lib.qs3x_test_tridiag_vector.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_void_p]
+    rng = np.random.default_rng(0)
+    for n in (1, 2, 3, 10, 200, 1500):
+        d = rng.normal(size=n) * 3
+        e = np.abs(rng.normal(size=max(n - 1, 1)))
+        T = np.diag(d) + (np.diag(e[: n - 1], 1) + np.diag(e[: n - 1], -1) if n > 1 else 0)
+        w, V = np.linalg.eigh(T)
+        for k in (0, 1, n - 1):
+            if k >= n:
+                continue
+            got = lib.qs3x_test_tridiag_eigenvalue(d.ctypes.data, e.ctypes.data, n, k)
+            assert got == pytest.approx(w[k], abs=1e-12 * max(1.0, np.max(np.abs(w))))
+        z = np.zeros(n)
+        lib.qs3x_test_tridiag_vector(d.ctypes.data, e.ctypes.data, n, ctypes.c_double(w[0]), z.ctypes.data)
+        assert abs(abs(np.dot(z, V[:, 0])) - 1.0) < 1e-9
+
+
+def test_host_dense_symmetric_solver(lib):
+    lib.qs3x_test_sym_eig.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    rng = np.random.default_rng(1)
+    for n in (1, 4, 17, 60):
+        A = rng.normal(size=(n, n))
+        A = A + A.T
+        ref = np.linalg.eigvalsh(A)
+        a = A.copy()
+        w = np.zeros(n)
+        V = np.zeros((n, n))
+        lib.qs3x_test_sym_eig(n, a.ctypes.data, w.ctypes.data, V.ctypes.data)
+        assert np.allclose(w, ref, atol=1e-12 * max(1, np.max(np.abs(ref))))
+        assert np.allclose(A @ V, V * w[None, :], atol=1e-10)
+
+
+def test_continued_fraction_matches_oracle():
+    from ed_amd import qs3 as q3
+    from oracle import qs3 as orc
+    rng = np.random.default_rng(2)
+    a = rng.normal(size=50)
+    b = np.concatenate([[0.0], np.abs(rng.normal(size=49))])
+    om = np.linspace(-2, 5, 1001)
+    for quirk in (True, False):
+        s1 = q3.dsf_analysis(a, b, 1.7, -0.3, om, 0.04, reference_quirk=quirk)
+        s2 = orc.continued_fraction(a, b, 1.7, -0.3, om, 0.04, reference_quirk=quirk)
+        assert np.allclose(s1, s2, rtol=1e-13, atol=0)
+
+
+def test_start_vector_formula_is_deterministic():
+    from ed_amd import qs3 as q3
+    a = q3.start_vector(1000, 42)
+    b = q3.start_vector(500, 42, row0=500)
+    assert np.array_equal(a[500:], b)
+    assert np.all(np.abs(a.real) < 1) and np.all(np.abs(a.imag) < 1)
+    assert abs(np.mean(a.real)) < 0.1 and abs(np.mean(a.imag)) < 0.1
