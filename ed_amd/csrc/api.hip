@@ -5,6 +5,7 @@
 #include <map>
 #include <utility>
 
+#include "blas.h"
 #include "qs3x_internal.h"
 
 namespace qs3x {
@@ -14,6 +15,11 @@ static int g_device = 0;
 static hipStream_t g_stream = nullptr;
 
 void set_error(const std::string &msg) { g_err = msg; }
+
+Options &options() {
+  static Options o;
+  return o;
+}
 
 int device() { return g_device; }
 
@@ -61,6 +67,25 @@ int qs3x_set_device(int32_t dev) {
     }
     g_device = dev;
     QS3X_HIP(hipSetDevice(dev));
+  });
+}
+
+int qs3x_set_option(const char *name, double value) {
+  return guarded([&] {
+    QS3X_REQUIRE(name, "qs3x_set_option: NULL name");
+    const std::string n(name);
+    if (n == "tr_reorth") {
+      QS3X_REQUIRE(value == 0 || value == 1, "tr_reorth must be 0 or 1");
+      options().tr_reorth = (int)value;
+    } else if (n == "tr_cgs_passes") {
+      QS3X_REQUIRE(value >= 1 && value <= 3, "tr_cgs_passes must be 1..3");
+      options().tr_cgs_passes = (int)value;
+    } else if (n == "fp32_floor") {
+      QS3X_REQUIRE(value > 0, "fp32_floor must be > 0");
+      options().fp32_floor = value;
+    } else {
+      throw Error{"qs3x_set_option: unknown option " + n};
+    }
   });
 }
 
@@ -235,8 +260,7 @@ int qs3x_hv_dev(qs3x_sector *s, int32_t prec, void *v0_dev, const void *v1_dev, 
   return guarded([&] {
     QS3X_REQUIRE(s && v0_dev && v1_dev, "qs3x_hv_dev: NULL argument");
     QS3X_REQUIRE(prec == QS3X_FP64 || prec == QS3X_FP32, "qs3x_hv_dev: bad precision");
-    launch_hv(s, prec, v0_dev, v1_dev, nullptr, nullptr, nullptr,
-              st ? (hipStream_t)st : stream());
+    launch_hv(s, prec, v0_dev, v1_dev, nullptr, nullptr, nullptr, (hipStream_t)st);
   });
 }
 

@@ -22,10 +22,19 @@ int vec_update3(int prec, void *w, const void *u, const void *up, int64_t n, con
                 int ib, int isb, double *part, hipStream_t st);
 void vec_axpy_acc(int prec, double *psi, const void *u, int64_t n, double z, const double *sc, int is,
                   hipStream_t st);
-void vec_mdot(int prec, const void *const *V, const double *scales, int m, const void *w, int64_t n, double *part,
-              double *h, hipStream_t st);
-int vec_maxpy(int prec, const void *const *V, const double *scales, int m, const double *h, void *w, int64_t n,
-              double *part, hipStream_t st);
+// V: device array of m vector pointers; scales[sidx ? sidx[j] : j] scales V[j]
+void vec_mdot(int prec, const void *const *V, const double *scales, const int *sidx, int m, const void *w, int64_t n,
+              double *part, double *h, hipStream_t st);
+int vec_maxpy(int prec, const void *const *V, const double *scales, const int *sidx, int m, const double *h,
+              void *w, int64_t n, double *part, hipStream_t st);
+
+// Tunables (qs3x_set_option): thick-restart reorthogonalisation and the FP32 phase.
+struct Options {
+  int tr_reorth = 0;          // 0: every previous basis vector; 1: locked Ritz vectors + last two
+  int tr_cgs_passes = 2;      // classical Gram-Schmidt passes per step
+  double fp32_floor = 1e-6;   // |delta theta| that ends the FP32 phase of the mixed thick restart
+};
+Options &options();
 
 // host eigensolvers (eig.cpp)
 double tridiag_eigenvalue(const double *d, const double *e, int n, int k);

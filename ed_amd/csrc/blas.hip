@@ -170,8 +170,9 @@ __global__ __launch_bounds__(kVecThreads) void k_axpy_acc(double2 *psi, const ty
 // h_j partials: part[(j * nb + b) * 2] = sum conj(s_j u_j) w over this block's
 // rows, for j in [j0, j0 + cnt), cnt <= kMaxGroup.
 template <class R>
-__global__ __launch_bounds__(kVecThreads) void k_mdot(const typename CT<R>::T *const *V, const double *scales, int j0,
-                                                      int cnt, const typename CT<R>::T *w, int64_t n, double *part) {
+__global__ __launch_bounds__(kVecThreads) void k_mdot(const typename CT<R>::T *const *V, const double *scales,
+                                                      const int *sidx, int j0, int cnt,
+                                                      const typename CT<R>::T *w, int64_t n, double *part) {
   double re[kMaxGroup], im[kMaxGroup];
 #pragma unroll
   for (int j = 0; j < kMaxGroup; ++j) re[j] = im[j] = 0.0;
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(kVecThreads) void k_mdot(const typename CT<R>::T *c
 #pragma unroll
   for (int j = 0; j < kMaxGroup; ++j) {
     if (j < cnt) {
-      const double s = scales ? scales[j0 + j] : 1.0;
+      const double s = scales ? scales[sidx ? sidx[j0 + j] : j0 + j] : 1.0;
       block_sum2(re[j] * s, im[j] * s, &part[2 * ((size_t)(j0 + j) * nb + blockIdx.x)],
                  &part[2 * ((size_t)(j0 + j) * nb + blockIdx.x) + 1]);
     }
@@ -215,12 +216,12 @@ __global__ __launch_bounds__(kVecThreads) void k_finalize_multi(const double *pa
 
 // w -= sum_j h_j s_j u_j (FP64 arithmetic, rounded to R once); part = |w|^2
 template <class R>
-__global__ __launch_bounds__(kVecThreads) void k_maxpy(const typename CT<R>::T *const *V, const double *scales, int m,
-                                                       const double *h, typename CT<R>::T *w, int64_t n,
-                                                       double *part) {
+__global__ __launch_bounds__(kVecThreads) void k_maxpy(const typename CT<R>::T *const *V, const double *scales,
+                                                       const int *sidx, int m, const double *h,
+                                                       typename CT<R>::T *w, int64_t n, double *part) {
   __shared__ double sh[2 * kMaxBasis];
   for (int j = threadIdx.x; j < m; j += blockDim.x) {
-    const double s = scales ? scales[j] : 1.0;
+    const double s = scales ? scales[sidx ? sidx[j] : j] : 1.0;
     sh[2 * j] = h[2 * j] * s;
     sh[2 * j + 1] = h[2 * j + 1] * s;
   }
@@ -321,30 +322,31 @@ void vec_axpy_acc(int prec, double *psi, const void *u, int64_t n, double z, con
 // h = V^H w for the m vectors of the device pointer array V (scaled by
 // scales[j] if given) -> h (complex, device, 2m doubles).  part must hold
 // 2 * m * vec_grid(n) doubles.
-void vec_mdot(int prec, const void *const *V, const double *scales, int m, const void *w, int64_t n, double *part,
-              double *h, hipStream_t st) {
+void vec_mdot(int prec, const void *const *V, const double *scales, const int *sidx, int m, const void *w, int64_t n,
+              double *part, double *h, hipStream_t st) {
   const int nb = vec_grid(n);
   for (int j0 = 0; j0 < m; j0 += kMaxGroup) {
     const int cnt = std::min(kMaxGroup, m - j0);
     if (prec == QS3X_FP64)
-      hipLaunchKernelGGL(k_mdot<double>, GRID(n), (const double2 *const *)V, scales, j0, cnt, (const double2 *)w, n,
-                         part);
+      hipLaunchKernelGGL(k_mdot<double>, GRID(n), (const double2 *const *)V, scales, sidx, j0, cnt,
+                         (const double2 *)w, n, part);
     else
-      hipLaunchKernelGGL(k_mdot<float>, GRID(n), (const float2 *const *)V, scales, j0, cnt, (const float2 *)w, n,
-                         part);
+      hipLaunchKernelGGL(k_mdot<float>, GRID(n), (const float2 *const *)V, scales, sidx, j0, cnt,
+                         (const float2 *)w, n, part);
   }
   hipLaunchKernelGGL(k_finalize_multi, dim3(m), dim3(kVecThreads), 0, st, part, nb, h);
   QS3X_HIP(hipGetLastError());
 }
 
-int vec_maxpy(int prec, const void *const *V, const double *scales, int m, const double *h, void *w, int64_t n,
-              double *part, hipStream_t st) {
+int vec_maxpy(int prec, const void *const *V, const double *scales, const int *sidx, int m, const double *h,
+              void *w, int64_t n, double *part, hipStream_t st) {
   QS3X_REQUIRE(m <= kMaxBasis, "vec_maxpy: too many basis vectors");
   const int nb = vec_grid(n);
   if (prec == QS3X_FP64)
-    hipLaunchKernelGGL(k_maxpy<double>, GRID(n), (const double2 *const *)V, scales, m, h, (double2 *)w, n, part);
+    hipLaunchKernelGGL(k_maxpy<double>, GRID(n), (const double2 *const *)V, scales, sidx, m, h, (double2 *)w, n,
+                       part);
   else
-    hipLaunchKernelGGL(k_maxpy<float>, GRID(n), (const float2 *const *)V, scales, m, h, (float2 *)w, n, part);
+    hipLaunchKernelGGL(k_maxpy<float>, GRID(n), (const float2 *const *)V, scales, sidx, m, h, (float2 *)w, n, part);
   QS3X_HIP(hipGetLastError());
   return nb;
 }

@@ -95,23 +95,36 @@ __device__ void block_reduce_store(double re, double im, double *out) {
 }
 
 // ------------------------------------------------------------ chain kernel
-template <int N, class R, bool REALPH, bool NORMLIST, bool COUNT>
+// Template parameters: N = N_down; R = vector precision; REALPH = every phase
+// the operator uses is 1 (k = 0); NORMLIST = norms read from list_r (sectors
+// supplied as lists) instead of the stabiliser-class table; COUNT = count the
+// in-sector hops instead of applying H; KeyT = uint32_t when the N-1 packed
+// gaps fit in 32 bits, else uint64_t.
+//
+// Fast path: the representative of a row is anchored at site 1 with the gap
+// ending at site 1 (the wrap gap) lexicographically first.  When a hop moves a
+// down spin other than the one at site 1 and the new wrap gap is still
+// strictly larger than every other gap, the hopped configuration is its own
+// representative (stabiliser order 1, ell = 0) and its anchored rank is read
+// directly from its sites.  Hops of the anchor and near-ties take the full
+// key search.
+template <int N, class R, bool REALPH, bool NORMLIST, bool COUNT, class KeyT>
 __global__ __launch_bounds__(kThreads) void k_hv_chain(HvArgs a) {
   using C = typename CT<R>::T;
   extern __shared__ __align__(16) unsigned char smem[];
   const int L = a.L;
-  // LDS: binom [N-1][L-1] int64 | rclass [N+1] R | jxy2 [nbt] R | jz4 [nbt] R |
+  // LDS: binom [N-1][L-1] int32 | rclass [N+1] R | jxy2 [nbt] R | jz4 [nbt] R |
   //      adj_ptr [L+2] int | adj [nadj] int
-  int64_t *s_binom = (int64_t *)smem;
   const int nb = (N > 1) ? (N - 1) * (L - 1) : 0;
-  R *s_rcl = (R *)(s_binom + nb);
+  int32_t *s_binom = (int32_t *)smem;
+  R *s_rcl = (R *)(smem + (((size_t)nb * 4 + 15) & ~(size_t)15));
   R *s_jxy2 = s_rcl + (N + 1);
   R *s_jz4 = s_jxy2 + a.nbt;
   int *s_ptr = (int *)(s_jz4 + a.nbt);
   int *s_adj = s_ptr + (L + 2);
   for (int q = threadIdx.x; q < nb; q += kThreads) {
     int m = q / (L - 1) + 1, x = q % (L - 1);
-    s_binom[q] = a.binom[(int64_t)x * a.kdim + m];
+    s_binom[q] = (int32_t)a.binom[(int64_t)x * a.kdim + m];
   }
   if (a.rclass)
     for (int q = threadIdx.x; q <= N; q += kThreads) s_rcl[q] = (R)a.rclass[q];
@@ -126,8 +139,8 @@ __global__ __launch_bounds__(kThreads) void k_hv_chain(HvArgs a) {
   const C *__restrict__ v1 = (const C *)a.v1;
   C *__restrict__ v0 = (C *)a.v0;
   const R scale = a.scale ? (R)(*a.scale) : (R)1;
-  const uint64_t fmask = (1ull << a.kbits) - 1ull;
   const int kb = a.kbits;
+  const KeyT fmask = (KeyT)((1ull << kb) - 1ull);
   double dre = 0.0, dim_ = 0.0;
   unsigned long long nnz = 0;
 
@@ -157,7 +170,7 @@ __global__ __launch_bounds__(kThreads) void k_hv_chain(HvArgs a) {
         for (int q = 0; q < N; ++q) in |= (n[q] == t);
         if (in) continue;
         anti += s_jz4[btp];
-        // hopped configuration: remove n[m], insert t (sorted)
+        // hopped configuration: remove n[m], insert t (sorted, branch-free)
         int r[N > 1 ? N - 1 : 1];
 #pragma unroll
         for (int q = 0; q < N - 1; ++q) r[q] = (q < m) ? n[q] : n[q + 1];
@@ -172,40 +185,55 @@ __global__ __launch_bounds__(kThreads) void k_hv_chain(HvArgs a) {
 #pragma unroll
         for (int q = 0; q < N - 1; ++q) g[q] = nn[q + 1] - nn[q];
         g[N - 1] = nn[0] + L - nn[N - 1];
-        uint64_t K[N];
+        bool fast = false;
+        if (m >= 1 && N > 1) {  // nn[0] = 1 still (site 1 stays occupied)
+          int gm = 0;
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-          uint64_t k = 0;
-#pragma unroll
-          for (int tt = 1; tt < N; ++tt) k = (k << kb) | (uint64_t)g[(j - tt + N) % N];
-          K[j] = k;
+          for (int q = 0; q < N - 1; ++q) gm = g[q] > gm ? g[q] : gm;
+          fast = g[N - 1] > gm;
         }
-        uint64_t kmax = K[0];
+        int32_t c = 0;
+        int count = 1, ei = 0;
+        if (fast) {
 #pragma unroll
-        for (int j = 1; j < N; ++j) kmax = K[j] > kmax ? K[j] : kmax;
-        int count = 0, mx = 0;
+          for (int q = 1; q < N; ++q) c += s_binom[(q - 1) * (L - 1) + (nn[q] - 2)];
+        } else {
+          KeyT K[N];
 #pragma unroll
-        for (int j = 0; j < N; ++j) {
-          const bool tie = (K[j] == kmax);
-          count += tie ? 1 : 0;
-          mx = (tie && nn[j] > mx) ? nn[j] : mx;
-        }
-        // anchored colex rank of the representative
-        int64_t c = 0;
-        if (N > 1) {
-          int f[N > 1 ? N - 1 : 1];
-          int sum = 0;
+          for (int j = 0; j < N; ++j) {
+            KeyT k = 0;
 #pragma unroll
-          for (int tt = 1; tt < N; ++tt) {
-            f[tt - 1] = (int)((kmax >> (kb * (N - 1 - tt))) & fmask);
-            sum += f[tt - 1];
+            for (int tt = 1; tt < N; ++tt) k = (KeyT)((k << kb) | (KeyT)g[(j - tt + N) % N]);
+            K[j] = k;
           }
-          int P = L - sum;  // gap following the anchor
-          c = s_binom[0 * (L - 1) + (P - 1)];
+          KeyT kmax = K[0];
 #pragma unroll
-          for (int mm = 2; mm < N; ++mm) {
-            P += f[N - mm];  // g[j*+mm-1] = field (N - (mm-1))
-            c += s_binom[(mm - 1) * (L - 1) + (P - 1)];
+          for (int j = 1; j < N; ++j) kmax = K[j] > kmax ? K[j] : kmax;
+          int mx = 0;
+          count = 0;
+#pragma unroll
+          for (int j = 0; j < N; ++j) {
+            const bool tie = (K[j] == kmax);
+            count += tie ? 1 : 0;
+            mx = (tie && nn[j] > mx) ? nn[j] : mx;
+          }
+          const bool isrep = (nn[0] == 1) && (K[0] == kmax);
+          ei = isrep ? 0 : (L + 1 - mx) + 3 * (L + 1);
+          if (N > 1) {
+            int f[N > 1 ? N - 1 : 1];
+            int sum = 0;
+#pragma unroll
+            for (int tt = 1; tt < N; ++tt) {
+              f[tt - 1] = (int)((kmax >> (kb * (N - 1 - tt))) & fmask);
+              sum += f[tt - 1];
+            }
+            int P = L - sum;  // gap following the anchor
+            c = s_binom[P - 1];
+#pragma unroll
+            for (int mm = 2; mm < N; ++mm) {
+              P += f[N - mm];  // g[j*+mm-1] = field (N - (mm-1))
+              c += s_binom[(mm - 1) * (L - 1) + (P - 1)];
+            }
           }
         }
         const int id = a.index[c];
@@ -221,9 +249,6 @@ __global__ __launch_bounds__(kThreads) void k_hv_chain(HvArgs a) {
           accx += cf * y.x;
           accy += cf * y.y;
         } else {
-          const bool isrep = (nn[0] == 1) && (K[0] == kmax);
-          const int ell = isrep ? 0 : (L + 1 - mx);
-          const int ei = isrep ? 0 : (ell + 3 * (L + 1));
           const R pr = (R)a.explist[2 * ei] * cf, pi = (R)a.explist[2 * ei + 1] * cf;
           accx += pr * y.x - pi * y.y;
           accy += pr * y.y + pi * y.x;
@@ -380,31 +405,38 @@ __global__ __launch_bounds__(kThreads) void k_hv_general(HvArgs a, int N) {
 // ----------------------------------------------------------------- dispatch
 using KFn = void (*)(HvArgs);
 
-template <int N>
-KFn pick_chain(int prec, bool realph, bool normlist, bool count) {
-  if (count) return k_hv_chain<N, double, true, false, true>;
+template <int N, class KeyT>
+KFn pick_chain_k(int prec, bool realph, bool normlist, bool count) {
+  if (count) return k_hv_chain<N, double, true, false, true, uint64_t>;
   if (prec == QS3X_FP64) {
     if (normlist)
-      return realph ? k_hv_chain<N, double, true, true, false> : k_hv_chain<N, double, false, true, false>;
-    return realph ? k_hv_chain<N, double, true, false, false> : k_hv_chain<N, double, false, false, false>;
+      return realph ? k_hv_chain<N, double, true, true, false, uint64_t>
+                    : k_hv_chain<N, double, false, true, false, uint64_t>;
+    return realph ? k_hv_chain<N, double, true, false, false, KeyT> : k_hv_chain<N, double, false, false, false, KeyT>;
   }
-  return realph ? k_hv_chain<N, float, true, false, false> : k_hv_chain<N, float, false, false, false>;
+  return realph ? k_hv_chain<N, float, true, false, false, KeyT> : k_hv_chain<N, float, false, false, false, KeyT>;
 }
 
-KFn chain_kernel(int N, int prec, bool realph, bool normlist, bool count) {
+template <int N>
+KFn pick_chain(int prec, bool realph, bool normlist, bool count, bool key32) {
+  return key32 ? pick_chain_k<N, uint32_t>(prec, realph, normlist, count)
+               : pick_chain_k<N, uint64_t>(prec, realph, normlist, count);
+}
+
+KFn chain_kernel(int N, int prec, bool realph, bool normlist, bool count, bool key32) {
   switch (N) {
-    case 1: return pick_chain<1>(prec, realph, normlist, count);
-    case 2: return pick_chain<2>(prec, realph, normlist, count);
-    case 3: return pick_chain<3>(prec, realph, normlist, count);
-    case 4: return pick_chain<4>(prec, realph, normlist, count);
-    case 5: return pick_chain<5>(prec, realph, normlist, count);
-    case 6: return pick_chain<6>(prec, realph, normlist, count);
-    case 7: return pick_chain<7>(prec, realph, normlist, count);
-    case 8: return pick_chain<8>(prec, realph, normlist, count);
-    case 9: return pick_chain<9>(prec, realph, normlist, count);
-    case 10: return pick_chain<10>(prec, realph, normlist, count);
-    case 11: return pick_chain<11>(prec, realph, normlist, count);
-    case 12: return pick_chain<12>(prec, realph, normlist, count);
+    case 1: return pick_chain<1>(prec, realph, normlist, count, key32);
+    case 2: return pick_chain<2>(prec, realph, normlist, count, key32);
+    case 3: return pick_chain<3>(prec, realph, normlist, count, key32);
+    case 4: return pick_chain<4>(prec, realph, normlist, count, key32);
+    case 5: return pick_chain<5>(prec, realph, normlist, count, key32);
+    case 6: return pick_chain<6>(prec, realph, normlist, count, key32);
+    case 7: return pick_chain<7>(prec, realph, normlist, count, key32);
+    case 8: return pick_chain<8>(prec, realph, normlist, count, key32);
+    case 9: return pick_chain<9>(prec, realph, normlist, count, key32);
+    case 10: return pick_chain<10>(prec, realph, normlist, count, key32);
+    case 11: return pick_chain<11>(prec, realph, normlist, count, key32);
+    case 12: return pick_chain<12>(prec, realph, normlist, count, key32);
     default: return nullptr;
   }
 }
@@ -460,8 +492,15 @@ static size_t chain_lds_bytes(qs3x_sector *s, int prec) {
   const int N = s->nod, L = s->model->NOS;
   const size_t rs = (prec == QS3X_FP64) ? 8 : 4;
   size_t nb = (N > 1) ? (size_t)(N - 1) * (L - 1) : 0;
-  return nb * 8 + rs * (N + 1) + 2 * rs * s->model->bt_jxy2.size() + 4 * (size_t)(L + 2) +
+  return ((nb * 4 + 15) & ~(size_t)15) + rs * (N + 1) + 2 * rs * s->model->bt_jxy2.size() + 4 * (size_t)(L + 2) +
          4 * s->model->adj.size();
+}
+
+// the N-1 packed gap fields of a key fit in 32 bits
+static bool key32(qs3x_sector *s) {
+  int kb = 1;
+  while ((1 << kb) <= s->model->NOS) ++kb;
+  return (int64_t)(s->nod - 1) * kb <= 32;
 }
 
 static bool use_chain(qs3x_sector *s, int prec) {
@@ -479,7 +518,7 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   if (nblocks_out) *nblocks_out = grid;
   const bool normlist = s->d_rclass.p == nullptr;  // sectors from caller lists carry list_r only
   if (use_chain(s, prec) && !(normlist && prec != QS3X_FP64)) {
-    KFn f = chain_kernel(s->nod, prec, s->real_phase, normlist, false);
+    KFn f = chain_kernel(s->nod, prec, s->real_phase, normlist, false, key32(s));
     if (f) {
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, prec), st, a);
       QS3X_HIP(hipGetLastError());
@@ -510,7 +549,7 @@ int64_t count_nnz(qs3x_sector *s) {
   const int grid = persistent_grid(a.ntiles);
   hipStream_t st = stream();
   const bool normlist = s->d_rclass.p == nullptr;
-  KFn f = use_chain(s, QS3X_FP64) ? chain_kernel(s->nod, QS3X_FP64, true, normlist, true) : nullptr;
+  KFn f = use_chain(s, QS3X_FP64) ? chain_kernel(s->nod, QS3X_FP64, true, normlist, true, false) : nullptr;
   if (f)
     hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, QS3X_FP64), st, a);
   else

@@ -12,7 +12,8 @@
 //
 // Operand layout of v_mfma_f64_16x16x4f64 (64 lanes): A (16 x 4) lane l holds
 // A[l % 16][l / 16]; B (4 x 16) lane l holds B[l / 16][l % 16]; D (16 x 16)
-// lane l holds D[4 (l / 16) + e][l % 16], e = 0..3.
+// lane l holds D[l / 16 + 4 e][l % 16] in its e-th result register, e = 0..3
+// (checked against a host GEMM by tests/test_gpu_solvers.py).
 #include <algorithm>
 
 #include "blas.h"
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_mfma(R *base, int64_t M 
       const int col = t * 16 + li;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int64_t row = row0 + 4 * lk + e;
+        const int64_t row = row0 + lk + 4 * e;
         if (row < M && col < nok) base[(size_t)col * M + row] = (R)acc[t][e];
       }
     }

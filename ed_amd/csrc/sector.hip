@@ -462,7 +462,8 @@ static void upload_common(qs3x_sector *s, int kmax) {
 static bool chain_fast_ok(const qs3x_model *m, int nod) {
   if (!m->chain || nod < 1 || nod > kMaxChainNod) return false;
   int kb = bits_for(m->NOS);
-  return (int64_t)(nod - 1) * kb <= 64;
+  // gap keys fit in 64 bits; anchored ranks (and the dense index) fit in int32
+  return (int64_t)(nod - 1) * kb <= 64 && binom_host(m->NOS - 1, nod - 1) < (int64_t)INT32_MAX;
 }
 
 void build_sector_device(qs3x_sector *s) {

@@ -159,10 +159,14 @@ struct TR {
   DevBuf<unsigned char> basis;  // (NOM+1) x dim, slot j at j*dim
   DevBuf<double> sc;            // [0,NOM): alpha' ; [NOM,2NOM): beta ; [2NOM, 3NOM+1): scales ; tmp
   DevBuf<double> part, h, Cdev;
-  DevBuf<const void *> vptr;
-  std::vector<const void *> hptr;
+  // per-step reorthogonalisation sets: tab[(phase * NOM + i) * (NOM + 1) + j]
+  // (phase 0: first cycle, 1: restart cycles), cnt[phase * NOM + i] entries
+  DevBuf<const void *> ptab;
+  DevBuf<int> stab;
+  std::vector<int> cnt;
   int64_t hv_count = 0;
   int cgs_passes = 2;
+  int reorth = 0;
 
   int A() const { return 0; }
   int Bt() const { return NOM; }
@@ -185,10 +189,36 @@ struct TR {
     part.alloc((size_t)2 * std::max(nb, 1) * (NOM + 2));
     h.alloc((size_t)2 * (NOM + 2));
     Cdev.alloc((size_t)NOM * NOM);
-    hptr.resize(NOM + 1);
-    for (int j = 0; j <= NOM; ++j) hptr[j] = slot(j);
-    vptr.alloc(NOM + 1);
-    vptr.upload(hptr.data(), NOM + 1);
+    cgs_passes = std::max(1, options().tr_cgs_passes);
+    reorth = options().tr_reorth;
+    // Step i writes slot i+1 and orthogonalises it against: every slot 0..i
+    // (reorth 0, the reference's full reorthogonalisation), or the locked Ritz
+    // vectors 0..NOK-1 (restart cycles) plus slots i-1, i (reorth 1).
+    const int W = NOM + 1;
+    std::vector<const void *> pt((size_t)2 * NOM * W, nullptr);
+    std::vector<int> sx((size_t)2 * NOM * W, 0);
+    cnt.assign((size_t)2 * NOM, 0);
+    for (int phase = 0; phase < 2; ++phase)
+      for (int i = 0; i < NOM; ++i) {
+        std::vector<int> set;
+        if (reorth == 0) {
+          for (int j = 0; j <= i; ++j) set.push_back(j);
+        } else {
+          if (phase == 1)
+            for (int j = 0; j < NOK && j <= i; ++j) set.push_back(j);
+          for (int j = std::max(0, i - 1); j <= i; ++j)
+            if (std::find(set.begin(), set.end(), j) == set.end()) set.push_back(j);
+        }
+        cnt[(size_t)phase * NOM + i] = (int)set.size();
+        for (size_t q = 0; q < set.size(); ++q) {
+          pt[((size_t)phase * NOM + i) * W + q] = slot(set[q]);
+          sx[((size_t)phase * NOM + i) * W + q] = set[q];
+        }
+      }
+    ptab.alloc(pt.size());
+    ptab.upload(pt.data(), pt.size());
+    stab.alloc(sx.size());
+    stab.upload(sx.data(), sx.size());
     QS3X_HIP(hipStreamSynchronize(st));
   }
 
@@ -198,15 +228,20 @@ struct TR {
     finalize(part.p, nb, sc.p, 2, TMP(), S() + j, 0.0, st);
   }
 
-  // One Lanczos step from slot i into slot i+1 with CGS2 against slots 0..i.
-  void step(int i) {
+  // One Lanczos step from slot i into slot i+1, then classical Gram-Schmidt
+  // (cgs_passes times, FP64 coefficients) against the step's slot set.
+  void step(int i, int phase) {
     int nb = 0;
     launch_hv(s, prec, slot(i + 1), slot(i), sc.p + S() + i, part.p, &nb, st);
     ++hv_count;
     finalize(part.p, nb, sc.p, 1, A() + i, TMP(), shift, st);
+    const size_t row = (size_t)phase * NOM + i;
+    const int m = cnt[row];
+    const void *const *P = (const void *const *)(ptab.p + row * (NOM + 1));
+    const int *X = stab.p + row * (NOM + 1);
     for (int pass = 0; pass < cgs_passes; ++pass) {
-      vec_mdot(prec, (const void *const *)vptr.p, sc.p + S(), i + 1, slot(i + 1), dim, part.p, h.p, st);
-      nb = vec_maxpy(prec, (const void *const *)vptr.p, sc.p + S(), i + 1, h.p, slot(i + 1), dim, part.p, st);
+      vec_mdot(prec, P, sc.p + S(), X, m, slot(i + 1), dim, part.p, h.p, st);
+      nb = vec_maxpy(prec, P, sc.p + S(), X, m, h.p, slot(i + 1), dim, part.p, st);
     }
     finalize(part.p, nb, sc.p, 2, Bt() + i, S() + i + 1, 0.0, st);
   }
@@ -251,7 +286,7 @@ TRResult tr_cycles(TR &t, int64_t maxitr, double tol, double abs_floor, std::vec
   const int NOK = t.NOK, NOM = t.NOM;
   TRResult r;
   t.set_norm_scale(0);
-  for (int i = 0; i < NOM; ++i) t.step(i);
+  for (int i = 0; i < NOM; ++i) t.step(i, 0);
   std::vector<double> a, b;
   t.read_ab(0, NOM, a, b);
   std::vector<double> T((size_t)NOM * NOM, 0.0), C((size_t)NOM * NOM), w(NOM);
@@ -284,7 +319,7 @@ TRResult tr_cycles(TR &t, int64_t maxitr, double tol, double abs_floor, std::vec
     for (int j = 0; j < NOK; ++j) prev[j] = w[j];
     std::vector<double> cb(NOK);
     for (int j = 0; j < NOK; ++j) cb[j] = beta_last * C[(size_t)(NOM - 1) * NOM + j];
-    for (int i = NOK; i < NOM; ++i) t.step(i);
+    for (int i = NOK; i < NOM; ++i) t.step(i, 1);
     t.read_ab(NOK, NOM, a, b);
     std::fill(T.begin(), T.end(), 0.0);
     for (int j = 0; j < NOK; ++j) {
@@ -334,7 +369,7 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
       t32s.init(s, QS3X_FP32, NOK, NOM);
       load_start(QS3X_FP32, t32s.slot(0), dim, start, seed, st);
       // FP32 phase: stop at the FP32 resolution of the shifted spectrum
-      const double floor32 = 1e-6;
+      const double floor32 = options().fp32_floor;
       TRResult r = tr_cycles(t32s, maxitr, std::max(conv, 1e-30), floor32, theta, resid);
       restarts += r.restarts;
       hv32 = t32s.hv_count;

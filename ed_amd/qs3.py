@@ -42,6 +42,7 @@ def _load():
     L.qs3x_wavevector.argtypes = [_i32, _i32]
     sigs = {
         "qs3x_set_device": [_i32],
+        "qs3x_set_option": [ctypes.c_char_p, _dbl],
         "qs3x_model_create": [_i32, _i32, _i32, _i32, _i32, _P, _P, _P, _P],
         "qs3x_model_destroy": [_P],
         "qs3x_fm_energy": [_P, _P],
@@ -87,6 +88,11 @@ def _p(a):
 
 def _c128(a):
     return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def set_option(name, value):
+    """qs3x_set_option: 'tr_reorth' (0 full, 1 locked + local), 'tr_cgs_passes', 'fp32_floor'."""
+    _check(_lib.qs3x_set_option(name.encode(), float(value)))
 
 
 def wavevector(K, L):

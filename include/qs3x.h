@@ -40,6 +40,16 @@ const char *qs3x_last_error(void);
 /* Device the library runs on (HIP ordinal); default 0. */
 int qs3x_set_device(int32_t device);
 
+/* Tunables (no counterpart in the reference):
+ *   "tr_reorth"     0 (default): the thick restart reorthogonalises every new
+ *                   Lanczos vector against all previous basis vectors, as
+ *                   eigen_solver.f90:233-239 does; 1: against the locked Ritz
+ *                   vectors and the last two Lanczos vectors only.
+ *   "tr_cgs_passes" classical Gram-Schmidt passes per step (default 2).
+ *   "fp32_floor"    Ritz-value change (absolute) that ends the FP32 phase of
+ *                   the mixed thick restart (default 1e-6). */
+int qs3x_set_option(const char *name, double value);
+
 /* ----------------------------------------------------------------- model */
 /* Lattice and XXZ bond list.  Replaces read_ip's bond handling
  * (input_param.f90:58-103: p1 < p2 swap, sJint = (Jxy/2, Jz/4)) and the
@@ -86,8 +96,8 @@ int qs3x_sector_nnz(qs3x_sector *s, int64_t *nnz);
  * Replaces ham_to_vec_wave_vector (ham2vec_v3.f90:37-72).  Stages both
  * vectors over PCIe; solvers below keep everything on the device. */
 int qs3x_ham_to_vec_wave_vector(qs3x_sector *s, double *v0, const double *v1);
-/* Device-pointer H . v on the library stream (hipStream_t passed as void*,
- * NULL = the library's stream).  prec: QS3X_FP64 -> v0, v1 complex double;
+/* Device-pointer H . v on the given stream (hipStream_t passed as void*;
+ * NULL is the HIP null stream).  prec: QS3X_FP64 -> v0, v1 complex double;
  * QS3X_FP32 -> complex float with the e_zz shift (v0 = (H - e_zz) v1). */
 int qs3x_hv_dev(qs3x_sector *s, int32_t prec, void *v0_dev, const void *v1_dev, void *stream);
 

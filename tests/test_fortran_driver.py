@@ -114,10 +114,11 @@ def test_driver_thick_restart_square_lattice(tmp_path):
 
 @pytest.mark.gpu
 def test_driver_lanczos_chain_with_dsf(tmp_path):
-    """ALG=1 on a chain (L=14, N=3, k=0) with S^-(q, w) at q = 2 pi 3/14."""
+    """ALG=1 on a chain (L=12, N=4, k=0; non-degenerate ground state) with
+    S^-(q, w) at q = 2 pi 3/12."""
     print("[gpu] fortran driver ALG=1 + DSF", flush=True)
     d = str(tmp_path)
-    L, nod = 14, 3
+    L, nod = 12, 4
     bonds = orc.chain_bonds(L, [(1, 1.0, 1.0)])
     q = (2 * np.pi * 3 / L, 0.0, 0.0)
     _write_deck(d, L, L, 1, nod, (0, 0), bonds, alg=1, cal_dsf=1, spsmsz=2, itr_dsf=25, q=q, rfield=0.1, nov=1)
