@@ -63,6 +63,27 @@ def cpu_baseline(sec, budget_s=12.0):
                       f"N=4 k=0 sector in {dt:.1f} s, extrapolated to D={sec.dim}"}
 
 
+def first_within(trace, e_ref, rel, prec=None):
+    """(seconds, H.v count) of the first trace row whose Ritz value is within
+    rel of e_ref (optionally only rows of the given precision), or None."""
+    for t, hv, e, p in trace:
+        if (prec is None or int(p) == prec) and abs(e / e_ref - 1.0) < rel:
+            return t, int(hv)
+    return None
+
+
+def calibrate_copy(nbytes=1 << 30, reps=5):
+    """A plain device copy of nbytes, for scaling the FETCH_SIZE / WRITE_SIZE
+    counters (run under rocprofv3 --pmc)."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "hv_c3_fp32_traffic.json")
     if os.path.exists(p):
@@ -79,12 +100,22 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--NOK", type=int, default=4)
-    ap.add_argument("--NOM", type=int, default=40)
+    ap.add_argument("--NOM", type=int, default=60)
+    ap.add_argument("--reorth", type=int, default=1, help="thick-restart reorthogonalisation: 0 full, 1 locked+local")
+    ap.add_argument("--cgs", type=int, default=1, help="Gram-Schmidt passes per thick-restart step")
+    ap.add_argument("--fp32-floor", type=float, default=1e-7)
+    ap.add_argument("--conv-plain", type=float, default=1e-11)
+    ap.add_argument("--conv-mixed", type=float, default=1e-10)
+    ap.add_argument("--calibrate-copy", action="store_true", help="only run a 1 GiB device copy (counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X; multi-GPU runs are not part of this build "
               "(see DESIGN.md for how the H.v would shard).", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     import torch
     from ed_amd import qs3 as q3
@@ -130,25 +161,45 @@ def main():
 
     solve = {}
     if not args.no_solve:
+        q3.set_option("verbose", 1)  # one stderr line per restart / 100 Lanczos steps
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        E64, _, it64 = q3.lanczos_routines(sec, maxitr=5000, minitr=20, itrint=5, lnc_ene_conv=1e-12, prec=q3.FP64,
-                                           seed=1, want_vector=False)
+        E64, _, it64 = q3.lanczos_routines(sec, maxitr=5000, minitr=20, itrint=5, lnc_ene_conv=args.conv_plain,
+                                           prec=q3.FP64, seed=1, want_vector=False)
         torch.cuda.synchronize()
         t64 = time.perf_counter() - t0
+        tr64 = q3.solver_trace()
+        hit64 = first_within(tr64, E0_EXACT, 1e-9)
         log(f"plain FP64 Lanczos: E0={E64:.15f} in {t64:.2f} s, {it64} steps, rel err {abs(E64 / E0_EXACT - 1):.2e}")
+        q3.set_option("tr_reorth", args.reorth)
+        q3.set_option("tr_cgs_passes", args.cgs)
+        q3.set_option("fp32_floor", args.fp32_floor)
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         ene, _, st = q3.my_trlanczos_routines_complex(sec, NOE=1, NOK=args.NOK, NOM=args.NOM, maxitr=400,
-                                                       lnc_ene_conv=1e-12, prec=q3.FP32, seed=1, want_vectors=False)
+                                                       lnc_ene_conv=args.conv_mixed, prec=q3.FP32, seed=1,
+                                                       want_vectors=False)
         torch.cuda.synchronize()
         t32 = time.perf_counter() - t0
+        tr32 = q3.solver_trace()
+        hit32_any = first_within(tr32, E0_EXACT, 1e-9)
+        hit32_ref = first_within(tr32, E0_EXACT, 1e-9, prec=64)
         E32 = float(ene[0])
         log(f"FP32-mixed thick restart: E0={E32:.15f} in {t32:.2f} s, stats {st}, rel err {abs(E32 / E0_EXACT - 1):.2e}")
         solve = {"time_to_E0_fp64_plain_s": t64, "E0_fp64_plain": E64, "iters_fp64_plain": it64,
                  "rel_err_fp64_plain": abs(E64 / E0_EXACT - 1),
                  "time_to_E0_fp32_mixed_s": t32, "E0_fp32_mixed": E32,
                  "rel_err_fp32_mixed": abs(E32 / E0_EXACT - 1), "fp32_mixed_stats": st,
-                 "speedup_fp32_mixed_vs_fp64_plain": t64 / t32 if t32 > 0 else None}
+                 "fp32_mixed_settings": {"NOK": args.NOK, "NOM": args.NOM, "tr_reorth": args.reorth,
+                                         "cgs_passes": args.cgs, "fp32_floor": args.fp32_floor,
+                                         "conv": args.conv_mixed},
+                 "fp64_plain_settings": {"conv": args.conv_plain, "itrint": 5, "minitr": 20},
+                 "speedup_fp32_mixed_vs_fp64_plain": t64 / t32 if t32 > 0 else None,
+                 # first Ritz value within 1e-9 of the Bethe-ansatz E0 (solver trace; seconds, H.v)
+                 "time_to_1e-9_fp64_plain": hit64,
+                 "time_to_1e-9_fp32_mixed_any": hit32_any,
+                 "time_to_1e-9_fp32_mixed_fp64_value": hit32_ref}
+        log(f"first within 1e-9: plain FP64 {hit64}, mixed (any) {hit32_any}, mixed (FP64 value) {hit32_ref}")
 
     cpu = None
     if not args.no_cpu:
@@ -163,7 +214,7 @@ def main():
     roof = {"bound": "hbm", "achieved": f32["alg_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": f32["alg_GBs"] / HBM_PEAK_GBS,
             "traffic": (traffic or {}).get("bytes_per_launch"),
-            "alg_bytes_per_launch": f32["alg_bytes"], "kernel": "k_hv_chain<4, float, real-phase>",
+            "alg_bytes_per_launch": f32["alg_bytes"], "kernel": "k_hv_chain_mlp<4, 2, float, real-phase, uint32>",
             "avg_launch_ms": f32["ms"]}
     out = {
         "metric": METRIC,

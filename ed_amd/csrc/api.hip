@@ -80,6 +80,12 @@ int qs3x_set_option(const char *name, double value) {
     } else if (n == "tr_cgs_passes") {
       QS3X_REQUIRE(value >= 1 && value <= 3, "tr_cgs_passes must be 1..3");
       options().tr_cgs_passes = (int)value;
+    } else if (n == "verbose") {
+      QS3X_REQUIRE(value >= 0 && value <= 2, "verbose must be 0..2");
+      options().verbose = (int)value;
+    } else if (n == "hv_batched") {
+      QS3X_REQUIRE(value == 0 || value == 1, "hv_batched must be 0 or 1");
+      options().hv_batched = (int)value;
     } else if (n == "fp32_floor") {
       QS3X_REQUIRE(value > 0, "fp32_floor must be > 0");
       options().fp32_floor = value;

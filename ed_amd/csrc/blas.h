@@ -33,6 +33,8 @@ struct Options {
   int tr_reorth = 0;          // 0: every previous basis vector; 1: locked Ritz vectors + last two
   int tr_cgs_passes = 2;      // classical Gram-Schmidt passes per step
   double fp32_floor = 1e-6;   // |delta theta| that ends the FP32 phase of the mixed thick restart
+  int hv_batched = 1;         // chain H.v: 1 = batched-load kernel where available, 0 = per-hop loads
+  int verbose = 0;            // 1: one stderr line per restart / per 100 Lanczos steps
 };
 Options &options();
 

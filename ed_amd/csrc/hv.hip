@@ -25,6 +25,7 @@
 // read) -- replacing findstate's binary search over list_s.
 #include <algorithm>
 
+#include "blas.h"
 #include "qs3x_internal.h"
 
 namespace qs3x {
@@ -275,6 +276,193 @@ __global__ __launch_bounds__(kThreads) void k_hv_chain(HvArgs a) {
   }
 }
 
+// ------------------------------------------------ chain kernel, batched loads
+// Same arithmetic as k_hv_chain, organised for memory-level parallelism: the
+// up to N * DEG hops of a row are first reduced to (anchored rank, coefficient,
+// phase index), then every index read is issued, then every gather, then the
+// row is accumulated.  A row's dependent loads are two rounds instead of
+// 2 * hops.  DEG is the largest bond-adjacency degree of any site.
+template <int N, int DEG, class R, bool REALPH, class KeyT>
+__global__ __launch_bounds__(kThreads) void k_hv_chain_mlp(HvArgs a) {
+  using C = typename CT<R>::T;
+  constexpr int H = N * DEG;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int L = a.L;
+  const int nb = (N > 1) ? (N - 1) * (L - 1) : 0;
+  int32_t *s_binom = (int32_t *)smem;
+  R *s_rcl = (R *)(smem + (((size_t)nb * 4 + 15) & ~(size_t)15));
+  R *s_jxy2 = s_rcl + (N + 1);
+  R *s_jz4 = s_jxy2 + a.nbt;
+  int *s_ptr = (int *)(s_jz4 + a.nbt);
+  int *s_adj = s_ptr + (L + 2);
+  for (int q = threadIdx.x; q < nb; q += kThreads) {
+    int m = q / (L - 1) + 1, x = q % (L - 1);
+    s_binom[q] = (int32_t)a.binom[(int64_t)x * a.kdim + m];
+  }
+  for (int q = threadIdx.x; q <= N; q += kThreads) s_rcl[q] = (R)a.rclass[q];
+  for (int q = threadIdx.x; q < a.nbt; q += kThreads) {
+    s_jxy2[q] = (R)a.bt[2 * q];
+    s_jz4[q] = (R)a.bt[2 * q + 1];
+  }
+  for (int q = threadIdx.x; q < L + 2; q += kThreads) s_ptr[q] = a.adj_ptr[q];
+  for (int q = threadIdx.x; q < a.nadj; q += kThreads) s_adj[q] = a.adj[q];
+  __syncthreads();
+
+  const C *__restrict__ v1 = (const C *)a.v1;
+  C *__restrict__ v0 = (C *)a.v0;
+  const R scale = a.scale ? (R)(*a.scale) : (R)1;
+  const int kb = a.kbits;
+  const KeyT fmask = (KeyT)((1ull << kb) - 1ull);
+  double dre = 0.0, dim_ = 0.0;
+
+  for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    const int64_t row = tile * kThreads + threadIdx.x;
+    if (row >= a.dim) continue;
+    const uint64_t w = a.cfg[row];
+    const C xin = v1[row];
+    int n[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) n[q] = (int)((w >> (q * a.bits)) & ((1ull << a.bits) - 1ull));
+    const R rinv = (R)1 / s_rcl[(int)(w >> 56)];
+    R anti = 0;
+    int32_t hc[H];
+    R hcf[H];
+    int hei[REALPH ? 1 : H];
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      hc[h] = -1;
+      hcf[h] = 0;
+      if constexpr (!REALPH) hei[h] = 0;
+    }
+    // phase 1: hop -> (anchored rank, coefficient, phase index)
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+      const int s = n[m];
+      const int e0 = s_ptr[s], deg = s_ptr[s + 1] - e0;
+#pragma unroll
+      for (int e = 0; e < DEG; ++e) {
+        if (e >= deg) break;
+        const int ent = s_adj[e0 + e];
+        const int t = ent & 0xffff, btp = ent >> 16;
+        bool in = false;
+#pragma unroll
+        for (int q = 0; q < N; ++q) in |= (n[q] == t);
+        if (in) continue;
+        anti += s_jz4[btp];
+        int r[N > 1 ? N - 1 : 1];
+#pragma unroll
+        for (int q = 0; q < N - 1; ++q) r[q] = (q < m) ? n[q] : n[q + 1];
+        int nn[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+          const bool ltq = (q < N - 1) ? (r[q < N - 1 ? q : 0] < t) : false;
+          const bool ltp = (q > 0) ? (r[q > 0 ? q - 1 : 0] < t) : true;
+          nn[q] = ltq ? r[q < N - 1 ? q : 0] : (ltp ? t : r[q > 0 ? q - 1 : 0]);
+        }
+        int g[N];
+#pragma unroll
+        for (int q = 0; q < N - 1; ++q) g[q] = nn[q + 1] - nn[q];
+        g[N - 1] = nn[0] + L - nn[N - 1];
+        bool fast = false;
+        if (m >= 1 && N > 1) {
+          int gm = 0;
+#pragma unroll
+          for (int q = 0; q < N - 1; ++q) gm = g[q] > gm ? g[q] : gm;
+          fast = g[N - 1] > gm;
+        }
+        int32_t c = 0;
+        int count = 1, ei = 0;
+        if (fast) {
+#pragma unroll
+          for (int q = 1; q < N; ++q) c += s_binom[(q - 1) * (L - 1) + (nn[q] - 2)];
+        } else {
+          KeyT K[N];
+#pragma unroll
+          for (int j = 0; j < N; ++j) {
+            KeyT k = 0;
+#pragma unroll
+            for (int tt = 1; tt < N; ++tt) k = (KeyT)((k << kb) | (KeyT)g[(j - tt + N) % N]);
+            K[j] = k;
+          }
+          KeyT kmax = K[0];
+#pragma unroll
+          for (int j = 1; j < N; ++j) kmax = K[j] > kmax ? K[j] : kmax;
+          int mx = 0;
+          count = 0;
+#pragma unroll
+          for (int j = 0; j < N; ++j) {
+            const bool tie = (K[j] == kmax);
+            count += tie ? 1 : 0;
+            mx = (tie && nn[j] > mx) ? nn[j] : mx;
+          }
+          const bool isrep = (nn[0] == 1) && (K[0] == kmax);
+          ei = isrep ? 0 : (L + 1 - mx) + 3 * (L + 1);
+          if (N > 1) {
+            int f[N > 1 ? N - 1 : 1];
+            int sum = 0;
+#pragma unroll
+            for (int tt = 1; tt < N; ++tt) {
+              f[tt - 1] = (int)((kmax >> (kb * (N - 1 - tt))) & fmask);
+              sum += f[tt - 1];
+            }
+            int P = L - sum;
+            c = s_binom[P - 1];
+#pragma unroll
+            for (int mm = 2; mm < N; ++mm) {
+              P += f[N - mm];
+              c += s_binom[(mm - 1) * (L - 1) + (P - 1)];
+            }
+          }
+        }
+        hc[m * DEG + e] = c;
+        hcf[m * DEG + e] = s_jxy2[btp] * s_rcl[count] * rinv;
+        if constexpr (!REALPH) hei[m * DEG + e] = ei;
+      }
+    }
+    // phase 2: every index read
+    int hid[H];
+#pragma unroll
+    for (int h = 0; h < H; ++h) hid[h] = (hc[h] >= 0) ? a.index[hc[h]] : -1;
+    // phase 3: every gather
+    C y[H];
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      if (hid[h] >= 0) y[h] = v1[hid[h]];
+      else {
+        y[h].x = 0;
+        y[h].y = 0;
+      }
+    }
+    // phase 4: accumulate
+    R accx = 0, accy = 0;
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      const R cf = (hid[h] >= 0) ? hcf[h] : (R)0;
+      if constexpr (REALPH) {
+        accx += cf * y[h].x;
+        accy += cf * y[h].y;
+      } else {
+        const R pr = (R)a.explist[2 * hei[h]] * cf, pi = (R)a.explist[2 * hei[h] + 1] * cf;
+        accx += pr * y[h].x - pi * y[h].y;
+        accy += pr * y[h].y + pi * y[h].x;
+      }
+    }
+    C x = xin;
+    x.x *= scale;
+    x.y *= scale;
+    const R dg = (sizeof(R) == 8 ? (R)a.ezz : (R)0) - (R)2 * anti;
+    C out;
+    out.x = dg * x.x + accx * scale;
+    out.y = dg * x.y + accy * scale;
+    v0[row] = out;
+    if (a.dot_partials) {
+      dre += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
+      dim_ += (double)x.x * (double)out.y - (double)x.y * (double)out.x;
+    }
+  }
+  if (a.dot_partials) block_reduce_store(dre, dim_, a.dot_partials);
+}
+
 // ---------------------------------------------------------- general kernel
 // Any LX x LY x LZ lattice: the representative is found by scanning the
 // translation group in the reference's loop order (k, j, i) with a running
@@ -441,6 +629,36 @@ KFn chain_kernel(int N, int prec, bool realph, bool normlist, bool count, bool k
   }
 }
 
+template <int N, int DEG, class KeyT>
+KFn pick_mlp3(int prec, bool realph) {
+  if (prec == QS3X_FP64)
+    return realph ? k_hv_chain_mlp<N, DEG, double, true, KeyT> : k_hv_chain_mlp<N, DEG, double, false, KeyT>;
+  return realph ? k_hv_chain_mlp<N, DEG, float, true, KeyT> : k_hv_chain_mlp<N, DEG, float, false, KeyT>;
+}
+
+template <int N>
+KFn pick_mlp2(int prec, bool realph, int deg, bool key32) {
+  if (deg <= 2)
+    return key32 ? pick_mlp3<N, 2, uint32_t>(prec, realph) : pick_mlp3<N, 2, uint64_t>(prec, realph);
+  return key32 ? pick_mlp3<N, 4, uint32_t>(prec, realph) : pick_mlp3<N, 4, uint64_t>(prec, realph);
+}
+
+// Batched-load chain kernels exist for N <= 8 and adjacency degree <= 4.
+KFn chain_mlp_kernel(int N, int prec, bool realph, int deg, bool key32) {
+  if (deg > 4) return nullptr;
+  switch (N) {
+    case 1: return pick_mlp2<1>(prec, realph, deg, key32);
+    case 2: return pick_mlp2<2>(prec, realph, deg, key32);
+    case 3: return pick_mlp2<3>(prec, realph, deg, key32);
+    case 4: return pick_mlp2<4>(prec, realph, deg, key32);
+    case 5: return pick_mlp2<5>(prec, realph, deg, key32);
+    case 6: return pick_mlp2<6>(prec, realph, deg, key32);
+    case 7: return pick_mlp2<7>(prec, realph, deg, key32);
+    case 8: return pick_mlp2<8>(prec, realph, deg, key32);
+    default: return nullptr;
+  }
+}
+
 int persistent_grid(int64_t ntiles) {
   static int cus = 0;
   if (!cus) {
@@ -517,6 +735,14 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   const int grid = persistent_grid(a.ntiles);
   if (nblocks_out) *nblocks_out = grid;
   const bool normlist = s->d_rclass.p == nullptr;  // sectors from caller lists carry list_r only
+  if (use_chain(s, prec) && !normlist && options().hv_batched) {
+    KFn f = chain_mlp_kernel(s->nod, prec, s->real_phase, s->model->max_deg, key32(s));
+    if (f) {
+      hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, prec), st, a);
+      QS3X_HIP(hipGetLastError());
+      return;
+    }
+  }
   if (use_chain(s, prec) && !(normlist && prec != QS3X_FP64)) {
     KFn f = chain_kernel(s->nod, prec, s->real_phase, normlist, false, key32(s));
     if (f) {

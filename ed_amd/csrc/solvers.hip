@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -49,6 +50,20 @@ double now_s() {
 
 size_t elem_bytes(int prec) { return prec == QS3X_FP64 ? 16 : 8; }
 
+// Convergence trace of the last solve: one row per convergence check (plain
+// Lanczos) or restart (thick restart): [seconds since the solve started,
+// H.v applications so far, lowest Ritz value (FP64, unshifted), precision].
+struct TracePoint {
+  double t, hv, e, prec;
+};
+std::vector<TracePoint> &trace() {
+  static std::vector<TracePoint> tr;
+  return tr;
+}
+void trace_point(double t0, int64_t hv, double e, int prec) {
+  trace().push_back({now_s() - t0, (double)hv, e, prec == QS3X_FP64 ? 64.0 : 32.0});
+}
+
 // Loads a start vector (host complex double) or generates the counter-based one.
 void load_start(int prec, void *dst, int64_t dim, const double *start, uint64_t seed, hipStream_t st) {
   if (start) {
@@ -64,6 +79,21 @@ void load_start(int prec, void *dst, int64_t dim, const double *start, uint64_t 
 }  // namespace
 
 extern "C" {
+
+int qs3x_solver_trace(double *out, int64_t cap, int64_t *n) {
+  return guarded2([&] {
+    QS3X_REQUIRE(n, "qs3x_solver_trace: NULL count");
+    const auto &tr = trace();
+    *n = (int64_t)tr.size();
+    if (out)
+      for (int64_t i = 0; i < std::min<int64_t>(cap, (int64_t)tr.size()); ++i) {
+        out[4 * i] = tr[i].t;
+        out[4 * i + 1] = tr[i].hv;
+        out[4 * i + 2] = tr[i].e;
+        out[4 * i + 3] = tr[i].prec;
+      }
+  });
+}
 
 // ------------------------------------------------------------ plain Lanczos
 // lancz_eigen_val (lanczos.f90:50-152).
@@ -85,6 +115,8 @@ int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, i
     DevBuf<double> sc((size_t)4 * mitr + 4);
     DevBuf<double> part((size_t)2 * 4096);
     const double shift = (prec == QS3X_FP32) ? s->model->ezz : 0.0;
+    trace().clear();
+    const double t_start = now_s();
     load_start(prec, U[0].p, dim, start, seed, st);
     int nb = vec_dot(prec, U[0].p, prec, U[0].p, dim, part.p, st);
     finalize(part.p, nb, sc.p, 2, (int)(SCALE + mitr + 1), (int)SCALE, 0.0, st);  // scale[0] = 1/|u0|
@@ -114,6 +146,9 @@ int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, i
         const double cv = std::fabs(tmp / E - 1.0);
         const bool breakdown = !(hb[ite - 1] > 1e-13 * std::fabs(E) + 1e-300);
         if (ite == maxitr || cv < conv || breakdown) done = true;
+        trace_point(t_start, ite, E, prec);
+        if (options().verbose && (done || options().verbose > 1 || ite % 100 == 0))
+          std::fprintf(stderr, "[qs3x] lanczos step %lld E=%.15g change %.3e\n", (long long)ite, E, cv);
         tmp = E;
       }
     }
@@ -165,6 +200,8 @@ struct TR {
   DevBuf<int> stab;
   std::vector<int> cnt;
   int64_t hv_count = 0;
+  int64_t hv_base = 0;   // H.v applications of earlier phases (for the trace)
+  double t_origin = 0;   // start of the solve (for the trace)
   int cgs_passes = 2;
   int reorth = 0;
 
@@ -175,7 +212,8 @@ struct TR {
 
   void *slot(int j) { return basis.p + (size_t)j * dim * elem_bytes(prec); }
 
-  void init(qs3x_sector *s_, int prec_, int nok, int nom) {
+  // reorth_ / cgs_ < 0: take tr_reorth / tr_cgs_passes from the options
+  void init(qs3x_sector *s_, int prec_, int nok, int nom, int reorth_ = -1, int cgs_ = -1) {
     s = s_;
     prec = prec_;
     NOK = nok;
@@ -189,8 +227,9 @@ struct TR {
     part.alloc((size_t)2 * std::max(nb, 1) * (NOM + 2));
     h.alloc((size_t)2 * (NOM + 2));
     Cdev.alloc((size_t)NOM * NOM);
-    cgs_passes = std::max(1, options().tr_cgs_passes);
-    reorth = options().tr_reorth;
+    cgs_passes = std::max(1, cgs_ < 0 ? options().tr_cgs_passes : cgs_);
+    reorth = reorth_ < 0 ? options().tr_reorth : reorth_;
+    t_origin = now_s();
     // Step i writes slot i+1 and orthogonalises it against: every slot 0..i
     // (reorth 0, the reference's full reorthogonalisation), or the locked Ritz
     // vectors 0..NOK-1 (restart cycles) plus slots i-1, i (reorth 1).
@@ -281,9 +320,13 @@ struct TRResult {
   double max_change = 0.0;
 };
 
+// abs_floor > 0 (FP32 phase): converged when the nwant lowest Ritz values
+// moved less than abs_floor (or tol relative) over two consecutive restarts;
+// abs_floor = 0: the reference's test over all NOK values (eigen_solver.f90:150-162).
 TRResult tr_cycles(TR &t, int64_t maxitr, double tol, double abs_floor, std::vector<double> &theta,
-                   std::vector<double> &resid) {
+                   std::vector<double> &resid, int nwant = 0) {
   const int NOK = t.NOK, NOM = t.NOM;
+  const int ncheck = (abs_floor > 0 && nwant > 0) ? std::min(nwant, NOK) : NOK;
   TRResult r;
   t.set_norm_scale(0);
   for (int i = 0; i < NOM; ++i) t.step(i, 0);
@@ -302,13 +345,18 @@ TRResult tr_cycles(TR &t, int64_t maxitr, double tol, double abs_floor, std::vec
   for (tritr = 1; tritr <= maxitr; ++tritr) {
     t.rotate(C);
     double mc = 0.0;
-    for (int j = 0; j < NOK; ++j) {
+    for (int j = 0; j < ncheck; ++j) {
       const double e = w[j] + t.shift, pe = prev[j] + t.shift;
       double ch = std::fabs(pe / e - 1.0);
       if (abs_floor > 0) ch = std::min(ch, std::fabs(pe - e) / abs_floor * tol);
       mc = std::max(mc, ch);
     }
     r.max_change = mc;
+    trace_point(t.t_origin, t.hv_base + t.hv_count, w[0] + t.shift, t.prec);
+    if (options().verbose)
+      std::fprintf(stderr, "[qs3x] thick restart %lld (%s): theta0=%.15g change %.3e, %lld H.v\n",
+                   (long long)tritr, t.prec == QS3X_FP32 ? "fp32" : "fp64", w[0] + t.shift, mc,
+                   (long long)t.hv_count);
     flag = (mc < tol) ? flag + 1 : 0;
     if (flag == 2) {
       r.converged = true;
@@ -341,6 +389,131 @@ TRResult tr_cycles(TR &t, int64_t maxitr, double tol, double abs_floor, std::vec
   return r;
 }
 
+// FP64 Rayleigh-Ritz on k vectors: slots 0..k-1 of t (FP64) hold the
+// candidate vectors (any scale).  They are orthonormalised (CGS2), W = H Y is
+// formed in slots k..2k-1, G = Y^H W is diagonalised on the host (as the real
+// symmetric 2k x 2k form of the Hermitian matrix), and for the nwant lowest
+// pairs the residual norms |W c - theta Y c| are computed.  On return slot
+// 2k + j (j < nwant) holds the refined Ritz vector y_j = Y c_j (scale 1).
+// Requires 2k + nwant <= NOM + 1.
+struct RRResult {
+  std::vector<double> theta, resid;
+};
+
+RRResult rayleigh_ritz(TR &t, int k, int nwant) {
+  hipStream_t st = t.st;
+  const int64_t dim = t.dim;
+  QS3X_REQUIRE(2 * k + nwant <= t.NOM + 1, "rayleigh_ritz: basis too small");
+  std::vector<const void *> hp(2 * k + 1);
+  for (int j = 0; j < 2 * k; ++j) hp[j] = t.slot(j);
+  DevBuf<const void *> P(2 * k);
+  P.upload(hp.data(), 2 * k);
+  std::vector<int> hs(2 * k);
+  for (int j = 0; j < 2 * k; ++j) hs[j] = j;
+  DevBuf<int> X(2 * k);
+  X.upload(hs.data(), 2 * k);
+  const void *const *PY = (const void *const *)P.p;
+  const void *const *PW = (const void *const *)(P.p + k);
+  // orthonormalise Y (CGS2), scales in sc[S()+j]
+  int nb = 0;
+  for (int j = 0; j < k; ++j) {
+    if (j > 0)
+      for (int pass = 0; pass < 2; ++pass) {
+        vec_mdot(QS3X_FP64, PY, t.sc.p + t.S(), X.p, j, t.slot(j), dim, t.part.p, t.h.p, st);
+        vec_maxpy(QS3X_FP64, PY, t.sc.p + t.S(), X.p, j, t.h.p, t.slot(j), dim, nullptr, st);
+      }
+    std::vector<double> one(1, 1.0);
+    QS3X_HIP(hipMemcpyAsync(t.sc.p + t.S() + j, one.data(), sizeof(double), hipMemcpyHostToDevice, st));
+    nb = vec_dot(QS3X_FP64, t.slot(j), QS3X_FP64, t.slot(j), dim, t.part.p, st);
+    finalize(t.part.p, nb, t.sc.p, 2, t.TMP(), t.S() + j, 0.0, st);
+    QS3X_HIP(hipStreamSynchronize(st));
+  }
+  // W_j = H y_j (y_j = s_j Y_j), W scale 1
+  for (int j = 0; j < k; ++j) {
+    launch_hv(t.s, QS3X_FP64, t.slot(k + j), t.slot(j), t.sc.p + t.S() + j, nullptr, nullptr, st);
+    ++t.hv_count;
+  }
+  {
+    std::vector<double> ones(k, 1.0);
+    QS3X_HIP(hipMemcpyAsync(t.sc.p + t.S() + k, ones.data(), sizeof(double) * k, hipMemcpyHostToDevice, st));
+  }
+  // G = Y^H W (complex k x k), column j from one multi-dot
+  std::vector<double> G((size_t)2 * k * k), hcol(2 * k);
+  for (int j = 0; j < k; ++j) {
+    vec_mdot(QS3X_FP64, PY, t.sc.p + t.S(), X.p, k, t.slot(k + j), dim, t.part.p, t.h.p, st);
+    QS3X_HIP(hipMemcpyAsync(hcol.data(), t.h.p, sizeof(double) * 2 * k, hipMemcpyDeviceToHost, st));
+    QS3X_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < k; ++i) {
+      G[2 * ((size_t)i * k + j)] = hcol[2 * i];
+      G[2 * ((size_t)i * k + j) + 1] = hcol[2 * i + 1];
+    }
+  }
+  // Hermitian part as the real symmetric [[Re, -Im], [Im, Re]]
+  const int n2 = 2 * k;
+  std::vector<double> A((size_t)n2 * n2), wv(n2), V((size_t)n2 * n2);
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < k; ++j) {
+      const double re = 0.5 * (G[2 * ((size_t)i * k + j)] + G[2 * ((size_t)j * k + i)]);
+      const double im = 0.5 * (G[2 * ((size_t)i * k + j) + 1] - G[2 * ((size_t)j * k + i) + 1]);
+      A[(size_t)i * n2 + j] = re;
+      A[(size_t)(i + k) * n2 + (j + k)] = re;
+      A[(size_t)i * n2 + (j + k)] = -im;
+      A[(size_t)(i + k) * n2 + j] = im;
+    }
+  sym_eig(n2, A.data(), wv.data(), V.data());
+  RRResult r;
+  r.theta.resize(k);
+  std::vector<std::vector<double>> cvec(k, std::vector<double>(2 * k));
+  for (int j = 0; j < k; ++j) {
+    // eigenvalues come in equal pairs; column 2j gives c = u + i v
+    r.theta[j] = wv[2 * j];
+    double nrm = 0.0;
+    for (int i = 0; i < k; ++i) {
+      const double u = V[(size_t)i * n2 + 2 * j], v = V[(size_t)(i + k) * n2 + 2 * j];
+      cvec[j][2 * i] = u;
+      cvec[j][2 * i + 1] = v;
+      nrm += u * u + v * v;
+    }
+    nrm = std::sqrt(nrm);
+    for (double &x : cvec[j]) x /= nrm;
+  }
+  // residuals and Ritz vectors of the nwant lowest pairs
+  r.resid.assign(nwant, 0.0);
+  const void *const *PYW = (const void *const *)P.p;  // Y then W
+  DevBuf<double> hh((size_t)4 * k);
+  for (int j = 0; j < nwant; ++j) {
+    void *z = t.slot(2 * k + j);
+    // z = sum_i c_ij W_i - theta_j sum_i c_ij Y_i  (maxpy: z -= sum h_i s_i u_i)
+    std::vector<double> h((size_t)4 * k);
+    for (int i = 0; i < k; ++i) {
+      h[2 * i] = r.theta[j] * cvec[j][2 * i];
+      h[2 * i + 1] = r.theta[j] * cvec[j][2 * i + 1];
+      h[2 * (k + i)] = -cvec[j][2 * i];
+      h[2 * (k + i) + 1] = -cvec[j][2 * i + 1];
+    }
+    hh.upload(h.data(), h.size());
+    QS3X_HIP(hipMemsetAsync(z, 0, 16 * dim, st));
+    nb = vec_maxpy(QS3X_FP64, PYW, t.sc.p + t.S(), X.p, 2 * k, hh.p, z, dim, t.part.p, st);
+    finalize(t.part.p, nb, t.sc.p, 3, t.TMP(), -1, 0.0, st);
+    std::vector<double> sc1(1);
+    QS3X_HIP(hipMemcpyAsync(sc1.data(), t.sc.p + t.TMP(), sizeof(double), hipMemcpyDeviceToHost, st));
+    QS3X_HIP(hipStreamSynchronize(st));
+    r.resid[j] = sc1[0];
+    // y_j = sum_i c_ij Y_i
+    for (int i = 0; i < k; ++i) {
+      h[2 * i] = -cvec[j][2 * i];
+      h[2 * i + 1] = -cvec[j][2 * i + 1];
+    }
+    hh.upload(h.data(), 2 * k);
+    QS3X_HIP(hipMemsetAsync(z, 0, 16 * dim, st));
+    vec_maxpy(QS3X_FP64, PY, t.sc.p + t.S(), X.p, k, hh.p, z, dim, nullptr, st);
+    std::vector<double> one(1, 1.0);
+    QS3X_HIP(hipMemcpyAsync(t.sc.p + t.S() + 2 * k + j, one.data(), sizeof(double), hipMemcpyHostToDevice, st));
+    QS3X_HIP(hipStreamSynchronize(st));
+  }
+  return r;
+}
+
 }  // namespace
 
 extern "C" {
@@ -364,39 +537,77 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
     int restarts = 0;
     std::vector<double> startv;  // FP64 start for the refinement (host not needed: device copy)
     TR t64;
+    bool refined = false;
+    int rr_j0 = 0;
+    trace().clear();
     if (prec == QS3X_FP32) {
       TR t32s;
       t32s.init(s, QS3X_FP32, NOK, NOM);
+      t32s.t_origin = t0;
       load_start(QS3X_FP32, t32s.slot(0), dim, start, seed, st);
-      // FP32 phase: stop at the FP32 resolution of the shifted spectrum
+      // FP32 phase: until the NOE lowest Ritz values stop moving at the FP32
+      // resolution of the shifted spectrum
       const double floor32 = options().fp32_floor;
-      TRResult r = tr_cycles(t32s, maxitr, std::max(conv, 1e-30), floor32, theta, resid);
+      TRResult r = tr_cycles(t32s, maxitr, std::max(conv, 1e-30), floor32, theta, resid, NOE);
       restarts += r.restarts;
       hv32 = t32s.hv_count;
       QS3X_HIP(hipStreamSynchronize(st));
       t32 = now_s() - t0;
-      // FP64 refinement from the FP32 Ritz vectors: start vector = their sum
-      t64.init(s, QS3X_FP64, NOK, NOM);
-      DevBuf<double> acc((size_t)2 * dim);
-      QS3X_HIP(hipMemsetAsync(acc.p, 0, 16 * dim, st));
-      for (int j = 0; j < NOE; ++j) vec_axpy_acc(QS3X_FP32, acc.p, t32s.slot(j), dim, 1.0, t32s.sc.p, t32s.S() + j, st);
-      vec_copy_scale(QS3X_FP64, t64.slot(0), QS3X_FP64, acc.p, dim, nullptr, -1, 1.0, st);
+      // FP64 Rayleigh-Ritz on the NOK FP32 Ritz vectors.  The FP64 stage
+      // always reorthogonalises fully (CGS2): started next to an eigenvector,
+      // local orthogonalisation loses the projection property in one cycle.
+      t64.init(s, QS3X_FP64, NOK, NOM, 0, 2);
+      t64.t_origin = t0;
+      t64.hv_base = hv32;
+      for (int j = 0; j < NOK; ++j)
+        vec_copy_scale(QS3X_FP64, t64.slot(j), QS3X_FP32, t32s.slot(j), dim, nullptr, -1, 1.0, st);
       QS3X_HIP(hipStreamSynchronize(st));
+      RRResult rr = rayleigh_ritz(t64, NOK, NOE);
+      trace_point(t0, hv32 + t64.hv_count, rr.theta[0], QS3X_FP64);
+      if (options().verbose)
+        std::fprintf(stderr, "[qs3x] fp64 Rayleigh-Ritz: theta0=%.15g residual %.3e\n", rr.theta[0], rr.resid[0]);
+      // accept when |theta - lambda| <= r^2 / gap (gap from the Rayleigh-Ritz values)
+      // stays within conv * |theta| for every wanted pair
+      bool ok = true;
+      for (int j = 0; j < NOE; ++j) {
+        double gap = 1e300;
+        for (int i = 0; i < NOK; ++i)
+          if (i != j) gap = std::min(gap, std::fabs(rr.theta[i] - rr.theta[j]));
+        const double bound = rr.resid[j] * rr.resid[j] / std::max(gap, 1e-300);
+        if (!(bound <= conv * std::fabs(rr.theta[j]))) ok = false;
+      }
+      resid = rr.resid;
+      theta = rr.theta;
+      if (ok) {
+        refined = true;
+        rr_j0 = 2 * NOK;
+      } else {
+        // fall back to FP64 thick restart from the refined vectors
+        DevBuf<double> acc((size_t)2 * dim);
+        QS3X_HIP(hipMemsetAsync(acc.p, 0, 16 * dim, st));
+        for (int j = 0; j < NOE; ++j)
+          vec_axpy_acc(QS3X_FP64, acc.p, t64.slot(2 * NOK + j), dim, 1.0, nullptr, -1, st);
+        vec_copy_scale(QS3X_FP64, t64.slot(0), QS3X_FP64, acc.p, dim, nullptr, -1, 1.0, st);
+        QS3X_HIP(hipStreamSynchronize(st));
+      }
     } else {
       t64.init(s, QS3X_FP64, NOK, NOM);
+      t64.t_origin = t0;
       load_start(QS3X_FP64, t64.slot(0), dim, start, seed, st);
     }
-    TRResult r = tr_cycles(t64, maxitr, conv, 0.0, theta, resid);
-    restarts += r.restarts;
+    if (!refined) {
+      TRResult r = tr_cycles(t64, maxitr, conv, 0.0, theta, resid);
+      restarts += r.restarts;
+    }
     hv64 = t64.hv_count;
     for (int j = 0; j < NOE; ++j) ene[j] = theta[j];
     if (psi) {
       for (int j = 0; j < NOE; ++j) {
         // Ritz vectors are stored normalised to FP64 accuracy after rotation;
         // normalise explicitly for the host copy.
-        t64.set_norm_scale(j);
+        t64.set_norm_scale(rr_j0 + j);
         DevBuf<double> tmp((size_t)2 * dim);
-        vec_copy_scale(QS3X_FP64, tmp.p, QS3X_FP64, t64.slot(j), dim, t64.sc.p, t64.S() + j, 1.0, st);
+        vec_copy_scale(QS3X_FP64, tmp.p, QS3X_FP64, t64.slot(rr_j0 + j), dim, t64.sc.p, t64.S() + rr_j0 + j, 1.0, st);
         tmp.download(psi + (size_t)2 * dim * j, (size_t)2 * dim);
       }
     }
@@ -410,7 +621,8 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
       stats[3] = now_s() - t0;
       stats[4] = t32;
       stats[5] = mr;
-      stats[6] = stats[7] = 0.0;
+      stats[6] = refined ? 1.0 : 0.0;  // FP32 Ritz vectors accepted after the FP64 Rayleigh-Ritz step
+      stats[7] = 0.0;
     }
   });
 }

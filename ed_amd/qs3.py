@@ -57,6 +57,7 @@ def _load():
         "qs3x_time_hv": [_P, _i32, _i32, _P],
         "qs3x_lanczos": [_P, _i32, _i64, _i64, _i64, _dbl, _P, _u64, _P, _P, _P],
         "qs3x_trlanczos": [_P, _i32, _i32, _i32, _i32, _i64, _dbl, _P, _u64, _P, _P, _P],
+        "qs3x_solver_trace": [_P, _i64, _P],
         "qs3x_correlations": [_P, _P, _i64, _P, _P, _P],
         "qs3x_local_mag": [_P, _P, _P],
         "qs3x_apply_sq": [_P, _P, _i32, _P, _P, _P, _P],
@@ -91,7 +92,8 @@ def _c128(a):
 
 
 def set_option(name, value):
-    """qs3x_set_option: 'tr_reorth' (0 full, 1 locked + local), 'tr_cgs_passes', 'fp32_floor'."""
+    """qs3x_set_option: 'tr_reorth' (0 full, 1 locked + local), 'tr_cgs_passes', 'fp32_floor',
+    'hv_batched', 'verbose'."""
     _check(_lib.qs3x_set_option(name.encode(), float(value)))
 
 
@@ -225,8 +227,19 @@ def my_trlanczos_routines_complex(sector, NOE, NOK, NOM, maxitr=1000, lnc_ene_co
     stats = np.zeros(8)
     _check(_lib.qs3x_trlanczos(sector._h, int(prec), int(NOE), int(NOK), int(NOM), int(maxitr),
                                float(lnc_ene_conv), _p(st), int(seed), _p(ene), _p(psi), _p(stats)))
-    keys = ["restarts", "hv_fp32", "hv_fp64", "seconds", "seconds_fp32", "max_residual"]
-    return ene, (psi.T.copy() if psi is not None else None), dict(zip(keys, stats[:6].tolist()))
+    keys = ["restarts", "hv_fp32", "hv_fp64", "seconds", "seconds_fp32", "max_residual", "rr_accepted"]
+    return ene, (psi.T.copy() if psi is not None else None), dict(zip(keys, stats[:7].tolist()))
+
+
+def solver_trace():
+    """Convergence trace of the last solve: array of rows
+    (seconds since start, H.v count, lowest Ritz value, precision 32/64)."""
+    n = np.zeros(1, dtype=np.int64)
+    _check(_lib.qs3x_solver_trace(ctypes.c_void_p(0), 0, _p(n)))
+    out = np.zeros((int(n[0]), 4))
+    if n[0]:
+        _check(_lib.qs3x_solver_trace(_p(out), int(n[0]), _p(n)))
+    return out
 
 
 def get_cf_2_wave_vector(sector, psi, pairs):

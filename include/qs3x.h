@@ -47,7 +47,11 @@ int qs3x_set_device(int32_t device);
  *                   vectors and the last two Lanczos vectors only.
  *   "tr_cgs_passes" classical Gram-Schmidt passes per step (default 2).
  *   "fp32_floor"    Ritz-value change (absolute) that ends the FP32 phase of
- *                   the mixed thick restart (default 1e-6). */
+ *                   the mixed thick restart (default 1e-6).
+ *   "hv_batched"    1 (default): chain H.v issues all of a row's index reads,
+ *                   then all its gathers; 0: one hop at a time.
+ *   "verbose"       0 (default); 1: a stderr line per thick restart and per
+ *                   100 Lanczos steps; 2: per convergence check. */
 int qs3x_set_option(const char *name, double value);
 
 /* ----------------------------------------------------------------- model */
@@ -122,10 +126,20 @@ int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, i
  * in FP64 until the FP64 criterion holds.  ene (length NOE) receives the
  * lowest Ritz values; psi (host, dim x NOE column-major, may be NULL) the
  * Ritz vectors; stats (may be NULL, 8 doubles): [restarts, hv_fp32, hv_fp64,
- * seconds_total, seconds_fp32, max_residual, 0, 0]. */
+ * seconds_total, seconds_fp32, max_residual, rr_accepted, 0].  In FP32 mode the
+ * FP32 Ritz vectors go through one FP64 Rayleigh-Ritz step (NOK FP64 H.v); the
+ * result is accepted (rr_accepted = 1) when r^2/gap <= conv |theta| for every
+ * wanted pair, otherwise an FP64 thick restart continues from them. */
 int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32_t NOM,
                    int64_t maxitr, double conv, const double *start, uint64_t seed,
                    double *ene, double *psi, double *stats);
+
+/* Convergence trace of the last qs3x_lanczos / qs3x_trlanczos call (no
+ * counterpart in the reference, whose solvers print progress instead):
+ * *n rows of [seconds since the solve started, H.v applications so far,
+ * lowest Ritz value, 32 or 64 = precision of the Krylov vectors]; the first
+ * min(cap, *n) rows are written to out (4 doubles each; out may be NULL). */
+int qs3x_solver_trace(double *out, int64_t cap, int64_t *n);
 
 /* ------------------------------------------------------------ observables */
 /* <S^z_i S^z_j> and <S^+_i S^-_j> for npairs ordered pairs (1-based sites,

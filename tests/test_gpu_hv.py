@@ -63,23 +63,30 @@ def test_sector_lists_bit_exact(case):
     assert np.array_equal(lr, osec.list_r), np.max(np.abs(lr - osec.list_r))
 
 
+@pytest.mark.parametrize("batched", [1, 0])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_hv_fp64_parity(case):
-    print(f"[gpu] hv fp64 {case[0]}", flush=True)
+def test_hv_fp64_parity(case, batched):
+    print(f"[gpu] hv fp64 {case[0]} batched={batched}", flush=True)
+    q3 = _q3()
     om, osec, gm, gsec = _build(case)
     rng = np.random.default_rng(7)
     v = rng.uniform(-1, 1, osec.dim) + 1j * rng.uniform(-1, 1, osec.dim)
     ref = om.hv(osec, v)
-    got = gsec.ham_to_vec_wave_vector(v)
+    q3.set_option("hv_batched", batched)
+    try:
+        got = gsec.ham_to_vec_wave_vector(v)
+    finally:
+        q3.set_option("hv_batched", 1)
     err = np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-300)
     assert err < 1e-12, err
     # nnz matches the oracle's count of in-sector hop targets
     assert gsec.nnz() == om.count_hops(osec)
 
 
+@pytest.mark.parametrize("batched", [1, 0])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_hv_fp32_parity(case):
-    print(f"[gpu] hv fp32 {case[0]}", flush=True)
+def test_hv_fp32_parity(case, batched):
+    print(f"[gpu] hv fp32 {case[0]} batched={batched}", flush=True)
     q3 = _q3()
     om, osec, gm, gsec = _build(case)
     rng = np.random.default_rng(11)
@@ -87,8 +94,12 @@ def test_hv_fp32_parity(case):
     ref = om.hv(osec, v) - gm.calcu_FM_energy() * v
     dv = torch.from_numpy(v.astype(np.complex64)).cuda()
     out = torch.empty_like(dv)
-    gsec.hv_dev(out.data_ptr(), dv.data_ptr(), q3.FP32)
-    torch.cuda.synchronize()
+    q3.set_option("hv_batched", batched)
+    try:
+        gsec.hv_dev(out.data_ptr(), dv.data_ptr(), q3.FP32)
+        torch.cuda.synchronize()
+    finally:
+        q3.set_option("hv_batched", 1)
     got = out.cpu().numpy().astype(np.complex128)
     err = np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-300)
     assert err < 2e-6, err
