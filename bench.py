@@ -85,7 +85,7 @@ def calibrate_copy(nbytes=1 << 30, reps=5):
 
 
 def load_traffic():
-    p = os.path.join(ROOT, "profiles", "hv_c3_fp32_traffic.json")
+    p = os.path.join(ROOT, "profiles", "hv_chain_L1000_N4_fp32_traffic.json")
     if os.path.exists(p):
         with open(p) as f:
             return json.load(f)
@@ -171,6 +171,16 @@ def main():
         tr64 = q3.solver_trace()
         hit64 = first_within(tr64, E0_EXACT, 1e-9)
         log(f"plain FP64 Lanczos: E0={E64:.15f} in {t64:.2f} s, {it64} steps, rel err {abs(E64 / E0_EXACT - 1):.2e}")
+        # the same three-term recurrence on complex-float vectors (FP64 alpha/beta)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        E32p, _, it32p = q3.lanczos_routines(sec, maxitr=5000, minitr=20, itrint=5, lnc_ene_conv=args.conv_plain,
+                                             prec=q3.FP32, seed=1, want_vector=False)
+        torch.cuda.synchronize()
+        t32p = time.perf_counter() - t0
+        hit32p = first_within(q3.solver_trace(), E0_EXACT, 1e-9)
+        log(f"plain FP32-mixed Lanczos: E0={E32p:.15f} in {t32p:.2f} s, {it32p} steps, "
+            f"rel err {abs(E32p / E0_EXACT - 1):.2e}, first within 1e-9 {hit32p}")
         q3.set_option("tr_reorth", args.reorth)
         q3.set_option("tr_cgs_passes", args.cgs)
         q3.set_option("fp32_floor", args.fp32_floor)
@@ -197,6 +207,8 @@ def main():
                  "speedup_fp32_mixed_vs_fp64_plain": t64 / t32 if t32 > 0 else None,
                  # first Ritz value within 1e-9 of the Bethe-ansatz E0 (solver trace; seconds, H.v)
                  "time_to_1e-9_fp64_plain": hit64,
+                 "plain_fp32_mixed": {"E0": E32p, "iters": it32p, "seconds": t32p,
+                                      "rel_err": abs(E32p / E0_EXACT - 1), "time_to_1e-9": hit32p},
                  "time_to_1e-9_fp32_mixed_any": hit32_any,
                  "time_to_1e-9_fp32_mixed_fp64_value": hit32_ref}
         log(f"first within 1e-9: plain FP64 {hit64}, mixed (any) {hit32_any}, mixed (FP64 value) {hit32_ref}")

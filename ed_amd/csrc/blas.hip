@@ -59,6 +59,23 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
+// Streaming kernels walk the vector in tiles of kVecThreads * U elements; in
+// tile T a thread owns elements T * kVecThreads * U + u * kVecThreads + tid
+// (u < U), so each load instruction is coalesced.  Full tiles are processed
+// without bounds checks (all loads of a tile issue back to back, U times the
+// vector count requests in flight); the one partial tile at the end, if any,
+// is handled element by element by the block that owns it.
+template <int U>
+struct Tiling {
+  int64_t n, nfull;
+  __device__ explicit Tiling(int64_t n_) : n(n_), nfull(n_ / ((int64_t)kVecThreads * U)) {}
+  __device__ int64_t base(int64_t T) const { return T * kVecThreads * U + threadIdx.x; }
+  // this block owns the partial tail tile
+  __device__ bool owns_tail() const {
+    return nfull * kVecThreads * U < n && (int64_t)blockIdx.x == nfull % gridDim.x;
+  }
+};
+
 template <class R>
 __global__ __launch_bounds__(kVecThreads) void k_randvec(typename CT<R>::T *v, int64_t n, uint64_t seed,
                                                          int64_t row0) {
@@ -74,29 +91,54 @@ __global__ __launch_bounds__(kVecThreads) void k_randvec(typename CT<R>::T *v, i
   }
 }
 
+constexpr int kU = 4;
+
 template <class RD, class RS>
 __global__ __launch_bounds__(kVecThreads) void k_copy_scale(typename CT<RD>::T *dst, const typename CT<RS>::T *src,
                                                             int64_t n, const double *sc, int is, double s0) {
   const double s = (is >= 0) ? sc[is] : s0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    typename CT<RS>::T x = src[i];
+  auto one = [&](typename CT<RS>::T x, int64_t i) {
     typename CT<RD>::T o;
     o.x = (RD)((double)x.x * s);
     o.y = (RD)((double)x.y * s);
     dst[i] = o;
+  };
+  Tiling<kU> tl(n);
+  for (int64_t T = blockIdx.x; T < tl.nfull; T += gridDim.x) {
+    const int64_t b = tl.base(T);
+    typename CT<RS>::T x[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) x[u] = src[b + u * kVecThreads];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) one(x[u], b + u * kVecThreads);
   }
+  if (tl.owns_tail())
+    for (int64_t i = tl.base(tl.nfull); i < n; i += kVecThreads) one(src[i], i);
 }
 
 template <class R1, class R2>
 __global__ __launch_bounds__(kVecThreads) void k_dot(const typename CT<R1>::T *x, const typename CT<R2>::T *y,
                                                      int64_t n, double *part) {
   double re = 0, im = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    auto a = x[i];
-    auto b = y[i];
+  auto acc = [&](typename CT<R1>::T a, typename CT<R2>::T b) {
     re += (double)a.x * (double)b.x + (double)a.y * (double)b.y;
     im += (double)a.x * (double)b.y - (double)a.y * (double)b.x;
+  };
+  Tiling<kU> tl(n);
+  for (int64_t T = blockIdx.x; T < tl.nfull; T += gridDim.x) {
+    const int64_t b = tl.base(T);
+    typename CT<R1>::T a[kU];
+    typename CT<R2>::T c[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      a[u] = x[b + u * kVecThreads];
+      c[u] = y[b + u * kVecThreads];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) acc(a[u], c[u]);
   }
+  if (tl.owns_tail())
+    for (int64_t i = tl.base(tl.nfull); i < n; i += kVecThreads) acc(x[i], y[i]);
   block_sum2(re, im, &part[2 * blockIdx.x], &part[2 * blockIdx.x + 1]);
 }
 
@@ -129,27 +171,41 @@ __global__ __launch_bounds__(kVecThreads) void k_finalize(const double *part, in
   }
 }
 
-// w = w - a*sa*u - b*sb*up (b term skipped when ib < 0); part = |w|^2.
-template <class R>
+// w = w - a*sa*u - b*sb*up (the b term only when HASB); part = |w|^2.
+template <class R, bool HASB>
 __global__ __launch_bounds__(kVecThreads) void k_update3(typename CT<R>::T *w, const typename CT<R>::T *u,
                                                          const typename CT<R>::T *up, int64_t n, const double *sc,
                                                          int ia, int isa, int ib, int isb, double *part) {
+  using T = typename CT<R>::T;
   const R ca = (R)(sc[ia] * (isa >= 0 ? sc[isa] : 1.0));
-  const R cb = (ib >= 0) ? (R)(sc[ib] * (isb >= 0 ? sc[isb] : 1.0)) : (R)0;
+  const R cb = HASB ? (R)(sc[ib] * (isb >= 0 ? sc[isb] : 1.0)) : (R)0;
   double acc = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    auto x = w[i];
-    auto a = u[i];
+  auto one = [&](T x, T a, T b, int64_t i) {
     x.x -= ca * a.x;
     x.y -= ca * a.y;
-    if (ib >= 0) {
-      auto b = up[i];
+    if (HASB) {
       x.x -= cb * b.x;
       x.y -= cb * b.y;
     }
     w[i] = x;
     acc += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
+  };
+  Tiling<kU> tl(n);
+  for (int64_t T0 = blockIdx.x; T0 < tl.nfull; T0 += gridDim.x) {
+    const int64_t b0 = tl.base(T0);
+    T x[kU], a[kU], b[kU];
+#pragma unroll
+    for (int q = 0; q < kU; ++q) {
+      x[q] = w[b0 + q * kVecThreads];
+      a[q] = u[b0 + q * kVecThreads];
+      if (HASB) b[q] = up[b0 + q * kVecThreads];
+      else b[q] = a[q];
+    }
+#pragma unroll
+    for (int q = 0; q < kU; ++q) one(x[q], a[q], b[q], b0 + q * kVecThreads);
   }
+  if (tl.owns_tail())
+    for (int64_t i = tl.base(tl.nfull); i < n; i += kVecThreads) one(w[i], u[i], HASB ? up[i] : u[i], i);
   if (part) block_sum2(acc, 0.0, &part[2 * blockIdx.x], &part[2 * blockIdx.x + 1]);
 }
 
@@ -158,43 +214,80 @@ template <class R>
 __global__ __launch_bounds__(kVecThreads) void k_axpy_acc(double2 *psi, const typename CT<R>::T *u, int64_t n,
                                                           double z, const double *sc, int is) {
   const double c = z * (is >= 0 ? sc[is] : 1.0);
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    auto a = u[i];
-    double2 p = psi[i];
+  auto one = [&](typename CT<R>::T a, double2 p, int64_t i) {
     p.x += c * (double)a.x;
     p.y += c * (double)a.y;
     psi[i] = p;
+  };
+  Tiling<kU> tl(n);
+  for (int64_t T = blockIdx.x; T < tl.nfull; T += gridDim.x) {
+    const int64_t b = tl.base(T);
+    typename CT<R>::T a[kU];
+    double2 p[kU];
+#pragma unroll
+    for (int q = 0; q < kU; ++q) {
+      a[q] = u[b + q * kVecThreads];
+      p[q] = psi[b + q * kVecThreads];
+    }
+#pragma unroll
+    for (int q = 0; q < kU; ++q) one(a[q], p[q], b + q * kVecThreads);
   }
+  if (tl.owns_tail())
+    for (int64_t i = tl.base(tl.nfull); i < n; i += kVecThreads) one(u[i], psi[i], i);
+}
+
+// Elements per thread per tile of the multi-vector kernels (FP64 vectors use
+// twice the registers per element).
+template <class R>
+constexpr int multi_u() {
+  return sizeof(R) == 8 ? 2 : 4;
 }
 
 // h_j partials: part[(j * nb + b) * 2] = sum conj(s_j u_j) w over this block's
-// rows, for j in [j0, j0 + cnt), cnt <= kMaxGroup.
-template <class R>
+// rows, for the CNT vectors j0 .. j0 + CNT - 1.
+template <class R, int CNT>
 __global__ __launch_bounds__(kVecThreads) void k_mdot(const typename CT<R>::T *const *V, const double *scales,
-                                                      const int *sidx, int j0, int cnt,
-                                                      const typename CT<R>::T *w, int64_t n, double *part) {
-  double re[kMaxGroup], im[kMaxGroup];
+                                                      const int *sidx, int j0, const typename CT<R>::T *w,
+                                                      int64_t n, double *part) {
+  constexpr int UU = multi_u<R>();
+  using T = typename CT<R>::T;
+  const T *vp[CNT];
 #pragma unroll
-  for (int j = 0; j < kMaxGroup; ++j) re[j] = im[j] = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    auto b = w[i];
+  for (int j = 0; j < CNT; ++j) vp[j] = V[j0 + j];
+  double re[CNT], im[CNT];
 #pragma unroll
-    for (int j = 0; j < kMaxGroup; ++j) {
-      if (j < cnt) {
-        auto a = V[j0 + j][i];
-        re[j] += (double)a.x * (double)b.x + (double)a.y * (double)b.y;
-        im[j] += (double)a.x * (double)b.y - (double)a.y * (double)b.x;
-      }
+  for (int j = 0; j < CNT; ++j) re[j] = im[j] = 0.0;
+  auto acc = [&](int j, T a, T b) {
+    re[j] += (double)a.x * (double)b.x + (double)a.y * (double)b.y;
+    im[j] += (double)a.x * (double)b.y - (double)a.y * (double)b.x;
+  };
+  Tiling<UU> tl(n);
+  for (int64_t T0 = blockIdx.x; T0 < tl.nfull; T0 += gridDim.x) {
+    const int64_t b0 = tl.base(T0);
+    T b[UU], a[CNT][UU];
+#pragma unroll
+    for (int q = 0; q < UU; ++q) {
+      b[q] = w[b0 + q * kVecThreads];
+#pragma unroll
+      for (int j = 0; j < CNT; ++j) a[j][q] = vp[j][b0 + q * kVecThreads];
     }
+#pragma unroll
+    for (int j = 0; j < CNT; ++j)
+#pragma unroll
+      for (int q = 0; q < UU; ++q) acc(j, a[j][q], b[q]);
   }
+  if (tl.owns_tail())
+    for (int64_t i = tl.base(tl.nfull); i < n; i += kVecThreads) {
+      const T b = w[i];
+#pragma unroll
+      for (int j = 0; j < CNT; ++j) acc(j, vp[j][i], b);
+    }
   const int nb = gridDim.x;
 #pragma unroll
-  for (int j = 0; j < kMaxGroup; ++j) {
-    if (j < cnt) {
-      const double s = scales ? scales[sidx ? sidx[j0 + j] : j0 + j] : 1.0;
-      block_sum2(re[j] * s, im[j] * s, &part[2 * ((size_t)(j0 + j) * nb + blockIdx.x)],
-                 &part[2 * ((size_t)(j0 + j) * nb + blockIdx.x) + 1]);
-    }
+  for (int j = 0; j < CNT; ++j) {
+    const double s = scales ? scales[sidx ? sidx[j0 + j] : j0 + j] : 1.0;
+    block_sum2(re[j] * s, im[j] * s, &part[2 * ((size_t)(j0 + j) * nb + blockIdx.x)],
+               &part[2 * ((size_t)(j0 + j) * nb + blockIdx.x) + 1]);
   }
 }
 
@@ -214,11 +307,38 @@ __global__ __launch_bounds__(kVecThreads) void k_finalize_multi(const double *pa
   }
 }
 
-// w -= sum_j h_j s_j u_j (FP64 arithmetic, rounded to R once); part = |w|^2
+// x -= sum_{j < G} c_{j0+j} V[j0+j][i] for the UU elements of a full tile
+// (coefficients in LDS, FP64 arithmetic).
+template <class R, int G, int UU>
+__device__ __forceinline__ void maxpy_group(const typename CT<R>::T *const *V, const double *sh, int j0, int64_t b0,
+                                            double *xr, double *xi) {
+  using T = typename CT<R>::T;
+  T a[G][UU];
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    const T *vj = V[j0 + j];
+#pragma unroll
+    for (int q = 0; q < UU; ++q) a[j][q] = vj[b0 + q * kVecThreads];
+  }
+#pragma unroll
+  for (int j = 0; j < G; ++j) {
+    const double hr = sh[2 * (j0 + j)], hi = sh[2 * (j0 + j) + 1];
+#pragma unroll
+    for (int q = 0; q < UU; ++q) {
+      xr[q] -= hr * (double)a[j][q].x - hi * (double)a[j][q].y;
+      xi[q] -= hr * (double)a[j][q].y + hi * (double)a[j][q].x;
+    }
+  }
+}
+
+// w -= sum_j h_j s_j u_j (FP64 arithmetic, rounded to R once); part = |w|^2.
+// The m vectors are taken kMaxGroup at a time, then the remainder.
 template <class R>
 __global__ __launch_bounds__(kVecThreads) void k_maxpy(const typename CT<R>::T *const *V, const double *scales,
                                                        const int *sidx, int m, const double *h,
                                                        typename CT<R>::T *w, int64_t n, double *part) {
+  constexpr int UU = multi_u<R>();
+  using T = typename CT<R>::T;
   __shared__ double sh[2 * kMaxBasis];
   for (int j = threadIdx.x; j < m; j += blockDim.x) {
     const double s = scales ? scales[sidx ? sidx[j] : j] : 1.0;
@@ -227,20 +347,51 @@ __global__ __launch_bounds__(kVecThreads) void k_maxpy(const typename CT<R>::T *
   }
   __syncthreads();
   double acc = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    auto x = w[i];
-    double xr = x.x, xi = x.y;
-    for (int j = 0; j < m; ++j) {
-      auto a = V[j][i];
-      const double hr = sh[2 * j], hi = sh[2 * j + 1];
-      xr -= hr * (double)a.x - hi * (double)a.y;
-      xi -= hr * (double)a.y + hi * (double)a.x;
+  Tiling<UU> tl(n);
+  const int mfull = m - m % kMaxGroup;
+  for (int64_t T0 = blockIdx.x; T0 < tl.nfull; T0 += gridDim.x) {
+    const int64_t b0 = tl.base(T0);
+    double xr[UU], xi[UU];
+#pragma unroll
+    for (int q = 0; q < UU; ++q) {
+      const T x = w[b0 + q * kVecThreads];
+      xr[q] = x.x;
+      xi[q] = x.y;
     }
-    x.x = (R)xr;
-    x.y = (R)xi;
-    w[i] = x;
-    acc += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
+    for (int j0 = 0; j0 < mfull; j0 += kMaxGroup) maxpy_group<R, kMaxGroup, UU>(V, sh, j0, b0, xr, xi);
+    switch (m - mfull) {
+      case 1: maxpy_group<R, 1, UU>(V, sh, mfull, b0, xr, xi); break;
+      case 2: maxpy_group<R, 2, UU>(V, sh, mfull, b0, xr, xi); break;
+      case 3: maxpy_group<R, 3, UU>(V, sh, mfull, b0, xr, xi); break;
+      case 4: maxpy_group<R, 4, UU>(V, sh, mfull, b0, xr, xi); break;
+      case 5: maxpy_group<R, 5, UU>(V, sh, mfull, b0, xr, xi); break;
+      case 6: maxpy_group<R, 6, UU>(V, sh, mfull, b0, xr, xi); break;
+      case 7: maxpy_group<R, 7, UU>(V, sh, mfull, b0, xr, xi); break;
+      default: break;
+    }
+#pragma unroll
+    for (int q = 0; q < UU; ++q) {
+      T x;
+      x.x = (R)xr[q];
+      x.y = (R)xi[q];
+      w[b0 + q * kVecThreads] = x;
+      acc += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
+    }
   }
+  if (tl.owns_tail())
+    for (int64_t i = tl.base(tl.nfull); i < n; i += kVecThreads) {
+      T x = w[i];
+      double xr = x.x, xi = x.y;
+      for (int j = 0; j < m; ++j) {
+        const T a = V[j][i];
+        xr -= sh[2 * j] * (double)a.x - sh[2 * j + 1] * (double)a.y;
+        xi -= sh[2 * j] * (double)a.y + sh[2 * j + 1] * (double)a.x;
+      }
+      x.x = (R)xr;
+      x.y = (R)xi;
+      w[i] = x;
+      acc += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
+    }
   if (part) block_sum2(acc, 0.0, &part[2 * blockIdx.x], &part[2 * blockIdx.x + 1]);
 }
 
@@ -253,8 +404,9 @@ int vec_grid(int64_t n) {
     if (hipGetDeviceProperties(&p, device()) == hipSuccess) cus = p.multiProcessorCount;
     if (cus <= 0) cus = 256;
   }
-  int64_t need = (n + kVecThreads - 1) / kVecThreads;
-  return (int)std::max<int64_t>(1, std::min<int64_t>(need, (int64_t)cus * 4));
+  // enough tiles of kVecThreads * 4 elements to give every CU 8 blocks
+  int64_t need = (n + 4 * kVecThreads - 1) / (4 * kVecThreads);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(need, std::min<int64_t>((int64_t)cus * 8, kMaxBlocks)));
 }
 
 #define GRID(n) dim3(vec_grid(n)), dim3(kVecThreads), 0, st
@@ -300,12 +452,21 @@ void finalize(const double *part, int nb, double *sc, int mode, int i0, int i1, 
 int vec_update3(int prec, void *w, const void *u, const void *up, int64_t n, const double *sc, int ia, int isa,
                 int ib, int isb, double *part, hipStream_t st) {
   const int nb = vec_grid(n);
-  if (prec == QS3X_FP64)
-    hipLaunchKernelGGL(k_update3<double>, GRID(n), (double2 *)w, (const double2 *)u, (const double2 *)up, n, sc, ia,
-                       isa, ib, isb, part);
-  else
-    hipLaunchKernelGGL(k_update3<float>, GRID(n), (float2 *)w, (const float2 *)u, (const float2 *)up, n, sc, ia, isa,
-                       ib, isb, part);
+  if (prec == QS3X_FP64) {
+    if (ib >= 0)
+      hipLaunchKernelGGL((k_update3<double, true>), GRID(n), (double2 *)w, (const double2 *)u, (const double2 *)up, n,
+                         sc, ia, isa, ib, isb, part);
+    else
+      hipLaunchKernelGGL((k_update3<double, false>), GRID(n), (double2 *)w, (const double2 *)u, (const double2 *)up,
+                         n, sc, ia, isa, ib, isb, part);
+  } else {
+    if (ib >= 0)
+      hipLaunchKernelGGL((k_update3<float, true>), GRID(n), (float2 *)w, (const float2 *)u, (const float2 *)up, n, sc,
+                         ia, isa, ib, isb, part);
+    else
+      hipLaunchKernelGGL((k_update3<float, false>), GRID(n), (float2 *)w, (const float2 *)u, (const float2 *)up, n,
+                         sc, ia, isa, ib, isb, part);
+  }
   QS3X_HIP(hipGetLastError());
   return nb;
 }
@@ -322,17 +483,29 @@ void vec_axpy_acc(int prec, double *psi, const void *u, int64_t n, double z, con
 // h = V^H w for the m vectors of the device pointer array V (scaled by
 // scales[j] if given) -> h (complex, device, 2m doubles).  part must hold
 // 2 * m * vec_grid(n) doubles.
+template <class R>
+static void mdot_group(const void *const *V, const double *scales, const int *sidx, int j0, int cnt, const void *w,
+                       int64_t n, double *part, hipStream_t st) {
+  using T = typename CT<R>::T;
+#define MD(C)                                                                                                \
+  case C:                                                                                                    \
+    hipLaunchKernelGGL((k_mdot<R, C>), GRID(n), (const T *const *)V, scales, sidx, j0, (const T *)w, n, part); \
+    break
+  switch (cnt) {
+    MD(1); MD(2); MD(3); MD(4); MD(5); MD(6); MD(7); MD(8);
+    default: throw Error{"vec_mdot: bad group size"};
+  }
+#undef MD
+}
+
 void vec_mdot(int prec, const void *const *V, const double *scales, const int *sidx, int m, const void *w, int64_t n,
               double *part, double *h, hipStream_t st) {
+  static_assert(kMaxGroup == 8, "mdot_group dispatches groups of up to 8 vectors");
   const int nb = vec_grid(n);
   for (int j0 = 0; j0 < m; j0 += kMaxGroup) {
     const int cnt = std::min(kMaxGroup, m - j0);
-    if (prec == QS3X_FP64)
-      hipLaunchKernelGGL(k_mdot<double>, GRID(n), (const double2 *const *)V, scales, sidx, j0, cnt,
-                         (const double2 *)w, n, part);
-    else
-      hipLaunchKernelGGL(k_mdot<float>, GRID(n), (const float2 *const *)V, scales, sidx, j0, cnt,
-                         (const float2 *)w, n, part);
+    if (prec == QS3X_FP64) mdot_group<double>(V, scales, sidx, j0, cnt, w, n, part, st);
+    else mdot_group<float>(V, scales, sidx, j0, cnt, w, n, part, st);
   }
   hipLaunchKernelGGL(k_finalize_multi, dim3(m), dim3(kVecThreads), 0, st, part, nb, h);
   QS3X_HIP(hipGetLastError());

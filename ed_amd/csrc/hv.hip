@@ -666,7 +666,7 @@ int persistent_grid(int64_t ntiles) {
     if (hipGetDeviceProperties(&p, device()) == hipSuccess) cus = p.multiProcessorCount;
     if (cus <= 0) cus = 256;
   }
-  int64_t g = (int64_t)cus * 8;
+  int64_t g = std::min<int64_t>((int64_t)cus * 8, kMaxBlocks);
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, ntiles));
 }
 

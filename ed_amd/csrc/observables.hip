@@ -544,7 +544,7 @@ int qs3x_dsf_recurrence(qs3x_sector *tgt, int32_t prec, int32_t nvec, const doub
     // slots: arec [NS] | aout [NS] | beta [NS] | scale [NS+1] | norm tmp
     DevBuf<double> sc((size_t)4 * NS + 8);
     const int A_REC = 0, A_OUT = NS, BETA = 2 * NS, SCALE = 3 * NS, TMP = 4 * NS + 2;
-    DevBuf<double> part((size_t)2 * 4096);
+    DevBuf<double> part((size_t)2 * kMaxBlocks);
     for (int v = 0; v < nvec; ++v) {
       tmp64.upload(f + (size_t)2 * dim * v, (size_t)2 * dim);
       vec_copy_scale(prec, U[0].p, QS3X_FP64, tmp64.p, dim, nullptr, -1, 1.0, st);

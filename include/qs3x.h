@@ -128,8 +128,10 @@ int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, i
  * Ritz vectors; stats (may be NULL, 8 doubles): [restarts, hv_fp32, hv_fp64,
  * seconds_total, seconds_fp32, max_residual, rr_accepted, 0].  In FP32 mode the
  * FP32 Ritz vectors go through one FP64 Rayleigh-Ritz step (NOK FP64 H.v); the
- * result is accepted (rr_accepted = 1) when r^2/gap <= conv |theta| for every
- * wanted pair, otherwise an FP64 thick restart continues from them. */
+ * result is accepted (rr_accepted = 1) when, for every wanted pair, r^2/gap <=
+ * conv |theta|, or the FP32 restarts met the convergence test and the FP64
+ * Rayleigh quotient equals the FP32 Ritz value to conv |theta|; otherwise an
+ * FP64 thick restart (full reorthogonalisation) continues from them. */
 int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32_t NOM,
                    int64_t maxitr, double conv, const double *start, uint64_t seed,
                    double *ene, double *psi, double *stats);
