@@ -83,9 +83,9 @@ int qs3x_set_option(const char *name, double value) {
     } else if (n == "verbose") {
       QS3X_REQUIRE(value >= 0 && value <= 2, "verbose must be 0..2");
       options().verbose = (int)value;
-    } else if (n == "hv_batched") {
-      QS3X_REQUIRE(value == 0 || value == 1, "hv_batched must be 0 or 1");
-      options().hv_batched = (int)value;
+    } else if (n == "hv_kernel") {
+      QS3X_REQUIRE(value >= 0 && value <= 3, "hv_kernel must be 0..3");
+      options().hv_kernel = (int)value;
     } else if (n == "fp32_floor") {
       QS3X_REQUIRE(value > 0, "fp32_floor must be > 0");
       options().fp32_floor = value;
@@ -152,6 +152,25 @@ int qs3x_model_create(int32_t NOS, int32_t LX, int32_t LY, int32_t LZ, int32_t N
       for (int32_t e : adj[s]) m->adj.push_back(e);
     }
     m->adj_ptr[0] = 0;
+    if (m->chain && NOS >= 3) {
+      bool nn = true;
+      m->nn_bt.assign(NOS + 1, 0);
+      for (int s = 1; s <= NOS && nn; ++s) {
+        const int right = s % NOS + 1, left = (s + NOS - 2) % NOS + 1;
+        if (adj[s].size() != 2) {
+          nn = false;
+          break;
+        }
+        const int o0 = adj[s][0] & 0xffff, o1 = adj[s][1] & 0xffff;
+        if (!((o0 == right && o1 == left) || (o0 == left && o1 == right))) nn = false;
+        m->nn_bt[s] = (uint16_t)((o0 == right ? adj[s][0] : adj[s][1]) >> 16);
+      }
+      m->nn_chain = nn;
+      if (nn) {
+        m->d_nn_bt.alloc(NOS + 1);
+        m->d_nn_bt.upload(m->nn_bt.data(), NOS + 1);
+      }
+    }
     m->d_adj_ptr.alloc(m->adj_ptr.size());
     m->d_adj_ptr.upload(m->adj_ptr.data(), m->adj_ptr.size());
     m->d_adj.alloc(std::max<size_t>(1, m->adj.size()));

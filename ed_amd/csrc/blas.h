@@ -34,7 +34,11 @@ struct Options {
   int tr_reorth = 0;          // 0: every previous basis vector; 1: locked Ritz vectors + last two
   int tr_cgs_passes = 2;      // classical Gram-Schmidt passes per step
   double fp32_floor = 1e-6;   // |delta theta| that ends the FP32 phase of the mixed thick restart
-  int hv_batched = 1;         // chain H.v: 1 = batched-load kernel where available, 0 = per-hop loads
+  // H.v kernel choice: 0 = fastest available (nearest-neighbour chain, then
+  // batched-load chain, then per-hop chain, then general lattice); 1 = from
+  // the batched-load chain kernel on; 2 = from the per-hop chain kernel on;
+  // 3 = general lattice kernel (translation scan).
+  int hv_kernel = 0;
   int verbose = 0;            // 1: one stderr line per restart / per 100 Lanczos steps
 };
 Options &options();

@@ -26,74 +26,13 @@
 #include <algorithm>
 
 #include "blas.h"
+#include "hv_common.h"
 #include "qs3x_internal.h"
 
 namespace qs3x {
 namespace {
 
-template <class R>
-struct CT;
-template <>
-struct CT<double> {
-  using T = double2;
-};
-template <>
-struct CT<float> {
-  using T = float2;
-};
-
-struct HvArgs {
-  const uint64_t *cfg;
-  const double *list_r;
-  const int32_t *index;
-  const int64_t *list_s;
-  const void *v1;
-  void *v0;
-  int64_t dim;
-  int64_t ntiles;
-  int L, bits, kbits, mode;
-  int LX, LY, LZ, NOS;
-  const int32_t *adj_ptr;
-  const int32_t *adj;
-  int nadj;
-  const double *bt;  // interleaved jxy2, jz4
-  int nbt;
-  const int64_t *binom;  // global [n][k]
-  int kdim;
-  const double *rclass;
-  const double *explist;  // complex
-  double ezz;
-  const double *scale;  // device scalar or null
-  double *dot_partials; // 2 per block or null
-  unsigned long long *nnz_out;
-};
-
-constexpr int kThreads = 256;
-
-// Block reduction of a complex double into out[2*blockIdx.x].
-__device__ void block_reduce_store(double re, double im, double *out) {
-  __shared__ double red[2][kThreads / 64];
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) {
-    re += __shfl_down(re, d, 64);
-    im += __shfl_down(im, d, 64);
-  }
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) {
-    red[0][wid] = re;
-    red[1][wid] = im;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double a = 0, b = 0;
-    for (int w = 0; w < kThreads / 64; ++w) {
-      a += red[0][w];
-      b += red[1][w];
-    }
-    out[2 * blockIdx.x] = a;
-    out[2 * blockIdx.x + 1] = b;
-  }
-}
+using namespace hvk;
 
 // ------------------------------------------------------------ chain kernel
 // Template parameters: N = N_down; R = vector precision; REALPH = every phase
@@ -591,7 +530,6 @@ __global__ __launch_bounds__(kThreads) void k_hv_general(HvArgs a, int N) {
 }
 
 // ----------------------------------------------------------------- dispatch
-using KFn = void (*)(HvArgs);
 
 template <int N, class KeyT>
 KFn pick_chain_k(int prec, bool realph, bool normlist, bool count) {
@@ -679,6 +617,8 @@ static HvArgs make_args(qs3x_sector *s, void *v0, const void *v1, const double *
   a.cfg = s->d_cfg.p;
   a.list_r = s->d_list_r.p;
   a.index = s->d_index.p;
+  a.bitmap = s->d_bitmap.p;
+  a.nn_bt = m->d_nn_bt.p;
   a.list_s = s->d_list_s.p;
   a.v1 = v1;
   a.v0 = v0;
@@ -735,7 +675,17 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   const int grid = persistent_grid(a.ntiles);
   if (nblocks_out) *nblocks_out = grid;
   const bool normlist = s->d_rclass.p == nullptr;  // sectors from caller lists carry list_r only
-  if (use_chain(s, prec) && !normlist && options().hv_batched) {
+  const int pick = options().hv_kernel;
+  if (pick <= 0 && use_chain(s, prec) && !normlist && s->model->nn_chain && s->d_bitmap.p) {
+    const size_t lds = chain_nn_lds_bytes(s->nod, s->model->NOS, (int)s->model->bt_jxy2.size(), prec);
+    KFn f = (lds <= 64 * 1024) ? chain_nn_kernel(s->nod, prec, s->real_phase, key32(s)) : nullptr;
+    if (f) {
+      hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), lds, st, a);
+      QS3X_HIP(hipGetLastError());
+      return;
+    }
+  }
+  if (pick <= 1 && use_chain(s, prec) && !normlist) {
     KFn f = chain_mlp_kernel(s->nod, prec, s->real_phase, s->model->max_deg, key32(s));
     if (f) {
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, prec), st, a);
@@ -743,7 +693,7 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
       return;
     }
   }
-  if (use_chain(s, prec) && !(normlist && prec != QS3X_FP64)) {
+  if (pick <= 2 && use_chain(s, prec) && !(normlist && prec != QS3X_FP64)) {
     KFn f = chain_kernel(s->nod, prec, s->real_phase, normlist, false, key32(s));
     if (f) {
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, prec), st, a);

@@ -1,0 +1,86 @@
+// hv_common.h -- definitions shared by the H.v kernels (hv.hip, hv_nn.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "qs3x_internal.h"
+
+namespace qs3x {
+namespace hvk {
+
+template <class R>
+struct CT;
+template <>
+struct CT<double> {
+  using T = double2;
+};
+template <>
+struct CT<float> {
+  using T = float2;
+};
+
+struct HvArgs {
+  const uint64_t *cfg;
+  const double *list_r;
+  const int32_t *index;
+  const ulonglong2 *bitmap;  // anchored index words: x = presence bits, low half of y = row of the first
+  const int64_t *list_s;
+  const void *v1;
+  void *v0;
+  int64_t dim;
+  int64_t ntiles;
+  int L, bits, kbits, mode;
+  int LX, LY, LZ, NOS;
+  const int32_t *adj_ptr;
+  const int32_t *adj;
+  int nadj;
+  const uint16_t *nn_bt;  // nearest-neighbour chain: bond type of (s, s+1), s = 1..L
+  const double *bt;       // interleaved jxy2, jz4
+  int nbt;
+  const int64_t *binom;  // global [n][k]
+  int kdim;
+  const double *rclass;
+  const double *explist;  // complex
+  double ezz;
+  const double *scale;  // device scalar or null
+  double *dot_partials; // 2 per block or null
+  unsigned long long *nnz_out;
+};
+
+constexpr int kThreads = 256;
+
+// Block reduction of a complex double into out[2*blockIdx.x].
+__device__ inline void block_reduce_store(double re, double im, double *out) {
+  __shared__ double red[2][kThreads / 64];
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) {
+    re += __shfl_down(re, d, 64);
+    im += __shfl_down(im, d, 64);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wid] = re;
+    red[1][wid] = im;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0, b = 0;
+    for (int w = 0; w < kThreads / 64; ++w) {
+      a += red[0][w];
+      b += red[1][w];
+    }
+    out[2 * blockIdx.x] = a;
+    out[2 * blockIdx.x + 1] = b;
+  }
+}
+
+using KFn = void (*)(HvArgs);
+
+}  // namespace hvk
+
+// Nearest-neighbour chain kernels (hv_nn.hip): null when no specialisation
+// exists for this N / precision.
+hvk::KFn chain_nn_kernel(int N, int prec, bool realph, bool key32);
+size_t chain_nn_lds_bytes(int N, int L, int nbt, int prec);
+
+}  // namespace qs3x

@@ -80,6 +80,12 @@ struct qs3x_model {
   std::vector<int32_t> adj_ptr, adj;
   std::vector<double> bt_jxy2, bt_jz4;  // per distinct (Jxy, Jz): Jxy/2, Jz/4
   int32_t max_deg = 0;
+  // Nearest-neighbour chain (LY = LZ = 1, NOS >= 3, every site bonded exactly
+  // to s - 1 and s + 1 cyclically, once each): nn_bt[s] = bond type of
+  // (s, s + 1 mod NOS) for s = 1..NOS (entry 0 unused).
+  bool nn_chain = false;
+  std::vector<uint16_t> nn_bt;
+  qs3x::DevBuf<uint16_t> d_nn_bt;
   // device copies
   qs3x::DevBuf<int32_t> d_adj_ptr, d_adj;
   qs3x::DevBuf<double> d_bt;  // [2*nbt]: jxy2, jz4 interleaved
@@ -111,6 +117,9 @@ struct qs3x_sector {
   qs3x::DevBuf<double> d_list_r;
   qs3x::DevBuf<uint64_t> d_cfg;
   qs3x::DevBuf<int32_t> d_index;
+  // Anchored mode: one word per 64 anchored ranks, x = presence bits, low 32
+  // bits of y = row of the word's first present rank (rows ascend with rank).
+  qs3x::DevBuf<ulonglong2> d_bitmap;
   qs3x::DevBuf<double> d_explist;   // complex, (LX+1)(LY+1)(LZ+1)
   qs3x::DevBuf<double> d_rclass;    // chain: r = sqrt(ro) per stabiliser order (0..nod)
   qs3x::DevBuf<int64_t> d_binom;    // row-major [n][k]

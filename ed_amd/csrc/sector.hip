@@ -423,6 +423,27 @@ __global__ void k_full_index_from_ranks(int64_t dim, const int64_t *__restrict__
   index[list_s[i] - 1] = (int32_t)i;
 }
 
+// Bitmap words from the dense anchored index: word w covers ranks 64w..64w+63.
+__global__ void k_bitmap_from_index(const int32_t *__restrict__ index, int64_t ncand,
+                                    ulonglong2 *__restrict__ bm, int64_t nwords) {
+  const int64_t wi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (wi >= nwords) return;
+  uint64_t bits = 0;
+  int32_t first = 0;
+  const int64_t c0 = wi * 64;
+  for (int b = 0; b < 64 && c0 + b < ncand; ++b) {
+    const int32_t id = index[c0 + b];
+    if (id >= 0) {
+      if (bits == 0) first = id;
+      bits |= 1ull << b;
+    }
+  }
+  ulonglong2 o;
+  o.x = bits;
+  o.y = (unsigned long long)(uint32_t)first;
+  bm[wi] = o;
+}
+
 __global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -465,6 +486,8 @@ static bool chain_fast_ok(const qs3x_model *m, int nod) {
   // gap keys fit in 64 bits; anchored ranks (and the dense index) fit in int32
   return (int64_t)(nod - 1) * kb <= 64 && binom_host(m->NOS - 1, nod - 1) < (int64_t)INT32_MAX;
 }
+
+static void build_bitmap(qs3x_sector *s);
 
 void build_sector_device(qs3x_sector *s) {
   qs3x_model *m = s->model;
@@ -512,6 +535,7 @@ void build_sector_device(qs3x_sector *s) {
                        ncand, N, L, s->d_binom.p, s->nbinom_k, d_ro.p, nullptr, d_tiles.p,
                        s->d_list_s.p, s->d_list_r.p, s->d_cfg.p, s->d_index.p, s->bits);
     QS3X_HIP(hipGetLastError());
+    build_bitmap(s);
     QS3X_HIP(hipStreamSynchronize(st));
     return;
   }
@@ -576,6 +600,16 @@ void build_sector_device(qs3x_sector *s) {
   QS3X_HIP(hipStreamSynchronize(st));
 }
 
+static void build_bitmap(qs3x_sector *s) {
+  const int64_t ncand = s->index_len;
+  const int64_t nwords = (ncand + 63) / 64;
+  s->d_bitmap.alloc(std::max<int64_t>(nwords, 1));
+  if (nwords)
+    hipLaunchKernelGGL(k_bitmap_from_index, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, stream(),
+                       s->d_index.p, ncand, s->d_bitmap.p, nwords);
+  QS3X_HIP(hipGetLastError());
+}
+
 void sector_from_lists(qs3x_sector *s, const int64_t *list_s, const double *list_r,
                        const double *explist) {
   qs3x_model *m = s->model;
@@ -626,6 +660,7 @@ void sector_from_lists(qs3x_sector *s, const int64_t *list_s, const double *list
     if (bad == 0) {
       s->chain = true;
       s->mode = kIndexAnchored;
+      build_bitmap(s);
       QS3X_HIP(hipStreamSynchronize(st));
       return;
     }

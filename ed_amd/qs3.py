@@ -93,7 +93,7 @@ def _c128(a):
 
 def set_option(name, value):
     """qs3x_set_option: 'tr_reorth' (0 full, 1 locked + local), 'tr_cgs_passes', 'fp32_floor',
-    'hv_batched', 'verbose'."""
+    'hv_kernel', 'verbose'."""
     _check(_lib.qs3x_set_option(name.encode(), float(value)))
 
 

@@ -48,8 +48,10 @@ int qs3x_set_device(int32_t device);
  *   "tr_cgs_passes" classical Gram-Schmidt passes per step (default 2).
  *   "fp32_floor"    Ritz-value change (absolute) that ends the FP32 phase of
  *                   the mixed thick restart (default 1e-6).
- *   "hv_batched"    1 (default): chain H.v issues all of a row's index reads,
- *                   then all its gathers; 0: one hop at a time.
+ *   "hv_kernel"     0 (default): the fastest applicable H.v kernel (nearest-
+ *                   neighbour chain, batched-load chain, per-hop chain, general
+ *                   lattice, in that order); 1, 2, 3: start the list at the
+ *                   batched-load chain, per-hop chain or general kernel.
  *   "verbose"       0 (default); 1: a stderr line per thick restart and per
  *                   100 Lanczos steps; 2: per convergence check. */
 int qs3x_set_option(const char *name, double value);

@@ -25,6 +25,11 @@ def chain(L, fam):
     return orc.chain_bonds(L, fam)
 
 
+def dimer(L, j1, j2, jz1, jz2):
+    """Nearest-neighbour chain with alternating bonds (two bond types)."""
+    return [(i, i % L + 1, j1 if i % 2 else j2, jz1 if i % 2 else jz2) for i in range(1, L + 1)]
+
+
 CASES = [
     # name, NOS, LX, LY, LZ, nod, (KX, KY, KZ), bonds
     ("chain12_n4_k0", 12, 12, 1, 1, 4, (0, 0, 0), chain(12, [(1, 1.0, 1.0)])),
@@ -35,6 +40,13 @@ CASES = [
     ("chain20_n1_k3", 20, 20, 1, 1, 1, (3, 0, 0), chain(20, [(1, 1.0, 1.0)])),
     ("chain24_n3_k7_nnn", 24, 24, 1, 1, 3, (7, 0, 0), chain(24, [(1, 1.0, 1.0), (2, 0.5, 0.5)])),
     ("chain36_n6_k0_C1", 36, 36, 1, 1, 6, (0, 0, 0), chain(36, [(1, 1.0, 1.0)])),
+    ("chain14_n7_k0", 14, 14, 1, 1, 7, (0, 0, 0), chain(14, [(1, 1.0, 1.0)])),
+    ("chain14_n7_k7", 14, 14, 1, 1, 7, (7, 0, 0), chain(14, [(1, 1.0, 0.6)])),
+    ("chain15_n5_k3_dimer", 15, 15, 1, 1, 5, (3, 0, 0), dimer(15, 1.0, 0.4, 0.8, 0.3)),
+    ("chain16_n8_k2_dimer", 16, 16, 1, 1, 8, (2, 0, 0), dimer(16, 1.0, 0.5, 1.0, 0.25)),
+    ("chain8_n2_k4", 8, 8, 1, 1, 2, (4, 0, 0), chain(8, [(1, 1.0, 1.0)])),
+    ("chain3_n2_k1", 3, 3, 1, 1, 2, (1, 0, 0), chain(3, [(1, 1.0, 0.5)])),
+    ("chain40_n3_k5", 40, 40, 1, 1, 3, (5, 0, 0), chain(40, [(1, 0.7, 1.3)])),
     ("torus6x6_n3_k00", 36, 6, 6, 1, 3, (0, 0, 0), orc.square_bonds(6, 6, 1.0)),
     ("torus6x6_n3_k12", 36, 6, 6, 1, 3, (1, 2, 0), orc.square_bonds(6, 6, 1.0)),
     ("torus4x4_n4_k10", 16, 4, 4, 1, 4, (1, 0, 0), orc.square_bonds(4, 4, 1.0)),
@@ -63,30 +75,35 @@ def test_sector_lists_bit_exact(case):
     assert np.array_equal(lr, osec.list_r), np.max(np.abs(lr - osec.list_r))
 
 
-@pytest.mark.parametrize("batched", [1, 0])
+# 0: fastest (nearest-neighbour chain where it applies), 1: batched-load chain,
+# 2: per-hop chain, 3: general translation scan
+KERNELS = [0, 1, 2, 3]
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_hv_fp64_parity(case, batched):
-    print(f"[gpu] hv fp64 {case[0]} batched={batched}", flush=True)
+def test_hv_fp64_parity(case, kernel):
+    print(f"[gpu] hv fp64 {case[0]} kernel={kernel}", flush=True)
     q3 = _q3()
     om, osec, gm, gsec = _build(case)
     rng = np.random.default_rng(7)
     v = rng.uniform(-1, 1, osec.dim) + 1j * rng.uniform(-1, 1, osec.dim)
     ref = om.hv(osec, v)
-    q3.set_option("hv_batched", batched)
+    q3.set_option("hv_kernel", kernel)
     try:
         got = gsec.ham_to_vec_wave_vector(v)
     finally:
-        q3.set_option("hv_batched", 1)
+        q3.set_option("hv_kernel", 0)
     err = np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-300)
     assert err < 1e-12, err
     # nnz matches the oracle's count of in-sector hop targets
     assert gsec.nnz() == om.count_hops(osec)
 
 
-@pytest.mark.parametrize("batched", [1, 0])
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_hv_fp32_parity(case, batched):
-    print(f"[gpu] hv fp32 {case[0]} batched={batched}", flush=True)
+def test_hv_fp32_parity(case, kernel):
+    print(f"[gpu] hv fp32 {case[0]} kernel={kernel}", flush=True)
     q3 = _q3()
     om, osec, gm, gsec = _build(case)
     rng = np.random.default_rng(11)
@@ -94,12 +111,12 @@ def test_hv_fp32_parity(case, batched):
     ref = om.hv(osec, v) - gm.calcu_FM_energy() * v
     dv = torch.from_numpy(v.astype(np.complex64)).cuda()
     out = torch.empty_like(dv)
-    q3.set_option("hv_batched", batched)
+    q3.set_option("hv_kernel", kernel)
     try:
         gsec.hv_dev(out.data_ptr(), dv.data_ptr(), q3.FP32)
         torch.cuda.synchronize()
     finally:
-        q3.set_option("hv_batched", 1)
+        q3.set_option("hv_kernel", 0)
     got = out.cpu().numpy().astype(np.complex128)
     err = np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-300)
     assert err < 2e-6, err
