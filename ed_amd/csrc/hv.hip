@@ -678,7 +678,8 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   const int pick = options().hv_kernel;
   if (pick <= 0 && use_chain(s, prec) && !normlist && s->model->nn_chain && s->d_bitmap.p) {
     const size_t lds = chain_nn_lds_bytes(s->nod, s->model->NOS, (int)s->model->bt_jxy2.size(), prec);
-    KFn f = (lds <= 64 * 1024) ? chain_nn_kernel(s->nod, prec, s->real_phase, key32(s)) : nullptr;
+    const bool uniform = s->model->bt_jxy2.size() == 1;
+    KFn f = (lds <= 64 * 1024) ? chain_nn_kernel(s->nod, prec, s->real_phase, uniform, key32(s)) : nullptr;
     if (f) {
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), lds, st, a);
       QS3X_HIP(hipGetLastError());

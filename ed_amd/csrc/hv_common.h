@@ -23,7 +23,7 @@ struct HvArgs {
   const uint64_t *cfg;
   const double *list_r;
   const int32_t *index;
-  const ulonglong2 *bitmap;  // anchored index words: x = presence bits, low half of y = row of the first
+  const uint2 *bitmap;  // anchored index, 32 ranks per word: x = presence bits, y = row of the first
   const int64_t *list_s;
   const void *v1;
   void *v0;
@@ -80,7 +80,8 @@ using KFn = void (*)(HvArgs);
 
 // Nearest-neighbour chain kernels (hv_nn.hip): null when no specialisation
 // exists for this N / precision.
-hvk::KFn chain_nn_kernel(int N, int prec, bool realph, bool key32);
+// uniform: every bond of the chain has the same (Jxy, Jz).
+hvk::KFn chain_nn_kernel(int N, int prec, bool realph, bool uniform, bool key32);
 size_t chain_nn_lds_bytes(int N, int L, int nbt, int prec);
 
 }  // namespace qs3x

@@ -117,9 +117,9 @@ struct qs3x_sector {
   qs3x::DevBuf<double> d_list_r;
   qs3x::DevBuf<uint64_t> d_cfg;
   qs3x::DevBuf<int32_t> d_index;
-  // Anchored mode: one word per 64 anchored ranks, x = presence bits, low 32
-  // bits of y = row of the word's first present rank (rows ascend with rank).
-  qs3x::DevBuf<ulonglong2> d_bitmap;
+  // Anchored mode: one word per 32 anchored ranks, x = presence bits, y = row
+  // of the word's first present rank (rows ascend with rank).
+  qs3x::DevBuf<uint2> d_bitmap;
   qs3x::DevBuf<double> d_explist;   // complex, (LX+1)(LY+1)(LZ+1)
   qs3x::DevBuf<double> d_rclass;    // chain: r = sqrt(ro) per stabiliser order (0..nod)
   qs3x::DevBuf<int64_t> d_binom;    // row-major [n][k]

@@ -423,24 +423,24 @@ __global__ void k_full_index_from_ranks(int64_t dim, const int64_t *__restrict__
   index[list_s[i] - 1] = (int32_t)i;
 }
 
-// Bitmap words from the dense anchored index: word w covers ranks 64w..64w+63.
-__global__ void k_bitmap_from_index(const int32_t *__restrict__ index, int64_t ncand,
-                                    ulonglong2 *__restrict__ bm, int64_t nwords) {
+// Bitmap words from the dense anchored index: word w covers ranks 32w..32w+31.
+__global__ void k_bitmap_from_index(const int32_t *__restrict__ index, int64_t ncand, uint2 *__restrict__ bm,
+                                    int64_t nwords) {
   const int64_t wi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (wi >= nwords) return;
-  uint64_t bits = 0;
+  uint32_t bits = 0;
   int32_t first = 0;
-  const int64_t c0 = wi * 64;
-  for (int b = 0; b < 64 && c0 + b < ncand; ++b) {
+  const int64_t c0 = wi * 32;
+  for (int b = 0; b < 32 && c0 + b < ncand; ++b) {
     const int32_t id = index[c0 + b];
     if (id >= 0) {
       if (bits == 0) first = id;
-      bits |= 1ull << b;
+      bits |= 1u << b;
     }
   }
-  ulonglong2 o;
+  uint2 o;
   o.x = bits;
-  o.y = (unsigned long long)(uint32_t)first;
+  o.y = (uint32_t)first;
   bm[wi] = o;
 }
 
@@ -602,7 +602,7 @@ void build_sector_device(qs3x_sector *s) {
 
 static void build_bitmap(qs3x_sector *s) {
   const int64_t ncand = s->index_len;
-  const int64_t nwords = (ncand + 63) / 64;
+  const int64_t nwords = (ncand + 31) / 32;
   s->d_bitmap.alloc(std::max<int64_t>(nwords, 1));
   if (nwords)
     hipLaunchKernelGGL(k_bitmap_from_index, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, stream(),
