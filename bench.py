@@ -226,7 +226,7 @@ def main():
     roof = {"bound": "hbm", "achieved": f32["alg_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": f32["alg_GBs"] / HBM_PEAK_GBS,
             "traffic": (traffic or {}).get("bytes_per_launch"),
-            "alg_bytes_per_launch": f32["alg_bytes"], "kernel": "k_hv_chain_mlp<4, 2, float, real-phase, uint32>",
+            "alg_bytes_per_launch": f32["alg_bytes"], "kernel": "k_hv_chain_nn<4, float, real-phase, uniform, uint32>",
             "avg_launch_ms": f32["ms"]}
     out = {
         "metric": METRIC,

@@ -597,6 +597,17 @@ KFn chain_mlp_kernel(int N, int prec, bool realph, int deg, bool key32) {
   }
 }
 
+// Resident blocks per CU for a kernel at this LDS size (cached per kernel and size).
+int blocks_per_cu(KFn f, size_t lds) {
+  static std::vector<std::pair<std::pair<void *, size_t>, int>> cache;
+  for (auto &e : cache)
+    if (e.first.first == (void *)f && e.first.second == lds) return e.second;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, kThreads, lds) != hipSuccess || nb <= 0) nb = 8;
+  cache.push_back({{(void *)f, lds}, nb});
+  return nb;
+}
+
 int persistent_grid(int64_t ntiles) {
   static int cus = 0;
   if (!cus) {
@@ -605,6 +616,18 @@ int persistent_grid(int64_t ntiles) {
     if (cus <= 0) cus = 256;
   }
   int64_t g = std::min<int64_t>((int64_t)cus * 8, kMaxBlocks);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(g, ntiles));
+}
+
+// Persistent grid that is resident all at once: CUs x resident blocks per CU.
+int resident_grid(KFn f, size_t lds, int64_t ntiles) {
+  static int cus = 0;
+  if (!cus) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device()) == hipSuccess) cus = p.multiProcessorCount;
+    if (cus <= 0) cus = 256;
+  }
+  int64_t g = std::min<int64_t>((int64_t)cus * blocks_per_cu(f, lds), kMaxBlocks);
   return (int)std::max<int64_t>(1, std::min<int64_t>(g, ntiles));
 }
 
@@ -672,7 +695,7 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
     return;
   }
   HvArgs a = make_args(s, v0, v1, scale_dev, dot_partials);
-  const int grid = persistent_grid(a.ntiles);
+  int grid = persistent_grid(a.ntiles);
   if (nblocks_out) *nblocks_out = grid;
   const bool normlist = s->d_rclass.p == nullptr;  // sectors from caller lists carry list_r only
   const int pick = options().hv_kernel;
@@ -681,6 +704,8 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
     const bool uniform = s->model->bt_jxy2.size() == 1;
     KFn f = (lds <= 64 * 1024) ? chain_nn_kernel(s->nod, prec, s->real_phase, uniform, key32(s)) : nullptr;
     if (f) {
+      grid = resident_grid(f, lds, a.ntiles);
+      if (nblocks_out) *nblocks_out = grid;
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), lds, st, a);
       QS3X_HIP(hipGetLastError());
       return;
@@ -689,6 +714,8 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   if (pick <= 1 && use_chain(s, prec) && !normlist) {
     KFn f = chain_mlp_kernel(s->nod, prec, s->real_phase, s->model->max_deg, key32(s));
     if (f) {
+      grid = resident_grid(f, chain_lds_bytes(s, prec), a.ntiles);
+      if (nblocks_out) *nblocks_out = grid;
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, prec), st, a);
       QS3X_HIP(hipGetLastError());
       return;
@@ -697,6 +724,8 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   if (pick <= 2 && use_chain(s, prec) && !(normlist && prec != QS3X_FP64)) {
     KFn f = chain_kernel(s->nod, prec, s->real_phase, normlist, false, key32(s));
     if (f) {
+      grid = resident_grid(f, chain_lds_bytes(s, prec), a.ntiles);
+      if (nblocks_out) *nblocks_out = grid;
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, prec), st, a);
       QS3X_HIP(hipGetLastError());
       return;

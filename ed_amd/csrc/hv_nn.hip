@@ -106,52 +106,65 @@ struct NnCtx {
   const uint16_t *s_bt;
 };
 
-// Index reads, gathers, accumulation, store and dot partial of one row whose
-// H hop slots hold (anchored rank or -1, coefficient, phase index).
-template <int N, class R, bool REALPH>
-__device__ __forceinline__ void finish_row(const HvArgs &a, int64_t row, typename CT<R>::T xin, const int *hc,
-                                           const R *hcf, const int *hei, R anti, R scale, double &dre,
-                                           double &dim_) {
+// Index reads, gathers, accumulation, store and dot partial of RPT rows whose
+// H hop slots hold (anchored rank or -1, coefficient, phase index).  Rows with
+// store[k] false are skipped (their slots must be -1).  All index reads of the
+// RPT rows are issued before any gather.
+template <int N, class R, bool REALPH, int RPT>
+__device__ __forceinline__ void finish_rows(const HvArgs &a, const int64_t *row, const typename CT<R>::T *xin,
+                                            const bool *store, const int (*hc)[2 * N], const R (*hcf)[2 * N],
+                                            const int (*hei)[2 * N], const R *anti, R scale, double &dre,
+                                            double &dim_) {
   using C = typename CT<R>::T;
   constexpr int H = 2 * N;
   const C *__restrict__ v1 = (const C *)a.v1;
-  uint2 wd[H];
+  uint2 wd[RPT][H];
 #pragma unroll
-  for (int h = 0; h < H; ++h) wd[h] = a.bitmap[(uint32_t)(hc[h] >= 0 ? hc[h] : 0) >> 5];
-  int hid[H];
+  for (int k = 0; k < RPT; ++k)
 #pragma unroll
-  for (int h = 0; h < H; ++h) {
-    const uint32_t b = (uint32_t)hc[h] & 31u;
-    const bool in = hc[h] >= 0 && ((wd[h].x >> b) & 1u);
-    hid[h] = in ? (int)wd[h].y + __popc(wd[h].x & ((1u << b) - 1u)) : -1;
-  }
-  C y[H];
+    for (int h = 0; h < H; ++h) wd[k][h] = a.bitmap[(uint32_t)(hc[k][h] >= 0 ? hc[k][h] : 0) >> 5];
+  int hid[RPT][H];
 #pragma unroll
-  for (int h = 0; h < H; ++h) y[h] = v1[hid[h] >= 0 ? (int64_t)hid[h] : row];
-  R accx = 0, accy = 0;
+  for (int k = 0; k < RPT; ++k)
 #pragma unroll
-  for (int h = 0; h < H; ++h) {
-    const R cf = (hid[h] >= 0) ? hcf[h] : (R)0;
-    if constexpr (REALPH) {
-      accx += cf * y[h].x;
-      accy += cf * y[h].y;
-    } else {
-      const R pr = (R)a.explist[2 * hei[h]] * cf, pi = (R)a.explist[2 * hei[h] + 1] * cf;
-      accx += pr * y[h].x - pi * y[h].y;
-      accy += pr * y[h].y + pi * y[h].x;
+    for (int h = 0; h < H; ++h) {
+      const uint32_t b = (uint32_t)hc[k][h] & 31u;
+      const bool in = hc[k][h] >= 0 && ((wd[k][h].x >> b) & 1u);
+      hid[k][h] = in ? (int)wd[k][h].y + __popc(wd[k][h].x & ((1u << b) - 1u)) : -1;
     }
-  }
-  C x = xin;
-  x.x *= scale;
-  x.y *= scale;
-  const R dg = (sizeof(R) == 8 ? (R)a.ezz : (R)0) - (R)2 * anti;
-  C out;
-  out.x = dg * x.x + accx * scale;
-  out.y = dg * x.y + accy * scale;
-  ((C *)a.v0)[row] = out;
-  if (a.dot_partials) {
-    dre += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
-    dim_ += (double)x.x * (double)out.y - (double)x.y * (double)out.x;
+  C y[RPT][H];
+#pragma unroll
+  for (int k = 0; k < RPT; ++k)
+#pragma unroll
+    for (int h = 0; h < H; ++h) y[k][h] = v1[hid[k][h] >= 0 ? (int64_t)hid[k][h] : row[k]];
+#pragma unroll
+  for (int k = 0; k < RPT; ++k) {
+    if (!store[k]) continue;
+    R accx = 0, accy = 0;
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      const R cf = (hid[k][h] >= 0) ? hcf[k][h] : (R)0;
+      if constexpr (REALPH) {
+        accx += cf * y[k][h].x;
+        accy += cf * y[k][h].y;
+      } else {
+        const R pr = (R)a.explist[2 * hei[k][h]] * cf, pi = (R)a.explist[2 * hei[k][h] + 1] * cf;
+        accx += pr * y[k][h].x - pi * y[k][h].y;
+        accy += pr * y[k][h].y + pi * y[k][h].x;
+      }
+    }
+    C x = xin[k];
+    x.x *= scale;
+    x.y *= scale;
+    const R dg = (sizeof(R) == 8 ? (R)a.ezz : (R)0) - (R)2 * anti[k];
+    C out;
+    out.x = dg * x.x + accx * scale;
+    out.y = dg * x.y + accy * scale;
+    ((C *)a.v0)[row[k]] = out;
+    if (a.dot_partials) {
+      dre += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
+      dim_ += (double)x.x * (double)out.y - (double)x.y * (double)out.x;
+    }
   }
 }
 
@@ -202,14 +215,74 @@ __device__ __forceinline__ void slow_row(const HvArgs &a, const NnCtx<N, R> &k, 
     hcf[h] = k.s_jxy2[bt] * k.s_rcl[count] * rinv;
     hei[h] = ei;
   }
-  finish_row<N, R, REALPH>(a, row, xin, hc, hcf, hei, anti, scale, dre, dim_);
+  const bool store = true;
+  finish_rows<N, R, REALPH, 1>(a, &row, &xin, &store, &hc, &hcf, &hei, &anti, scale, dre, dim_);
 }
 
+// Hop slots of a clear row (see the file comment); returns false, leaving
+// the slots untouched, when the row needs the key search.
+template <int N, class R, bool REALPH, bool UNI>
+__device__ __forceinline__ bool clear_slots(const NnCtx<N, R> &k, int bits, uint64_t w, R jxy2u, R jz4u, int ei_ar,
+                                            int ei_al, int *hc, R *hcf, int *hei, R &anti) {
+  constexpr int H = 2 * N;
+  const int L = k.L;
+  const int32_t *s_binom = k.s_binom;
+  int n[N];
+#pragma unroll
+  for (int q = 0; q < N; ++q) n[q] = (int)((w >> (q * bits)) & ((1ull << bits) - 1ull));
+  int g[N];
+#pragma unroll
+  for (int q = 0; q < N - 1; ++q) g[q] = n[q + 1] - n[q];
+  const int W = L + 1 - n[N - 1];
+  g[N - 1] = W;
+  int M = 0;
+#pragma unroll
+  for (int q = 0; q < N - 1; ++q) M = max(M, g[q]);
+  if (!(M <= W - 3)) return false;
+  const R cfc = k.s_cfc[(int)(w >> 56)];  // r(1) / r(row)
+  int crow = 0, c_ar = 0, c_al = 0;
+#pragma unroll
+  for (int q = 1; q < N; ++q) {
+    const int base = (q - 1) * (L - 1) + n[q];
+    crow += s_binom[base - 2];
+    c_ar += s_binom[max(base - 3, (q - 1) * (L - 1))];  // C(n_q - 3, q); clamped when n_1 = 2
+    c_al += s_binom[base - 1];
+  }
+  R an = 0;
+  int nanti = 0;
+#pragma unroll
+  for (int h = 0; h < H; ++h) {
+    const int m = h >> 1;
+    const bool right = (h & 1) == 0;
+    const bool ok = right ? (g[m] > 1) : ((m == 0 ? W : g[m - 1]) > 1);
+    int c;
+    if (m == 0) c = right ? c_ar : c_al;
+    else if (m == 1) c = crow + (right ? 1 : -1);
+    else c = right ? crow + s_binom[(m - 2) * (L - 1) + n[m] - 2] : crow - s_binom[(m - 2) * (L - 1) + n[m] - 3];
+    hc[h] = ok ? c : -1;
+    if constexpr (UNI) {
+      nanti += ok ? 1 : 0;
+      hcf[h] = jxy2u * cfc;
+    } else {
+      const int bt = k.s_bt[right ? n[m] : (m == 0 ? L : n[m] - 1)];
+      an += ok ? k.s_jz4[bt] : (R)0;
+      hcf[h] = k.s_jxy2[bt] * cfc;
+    }
+    if constexpr (!REALPH) hei[h] = (m == 0) ? (right ? ei_ar : ei_al) : 0;
+  }
+  anti = UNI ? jz4u * (R)nanti : an;
+  return true;
+}
+
+// RPT rows per lane per iteration.  One: two rows per lane (more loads in
+// flight per wavefront) measured the same at L = 1000 once the grid matches
+// the kernel's occupancy, and costs registers.
 template <int N, class R, bool REALPH, bool UNI, class KeyT>
 __global__ __launch_bounds__(kThreads) void k_hv_chain_nn(HvArgs a) {
   static_assert(N >= 2, "N_down = 1 uses the general chain kernel");
   using C = typename CT<R>::T;
   constexpr int H = 2 * N;  // hop slots: 2m = spin m to the right, 2m + 1 = to the left
+  constexpr int RPT = 1;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int32_t s_queue[kThreads / 64][kQueue];
   const int L = a.L;
@@ -247,87 +320,69 @@ __global__ __launch_bounds__(kThreads) void k_hv_chain_nn(HvArgs a) {
   int32_t *queue = s_queue[wid];
   int qhead = 0, qcount = 0;  // wave-uniform
   double dre = 0.0, dim_ = 0.0;
+  const int64_t nsuper = (a.dim + (int64_t)kThreads * RPT - 1) / ((int64_t)kThreads * RPT);
 
-  for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
-    const int64_t row = tile * kThreads + threadIdx.x;
-    const bool valid = row < a.dim;
-    bool clear = false;
-    if (valid) {
-      const uint64_t w = a.cfg[row];
-      const C xin = v1[row];
-      int n[N];
+  for (int64_t t = blockIdx.x; t < nsuper; t += gridDim.x) {
+    int64_t row[RPT];
+    bool valid[RPT], clear[RPT];
+    uint64_t w[RPT];
+    C xin[RPT];
 #pragma unroll
-      for (int q = 0; q < N; ++q) n[q] = (int)((w >> (q * a.bits)) & ((1ull << a.bits) - 1ull));
-      int g[N];
+    for (int r = 0; r < RPT; ++r) {
+      row[r] = (t * RPT + r) * kThreads + threadIdx.x;
+      valid[r] = row[r] < a.dim;
+      w[r] = valid[r] ? a.cfg[row[r]] : 0ull;
+      if (valid[r]) xin[r] = v1[row[r]];
+      else xin[r].x = xin[r].y = (R)0;
+    }
+    int hc[RPT][H], hei[RPT][REALPH ? 1 : H];
+    R hcf[RPT][H], anti[RPT];
 #pragma unroll
-      for (int q = 0; q < N - 1; ++q) g[q] = n[q + 1] - n[q];
-      const int W = L + 1 - n[N - 1];
-      g[N - 1] = W;
-      int M = 0;
-#pragma unroll
-      for (int q = 0; q < N - 1; ++q) M = max(M, g[q]);
-      clear = M <= W - 3;
-      if (clear) {
-        const R cfc = s_cfc[(int)(w >> 56)];  // r(1) / r(row)
-        int crow = 0, c_ar = 0, c_al = 0;
-#pragma unroll
-        for (int q = 1; q < N; ++q) {
-          const int base = (q - 1) * (L - 1) + n[q];
-          crow += s_binom[base - 2];
-          c_ar += s_binom[max(base - 3, (q - 1) * (L - 1))];  // C(n_q - 3, q); clamped when n_1 = 2
-          c_al += s_binom[base - 1];
-        }
-        int hc[H], hei[REALPH ? 1 : H];
-        R hcf[H];
-        R anti = 0;
-        int nanti = 0;
+    for (int r = 0; r < RPT; ++r) {
+      clear[r] = valid[r] && clear_slots<N, R, REALPH, UNI>(k, a.bits, w[r], jxy2u, jz4u, ei_ar, ei_al, hc[r],
+                                                             hcf[r], REALPH ? nullptr : hei[r], anti[r]);
+      if (!clear[r]) {
 #pragma unroll
         for (int h = 0; h < H; ++h) {
-          const int m = h >> 1;
-          const bool right = (h & 1) == 0;
-          const bool ok = right ? (g[m] > 1) : ((m == 0 ? W : g[m - 1]) > 1);
-          int c;
-          if (m == 0) c = right ? c_ar : c_al;
-          else if (m == 1) c = crow + (right ? 1 : -1);
-          else c = right ? crow + s_binom[(m - 2) * (L - 1) + n[m] - 2] : crow - s_binom[(m - 2) * (L - 1) + n[m] - 3];
-          hc[h] = ok ? c : -1;
-          if constexpr (UNI) {
-            nanti += ok ? 1 : 0;
-            hcf[h] = jxy2u * cfc;
-          } else {
-            const int bt = s_bt[right ? n[m] : (m == 0 ? L : n[m] - 1)];
-            anti += ok ? s_jz4[bt] : (R)0;
-            hcf[h] = s_jxy2[bt] * cfc;
-          }
-          if constexpr (!REALPH) hei[h] = (m == 0) ? (right ? ei_ar : ei_al) : 0;
+          hc[r][h] = -1;
+          hcf[r][h] = (R)0;
+          if constexpr (!REALPH) hei[r][h] = 0;
         }
-        if constexpr (UNI) anti = jz4u * (R)nanti;
-        finish_row<N, R, REALPH>(a, row, xin, hc, hcf, hei, anti, scale, dre, dim_);
+        anti[r] = (R)0;
       }
     }
-    // queue the rows that need the key search (order: lane order within the wavefront)
-    const bool needs = valid && !clear;
-    const uint64_t mask = __ballot(needs);
-    if (mask != 0ull) {
-      const int before = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-      if (needs) queue[(qhead + qcount + before) & (kQueue - 1)] = (int32_t)row;
-      qcount += __popcll(mask);
-      __builtin_amdgcn_wave_barrier();
-      if (qcount >= 64) {
-        const int64_t r = queue[(qhead + lane) & (kQueue - 1)];
+    if constexpr (REALPH) {
+      int hz[RPT][H];
+      finish_rows<N, R, REALPH, RPT>(a, row, xin, clear, hc, hcf, hz, anti, scale, dre, dim_);
+    } else {
+      finish_rows<N, R, REALPH, RPT>(a, row, xin, clear, hc, hcf, hei, anti, scale, dre, dim_);
+    }
+    // queue the rows that need the key search (order: slice, then lane)
+#pragma unroll
+    for (int r = 0; r < RPT; ++r) {
+      const bool needs = valid[r] && !clear[r];
+      const uint64_t mask = __ballot(needs);
+      if (mask != 0ull) {
+        const int before = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        if (needs) queue[(qhead + qcount + before) & (kQueue - 1)] = (int32_t)row[r];
+        qcount += __popcll(mask);
         __builtin_amdgcn_wave_barrier();
-        qhead = (qhead + 64) & (kQueue - 1);
-        qcount -= 64;
-        slow_row<N, R, REALPH, UNI, KeyT>(a, k, r, scale, dre, dim_);
+        if (qcount >= 64) {
+          const int64_t rr = queue[(qhead + lane) & (kQueue - 1)];
+          __builtin_amdgcn_wave_barrier();
+          qhead = (qhead + 64) & (kQueue - 1);
+          qcount -= 64;
+          slow_row<N, R, REALPH, UNI, KeyT>(a, k, rr, scale, dre, dim_);
+        }
       }
     }
   }
   if (qcount > 0) {
     __builtin_amdgcn_wave_barrier();
     if (lane < qcount) {
-      const int64_t r = queue[(qhead + lane) & (kQueue - 1)];
-      slow_row<N, R, REALPH, UNI, KeyT>(a, k, r, scale, dre, dim_);
+      const int64_t rr = queue[(qhead + lane) & (kQueue - 1)];
+      slow_row<N, R, REALPH, UNI, KeyT>(a, k, rr, scale, dre, dim_);
     }
   }
   if (a.dot_partials) block_reduce_store(dre, dim_, a.dot_partials);
