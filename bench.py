@@ -13,6 +13,7 @@ H.v kernel and the CPU baseline (the oracle's literal restatement of
 ham_to_vec_wave_vector timed on a bounded sample of rows).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu]
+       python bench.py --dsf [--dsf-steps 200]   (batched 64-q DSF workload, its own JSON line)
 """
 import argparse
 import json
@@ -84,6 +85,75 @@ def calibrate_copy(nbytes=1 << 30, reps=5):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def dsf_workload(args):
+    """Dynamical structure factor S^{-+}(q, w) of the L = 1000 NN Heisenberg
+    chain: the lowest k = 0 state of N_down = 3 (D = 166,167), S^-_q into the
+    N_down = 4, k = q sectors for 64 q = 2 pi n_m / 1000, n_m = round(1000 m / 64),
+    and 64 continued fractions of `--dsf-steps` Lanczos steps, all 64 in one
+    batched block (FP32 vectors, FP64 coefficients).  Reports the batched time,
+    the per-sector time for a sample of q (extrapolated to 64), and the error
+    of the FP32 spectra against the FP64 batch (Lorentzian eta = 0.02, 2001
+    points of w in [0, 5])."""
+    import torch
+    from ed_amd import qs3 as q3
+    L, Ngs = 1000, 3
+    gm = q3.Model(L, L, 1, 1, q3.chain_bonds(L, [(1, 1.0, 1.0)]))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gs = q3.Sector.build(gm, Ngs, (0.0, 0.0, 0.0))
+    E0, psi, it = q3.lanczos_routines(gs, maxitr=3000, lnc_ene_conv=1e-13, prec=q3.FP64, seed=1)
+    t_gs = time.perf_counter() - t0
+    log(f"ground state N_down={Ngs} k=0: D={gs.dim}, E0={E0:.12f}, {it} steps, {t_gs:.2f} s")
+    pos = np.array([[i, 0.0, 0.0] for i in range(L)], dtype=float)
+    Ks = [int(round(1000 * m / 64)) for m in range(64)]
+    steps = args.dsf_steps
+    out = {}
+    for prec, name in ((q3.FP32, "fp32"), (q3.FP64, "fp64")):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        a, b, n, st = q3.dsf_sminus_batch(gs, Ks, pos, psi, steps, prec=prec)
+        torch.cuda.synchronize()
+        out[name] = (a, b, n, st, time.perf_counter() - t0)
+        log(f"batch {name}: {out[name][4]:.2f} s, {st}")
+    a32, b32, n32, st32, t32 = out["fp32"]
+    a64, b64, n64, st64, t64 = out["fp64"]
+    om = np.linspace(0.0, 5.0, 2001)
+    errs = []
+    for j in range(64):
+        if n64[j] <= 0:
+            continue
+        s64 = q3.dsf_analysis(a64[j], b64[j], n64[j], E0, om, 0.02)
+        s32 = q3.dsf_analysis(a32[j], b32[j], n32[j], E0, om, 0.02)
+        scale = max(np.max(np.abs(s64)), 1e-300)
+        errs.append(np.max(np.abs(s32 - s64)) / scale)
+    errs = np.array(errs)
+    # per-sector path for a sample of q, FP32 (sector build + S^-_q + recurrence)
+    sample = [1, 17, 40]
+    tps = []
+    for j in sample:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rk = q3.wavevector(Ks[j], L)
+        tgt = q3.Sector.build(gm, Ngs + 1, (rk, 0.0, 0.0))
+        f = q3.calcu_Sq_a(2, gs, tgt, np.array([rk, 0.0, 0.0]), pos, psi)
+        q3.dsf_recurrence(tgt, f, steps, prec=q3.FP32)
+        torch.cuda.synchronize()
+        tps.append(time.perf_counter() - t0)
+        del tgt, f
+    per_sector = float(np.mean(tps)) * 64
+    res = {"metric": "DSF S^{-+}(q,w): 64 q x %d-step continued fractions, L=1000 N_down=3->4" % steps,
+           "value": t32, "unit": "s", "higher_is_better": False, "dtype": "complex-fp32 (fp64 coefficients)",
+           "data": "synthetic", "config": {"workload": "periodic NN Heisenberg chain L=1000, ground state N_down=3 "
+                                                       "k=0, S^-_q into N_down=4 k=q (D~41.4M each), 64 q",
+                                           "steps": steps, "eta": 0.02, "omega_points": 2001},
+           "seconds_fp32_batch": t32, "seconds_fp64_batch": t64, "ms_per_block_step_fp32": st32["ms_per_block_step"],
+           "ms_per_block_step_fp64": st64["ms_per_block_step"], "seconds_per_sector_fp32_extrapolated": per_sector,
+           "per_sector_sample_seconds": tps, "speedup_batch_vs_per_sector": per_sector / t32,
+           "fp32_vs_fp64_S_rel_err_median": float(np.median(errs)), "fp32_vs_fp64_S_rel_err_max": float(np.max(errs)),
+           "fp32_vs_fp64_S_rel_err_q_counted": int(len(errs)), "ground_state_seconds": t_gs, "E0": E0}
+    print(json.dumps(res), flush=True)
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "hv_chain_L1000_N4_fp32_traffic.json")
     if os.path.exists(p):
@@ -107,6 +177,8 @@ def main():
     ap.add_argument("--conv-plain", type=float, default=1e-11)
     ap.add_argument("--conv-mixed", type=float, default=1e-10)
     ap.add_argument("--calibrate-copy", action="store_true", help="only run a 1 GiB device copy (counter scaling)")
+    ap.add_argument("--dsf", action="store_true", help="run the batched 64-q DSF workload instead of the headline")
+    ap.add_argument("--dsf-steps", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X; multi-GPU runs are not part of this build "
@@ -115,6 +187,9 @@ def main():
 
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.dsf:
+        dsf_workload(args)
         return
 
     import torch

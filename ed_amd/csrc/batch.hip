@@ -1,0 +1,848 @@
+// batch.hip -- the DSF recurrence for many wave vectors in one block.
+//
+// The reference runs calcu_DSF_wavevector (get_expectation_values.f90:379-605)
+// once per transferred wave vector q: S^-_q |psi0> into the k = q sector of
+// N_down + 1, then itr_dsf Lanczos steps without reorthogonalisation.  Here
+// up to 64 wave vectors run together:
+//
+// * One "all-k" sector holds every translation representative of the
+//   target N_down (a chain), in the reference's row order.  The k = q sector
+//   is the subset of rows whose stabiliser class has ro_q > 1e-15
+//   (state_lists.f90:244-292), so a vector of the k = q sector is a vector of
+//   the all-k sector that is zero on the other rows, and H_q maps it to one
+//   that is again zero there (hops into excluded rows carry r_q = 0).
+// * The 64 vectors are interleaved row-major, [row][q]: the hop structure of
+//   a row (targets, ell, stabiliser orders) is computed once and applied to
+//   all 64 vectors, and every gather reads one contiguous 64-element row.
+// * Block H.v (nearest-neighbour chains): a wavefront takes 64 rows; with
+//   lanes = rows it computes each row's hops (closed forms of hv_nn.hip, key
+//   search for near-tie rows) into LDS; with lanes = q it then applies them:
+//   y_q(row) = (diag) x_q(row) + sum_hops Jxy/2 r_q(ct)/r_q(cr) explist_q(ell) x_q(target).
+// * S^-_q |psi0> (calcu_Sq_a, get_expectation_values.f90:281-377, pull form)
+//   is built the same way: lanes = rows find the ground-state rows that feed
+//   each target row, lanes = q weight them with exp(-i q R_i) and
+//   conj(explist_q(ell)).
+// * The recurrence keeps per-q scalars in one device array [slot][q].
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "blas.h"
+#include "hv_common.h"
+#include "qs3x_internal.h"
+#include "rep.h"
+
+namespace qs3x {
+
+namespace {
+
+using namespace hvk;
+
+constexpr int kQ = 64;         // vectors per block, one per lane
+constexpr int kBThreads = 256;  // 4 wavefronts
+constexpr int kBWaves = kBThreads / 64;
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <class F>
+int guarded4(F &&f) {
+  try {
+    f();
+    return 0;
+  } catch (const Error &e) {
+    set_error(e.msg);
+    return 1;
+  } catch (const std::exception &e) {
+    set_error(std::string("qs3x: ") + e.what());
+    return 2;
+  }
+}
+
+// --------------------------------------------------------------- block H.v
+struct BlockHvArgs {
+  const uint64_t *cfg;
+  const uint2 *bitmap;
+  int64_t dim;
+  int L, bits, kbits;
+  const int64_t *binom;
+  int kdim;
+  const uint16_t *nn_bt;
+  const double *bt;  // interleaved jxy2, jz4
+  int nbt;
+  double shift;       // diagonal constant in this precision (e_zz for FP64, 0 for FP32)
+  const void *x;      // [dim][kQ]
+  void *y;            // [dim][kQ]
+  const double *sc;   // per-q scalars [slot][kQ]
+  int iscale;         // slot of the scale of x (< 0: 1)
+  const double *ratio;    // [N+1][N+1][kQ]: r_q(ct) / r_q(cr), 0 when r_q(cr) = 0
+  const double *rowmask;  // [N+1][kQ]: 1 when r_q(c) > 0
+  const void *eph;        // [L+1][kQ] complex in precision R: explist_q(ell, 1, 1); entry 0 = 1
+  double *dot_partials;   // [grid][kQ] complex
+};
+
+// C(x, q) for 1 <= q <= N - 1 from the LDS table.
+__device__ __forceinline__ int bnm(const int32_t *s_binom, int L, int q, int x) {
+  return s_binom[(q - 1) * (L - 1) + x];
+}
+
+// Representative of a sorted chain configuration: anchored rank, stabiliser
+// order, ell_x (0 when nn already is the representative).
+template <int N, class KeyT>
+__device__ __forceinline__ void chain_search_b(const int *nn, int L, int kb, const int32_t *s_binom, int &c,
+                                               int &count, int &ell) {
+  const KeyT fmask = (KeyT)((1ull << kb) - 1ull);
+  int g[N];
+#pragma unroll
+  for (int q = 0; q < N - 1; ++q) g[q] = nn[q + 1] - nn[q];
+  g[N - 1] = nn[0] + L - nn[N - 1];
+  KeyT K[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    KeyT k = 0;
+#pragma unroll
+    for (int tt = 1; tt < N; ++tt) k = (KeyT)((k << kb) | (KeyT)g[(j - tt + N) % N]);
+    K[j] = k;
+  }
+  KeyT kmax = K[0];
+#pragma unroll
+  for (int j = 1; j < N; ++j) kmax = K[j] > kmax ? K[j] : kmax;
+  int mx = 0;
+  count = 0;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const bool tie = (K[j] == kmax);
+    count += tie ? 1 : 0;
+    mx = (tie && nn[j] > mx) ? nn[j] : mx;
+  }
+  const bool isrep = (nn[0] == 1) && (K[0] == kmax);
+  ell = isrep ? 0 : (L + 1 - mx);
+  int f[N - 1];
+  int sum = 0;
+#pragma unroll
+  for (int tt = 1; tt < N; ++tt) {
+    f[tt - 1] = (int)((kmax >> (kb * (N - 1 - tt))) & fmask);
+    sum += f[tt - 1];
+  }
+  int P = L - sum;
+  c = s_binom[P - 1];
+#pragma unroll
+  for (int mm = 2; mm < N; ++mm) {
+    P += f[N - mm];
+    c += s_binom[(mm - 1) * (L - 1) + (P - 1)];
+  }
+}
+
+// Hops of one row of a nearest-neighbour chain: for slot h (2m: spin m to
+// the right, 2m + 1: to the left) the anchored rank of the image's
+// representative (-1: no hop), its stabiliser order, ell_x and the bond site
+// (bond (bsite, bsite + 1)).  Closed forms for clear rows (hv_nn.hip), the key
+// search otherwise.
+template <int N, class KeyT>
+__device__ __forceinline__ void nn_row_hops(const int *n, int L, int kb, const int32_t *s_binom, int *hc, int *hcnt,
+                                            int *hell, int *hbs) {
+  constexpr int H = 2 * N;
+  int g[N];
+#pragma unroll
+  for (int q = 0; q < N - 1; ++q) g[q] = n[q + 1] - n[q];
+  const int W = L + 1 - n[N - 1];
+  g[N - 1] = W;
+  int M = 0;
+#pragma unroll
+  for (int q = 0; q < N - 1; ++q) M = max(M, g[q]);
+  const bool clear = M <= W - 3;
+  int crow = 0, c_ar = 0, c_al = 0;
+#pragma unroll
+  for (int q = 1; q < N; ++q) {
+    const int base = (q - 1) * (L - 1) + n[q];
+    crow += s_binom[base - 2];
+    c_ar += s_binom[max(base - 3, (q - 1) * (L - 1))];
+    c_al += s_binom[min(base - 1, (q - 1) * (L - 1) + L - 2)];
+  }
+#pragma unroll
+  for (int h = 0; h < H; ++h) {
+    const int m = h >> 1;
+    const bool right = (h & 1) == 0;
+    const bool ok = right ? (g[m] > 1) : ((m == 0 ? W : g[m - 1]) > 1);
+    hbs[h] = right ? n[m] : (m == 0 ? L : n[m] - 1);
+    hcnt[h] = 1;
+    hell[h] = 0;
+    if (!ok) {
+      hc[h] = -1;
+      continue;
+    }
+    if (clear) {
+      if (m == 0) {
+        hc[h] = right ? c_ar : c_al;
+        hell[h] = right ? L - 1 : 1;
+      } else if (m == 1) {
+        hc[h] = crow + (right ? 1 : -1);
+      } else {
+        hc[h] = right ? crow + bnm(s_binom, L, m - 1, n[m] - 2) : crow - bnm(s_binom, L, m - 1, n[m] - 3);
+      }
+    } else {
+      int nn[N];
+      if (m == 0 && right) {
+        nn[0] = 2;
+#pragma unroll
+        for (int q = 1; q < N; ++q) nn[q] = n[q];
+      } else if (m == 0) {
+#pragma unroll
+        for (int q = 0; q < N - 1; ++q) nn[q] = n[q + 1];
+        nn[N - 1] = L;
+      } else {
+#pragma unroll
+        for (int q = 0; q < N; ++q) nn[q] = n[q] + ((q == m) ? (right ? 1 : -1) : 0);
+      }
+      int c, cnt, ell;
+      chain_search_b<N, KeyT>(nn, L, kb, s_binom, c, cnt, ell);
+      hc[h] = c;
+      hcnt[h] = cnt;
+      hell[h] = ell;
+    }
+  }
+}
+
+template <class R>
+__device__ __forceinline__ typename CT<R>::T cload(const void *p, int64_t i) {
+  return ((const typename CT<R>::T *)p)[i];
+}
+
+// y = H_q (s_q x) for the kQ interleaved vectors (see the file comment).
+template <int N, class R, class KeyT>
+__global__ __launch_bounds__(kBThreads) void k_hv_block(BlockHvArgs a) {
+  using C = typename CT<R>::T;
+  constexpr int H = 2 * N;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int L = a.L;
+  const int nb = (N - 1) * (L - 1);
+  // LDS: binom int32 [nb] | ratio R [(N+1)^2][kQ] | rowmask R [N+1][kQ] | jxy2 R [nbt] | jz4 R [nbt] |
+  //      per wave: hop id int32 [64][H], hop aux int32 [64][H], row class int32 [64], anti R [64]
+  int32_t *s_binom = (int32_t *)smem;
+  R *s_ratio = (R *)(smem + (((size_t)nb * 4 + 15) & ~(size_t)15));
+  R *s_mask = s_ratio + (N + 1) * (N + 1) * kQ;
+  R *s_jxy2 = s_mask + (N + 1) * kQ;
+  R *s_jz4 = s_jxy2 + a.nbt;
+  int32_t *s_wave = (int32_t *)(((uintptr_t)(s_jz4 + a.nbt) + 15) & ~(uintptr_t)15);
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const size_t wave_words = (size_t)64 * H * 2 + 64 + 64 * (sizeof(R) / 4);
+  int32_t *w_id = s_wave + wid * wave_words;
+  int32_t *w_aux = w_id + 64 * H;
+  int32_t *w_cls = w_aux + 64 * H;
+  R *w_anti = (R *)(w_cls + 64);
+  for (int q = threadIdx.x; q < nb; q += kBThreads) {
+    const int m = q / (L - 1) + 1, x = q % (L - 1);
+    s_binom[q] = (int32_t)a.binom[(int64_t)x * a.kdim + m];
+  }
+  for (int q = threadIdx.x; q < (N + 1) * (N + 1) * kQ; q += kBThreads) s_ratio[q] = (R)a.ratio[q];
+  for (int q = threadIdx.x; q < (N + 1) * kQ; q += kBThreads) s_mask[q] = (R)a.rowmask[q];
+  for (int q = threadIdx.x; q < a.nbt; q += kBThreads) {
+    s_jxy2[q] = (R)a.bt[2 * q];
+    s_jz4[q] = (R)a.bt[2 * q + 1];
+  }
+  __syncthreads();
+
+  const R scale = a.iscale >= 0 ? (R)a.sc[(size_t)a.iscale * kQ + lane] : (R)1;
+  // phases of the closed-form anchor hops (ell = 1 and L - 1) of this lane's q
+  const C e1 = cload<R>(a.eph, (size_t)1 * kQ + lane);
+  const C eL1 = cload<R>(a.eph, (size_t)(L - 1) * kQ + lane);
+  double dre = 0.0, dim_ = 0.0;
+  const int64_t nchunks = (a.dim + 63) / 64;
+  for (int64_t ch = (int64_t)blockIdx.x * kBWaves + wid; ch < nchunks; ch += (int64_t)gridDim.x * kBWaves) {
+    const int64_t base = ch * 64;
+    // lanes = rows: hop structure of the chunk
+    {
+      const int64_t row = base + lane;
+      int hc[H], hcnt[H], hell[H], hbs[H];
+      int cls = 0;
+      R anti = 0;
+      if (row < a.dim) {
+        const uint64_t w = a.cfg[row];
+        cls = (int)(w >> 56);
+        int n[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) n[q] = (int)((w >> (q * a.bits)) & ((1ull << a.bits) - 1ull));
+        nn_row_hops<N, KeyT>(n, L, a.kbits, s_binom, hc, hcnt, hell, hbs);
+      } else {
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+          hc[h] = -1;
+          hcnt[h] = 1;
+          hell[h] = 0;
+          hbs[h] = 1;
+        }
+      }
+      uint2 wd[H];
+#pragma unroll
+      for (int h = 0; h < H; ++h) wd[h] = a.bitmap[(uint32_t)(hc[h] >= 0 ? hc[h] : 0) >> 5];
+#pragma unroll
+      for (int h = 0; h < H; ++h) {
+        const uint32_t b = (uint32_t)hc[h] & 31u;
+        const bool in = hc[h] >= 0 && ((wd[h].x >> b) & 1u);
+        const int id = in ? (int)wd[h].y + __popc(wd[h].x & ((1u << b) - 1u)) : -1;
+        const int bt = a.nn_bt[hbs[h]];
+        if (hc[h] >= 0) anti += s_jz4[bt];
+        w_id[lane * H + h] = id;
+        w_aux[lane * H + h] = hcnt[h] | (hell[h] << 5) | (bt << 17);
+      }
+      w_cls[lane] = cls;
+      w_anti[lane] = anti;
+    }
+    __builtin_amdgcn_wave_barrier();
+    // lanes = vectors: apply the chunk's rows to the kQ vectors, two rows at a time
+    const int nr = (int)min<int64_t>(64, a.dim - base);
+    for (int r0 = 0; r0 < nr; r0 += 2) {
+      constexpr int RR = 2;
+      C xin[RR], y[RR][H];
+      int id[RR][H], aux[RR][H];
+#pragma unroll
+      for (int k = 0; k < RR; ++k) {
+        const int r = min(r0 + k, nr - 1);
+        xin[k] = cload<R>(a.x, (base + r) * kQ + lane);
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+          id[k][h] = w_id[r * H + h];
+          aux[k][h] = w_aux[r * H + h];
+        }
+#pragma unroll
+        for (int h = 0; h < H; ++h) y[k][h] = cload<R>(a.x, (int64_t)(id[k][h] >= 0 ? id[k][h] : base + r) * kQ + lane);
+      }
+#pragma unroll
+      for (int k = 0; k < RR; ++k) {
+        const int r = r0 + k;
+        if (r >= nr) break;
+        const int cr = w_cls[r];
+        R accx = 0, accy = 0;
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+          if (id[k][h] < 0) continue;
+          const int ct = aux[k][h] & 31, ell = (aux[k][h] >> 5) & 0xfff, bt = aux[k][h] >> 17;
+          const R cf = s_jxy2[bt] * s_ratio[(ct * (N + 1) + cr) * kQ + lane];
+          C e;
+          if (ell == 0) {
+            e.x = (R)1;
+            e.y = (R)0;
+          } else if (ell == 1) {
+            e = e1;
+          } else if (ell == L - 1) {
+            e = eL1;
+          } else {
+            e = cload<R>(a.eph, (size_t)ell * kQ + lane);
+          }
+          const R pr = cf * e.x, pi = cf * e.y;
+          accx += pr * y[k][h].x - pi * y[k][h].y;
+          accy += pr * y[k][h].y + pi * y[k][h].x;
+        }
+        C x = xin[k];
+        x.x *= scale;
+        x.y *= scale;
+        const R dg = (R)a.shift - (R)2 * w_anti[r];
+        const R mk = s_mask[cr * kQ + lane];
+        C out;
+        out.x = mk * (dg * x.x + accx * scale);
+        out.y = mk * (dg * x.y + accy * scale);
+        ((C *)a.y)[(base + r) * kQ + lane] = out;
+        dre += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
+        dim_ += (double)x.x * (double)out.y - (double)x.y * (double)out.x;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  // per-q block partials: sum the 4 wavefronts in fixed order
+  __shared__ double red[2][kBWaves][kQ];
+  red[0][wid][lane] = dre;
+  red[1][wid][lane] = dim_;
+  __syncthreads();
+  if (a.dot_partials && wid == 0) {
+    double sr = 0, si = 0;
+    for (int w = 0; w < kBWaves; ++w) {
+      sr += red[0][w][lane];
+      si += red[1][w][lane];
+    }
+    a.dot_partials[2 * ((size_t)blockIdx.x * kQ + lane)] = sr;
+    a.dot_partials[2 * ((size_t)blockIdx.x * kQ + lane) + 1] = si;
+  }
+}
+
+template <int N>
+size_t hv_block_lds(int L, int nbt, int prec) {
+  const size_t rs = prec == QS3X_FP64 ? 8 : 4;
+  const size_t nb = (size_t)(N - 1) * (L - 1);
+  size_t b = ((nb * 4 + 15) & ~(size_t)15) + rs * ((N + 1) * (N + 1) * kQ + (N + 1) * kQ + 2 * nbt);
+  b = (b + 15) & ~(size_t)15;
+  const size_t wave_words = (size_t)64 * 2 * N * 2 + 64 + 64 * (rs / 4);
+  return b + 4 * wave_words * kBWaves;
+}
+
+using BFn = void (*)(BlockHvArgs);
+
+template <int N>
+BFn pick_block(int prec, bool key32) {
+  if (prec == QS3X_FP64) return key32 ? k_hv_block<N, double, uint32_t> : k_hv_block<N, double, uint64_t>;
+  return key32 ? k_hv_block<N, float, uint32_t> : k_hv_block<N, float, uint64_t>;
+}
+
+BFn block_kernel(int N, int prec, bool key32, size_t *lds, int L, int nbt) {
+  switch (N) {
+#define BK(n)                                \
+  case n:                                    \
+    *lds = hv_block_lds<n>(L, nbt, prec);   \
+    return pick_block<n>(prec, key32)
+    BK(2); BK(3); BK(4); BK(5); BK(6); BK(7); BK(8);
+#undef BK
+    default: return nullptr;
+  }
+}
+
+// --------------------------------------------------------- block update
+// w_q -= a_q s_q u_q + b_q s'_q up_q (b term when HASB); part[block][q] = |w_q|^2
+template <class R, bool HASB>
+__global__ __launch_bounds__(kBThreads) void k_update3_block(typename CT<R>::T *w, const typename CT<R>::T *u,
+                                                             const typename CT<R>::T *up, int64_t dim,
+                                                             const double *sc, int ia, int isa, int ib, int isb,
+                                                             double *part) {
+  using C = typename CT<R>::T;
+  const int q = threadIdx.x & 63, rs = threadIdx.x >> 6;
+  const R ca = (R)(sc[(size_t)ia * kQ + q] * (isa >= 0 ? sc[(size_t)isa * kQ + q] : 1.0));
+  const R cb = HASB ? (R)(sc[(size_t)ib * kQ + q] * (isb >= 0 ? sc[(size_t)isb * kQ + q] : 1.0)) : (R)0;
+  double acc = 0;
+  constexpr int U = 4;
+  const int64_t step = (int64_t)gridDim.x * kBWaves;
+  for (int64_t r0 = (int64_t)blockIdx.x * kBWaves + rs; r0 < dim; r0 += step * U) {
+    C x[U], a[U], b[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t r = r0 + k * step;
+      if (r < dim) {
+        x[k] = w[r * kQ + q];
+        a[k] = u[r * kQ + q];
+        if (HASB) b[k] = up[r * kQ + q];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int64_t r = r0 + k * step;
+      if (r < dim) {
+        x[k].x -= ca * a[k].x;
+        x[k].y -= ca * a[k].y;
+        if (HASB) {
+          x[k].x -= cb * b[k].x;
+          x[k].y -= cb * b[k].y;
+        }
+        w[r * kQ + q] = x[k];
+        acc += (double)x[k].x * (double)x[k].x + (double)x[k].y * (double)x[k].y;
+      }
+    }
+  }
+  __shared__ double red[kBWaves][kQ];
+  red[rs][q] = acc;
+  __syncthreads();
+  if (rs == 0) {
+    double s = 0;
+    for (int k = 0; k < kBWaves; ++k) s += red[k][q];
+    part[2 * ((size_t)blockIdx.x * kQ + q)] = s;
+    part[2 * ((size_t)blockIdx.x * kQ + q) + 1] = 0.0;
+  }
+}
+
+// Per-q finalize (one block per q), modes as k_finalize in blas.hip:
+// 0: sc[i0] = re; 1: sc[i0] = re, sc[i1] = re + shift; 2: b = sqrt(re): sc[i0] = b, sc[i1] = 1/b (0 if b = 0)
+__global__ __launch_bounds__(kBThreads) void k_finalize_q(const double *part, int nb, double *sc, int mode, int i0,
+                                                          int i1, double shift) {
+  const int q = blockIdx.x;
+  double re = 0;
+  for (int b = threadIdx.x; b < nb; b += kBThreads) re += part[2 * ((size_t)b * kQ + q)];
+  __shared__ double red[kBThreads];
+  red[threadIdx.x] = re;
+  __syncthreads();
+  for (int s = kBThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double v = red[0];
+    if (mode == 0) {
+      sc[(size_t)i0 * kQ + q] = v;
+    } else if (mode == 1) {
+      sc[(size_t)i0 * kQ + q] = v;
+      if (i1 >= 0) sc[(size_t)i1 * kQ + q] = v + shift;
+    } else {
+      const double b = sqrt(v > 0 ? v : 0.0);
+      sc[(size_t)i0 * kQ + q] = b;
+      if (i1 >= 0) sc[(size_t)i1 * kQ + q] = b > 0 ? 1.0 / b : 0.0;
+    }
+  }
+}
+
+// ------------------------------------------------------------ block S^-_q
+// For target row lp (config m, NT down spins, an all-k chain row) and wave
+// vector q: f_q(lp) = r_q(m) / sqrt(NOS) sum over the distinct rotations y of
+// m that put a down spin on site 1, and over the down spins i of y with
+// y \ {i} a ground-state representative l:
+//     psi(l) / r_gs(l) * exp(-i q R_i) * conj(explist_q(ell_y)),
+// as sq_accumulate / k_sq (observables.hip) evaluate calcu_Sq_a for S^-.
+template <int NT, class R>
+__global__ __launch_bounds__(kBThreads) void k_sminus_block(SecView gs, SecView tg, const double2 *__restrict__ psi,
+                                                            const double *rq,    // [NT+1][kQ] r_q(c)
+                                                            const double *eqr,   // [L][kQ] complex exp(-i q R_i)
+                                                            const double *eph,   // [L+1][kQ] complex explist_q
+                                                            typename CT<R>::T *F, double norm, double *part) {
+  using C = typename CT<R>::T;
+  constexpr int T = NT * NT;  // terms per row at most
+  constexpr int CH = NT <= 4 ? 64 : (NT <= 6 ? 32 : 16);
+  __shared__ int32_t s_l[kBWaves][CH][T];
+  __shared__ int32_t s_aux[kBWaves][CH][T];  // site i | ell << 16
+  __shared__ int32_t s_cnt[kBWaves][CH];
+  __shared__ int32_t s_nt[kBWaves][CH];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int L = tg.L;
+  double acc2 = 0;
+  const int64_t nchunks = (tg.dim + CH - 1) / CH;
+  for (int64_t ch = (int64_t)blockIdx.x * kBWaves + wid; ch < nchunks; ch += (int64_t)gridDim.x * kBWaves) {
+    const int64_t base = ch * CH;
+    if (lane < CH) {
+      const int64_t lp = base + lane;
+      int nt = 0;
+      int cnt = 0;
+      if (lp < tg.dim) {
+        int m[kMaxNod], y[kMaxNod], yp[kMaxNod], src[kMaxNod];
+        sv_unpack(tg, lp, m);
+        cnt = (int)(tg.cfg[lp] >> 56);
+        for (int j = 0; j < NT; ++j) {
+          for (int q = 0; q < NT; ++q) {
+            const int s = m[(j + q) % NT] - m[j];
+            y[q] = 1 + (s < 0 ? s + L : s);
+          }
+          bool seen = false;
+          for (int j2 = 0; j2 < j && !seen; ++j2) {
+            bool same = true;
+            for (int q = 0; q < NT; ++q) {
+              const int s = m[(j2 + q) % NT] - m[j2];
+              yp[q] = 1 + (s < 0 ? s + L : s);
+              same = same && (yp[q] == y[q]);
+            }
+            seen = same;
+          }
+          if (seen) continue;
+          int ei = 0, isrep = 0;
+          double rt = 0;
+          (void)sv_lookup(tg, y, &ei, &isrep, &rt);
+          const int ell = ei ? ei - 3 * (L + 1) : 0;
+          for (int r = 0; r < NT; ++r) {
+            int p = 0;
+            for (int q = 0; q < NT; ++q)
+              if (q != r) src[p++] = y[q];
+            int gi = 0, grep = 0;
+            double grt = 0;
+            const int64_t l = sv_lookup(gs, src, &gi, &grep, &grt);
+            if (l < 0 || !grep) continue;
+            s_l[wid][lane][nt] = (int32_t)l;
+            s_aux[wid][lane][nt] = y[r] | (ell << 16);
+            ++nt;
+          }
+        }
+      }
+      s_nt[wid][lane] = nt;
+      s_cnt[wid][lane] = cnt;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int nr = (int)min<int64_t>(CH, tg.dim - base);
+    for (int r = 0; r < nr; ++r) {
+      const int nt = s_nt[wid][r];
+      double ar = 0, ai = 0;
+      for (int t = 0; t < nt; ++t) {
+        const int64_t l = s_l[wid][r][t];
+        const int i = s_aux[wid][r][t] & 0xffff, ell = s_aux[wid][r][t] >> 16;
+        const double2 ps = psi[l];
+        const double c = 1.0 / gs.list_r[l];
+        const double wr = eqr[2 * ((size_t)(i - 1) * kQ + lane)], wi = eqr[2 * ((size_t)(i - 1) * kQ + lane) + 1];
+        const double er = eph[2 * ((size_t)ell * kQ + lane)], eim = -eph[2 * ((size_t)ell * kQ + lane) + 1];
+        const double br = c * (ps.x * wr - ps.y * wi), bi = c * (ps.x * wi + ps.y * wr);
+        ar += br * er - bi * eim;
+        ai += br * eim + bi * er;
+      }
+      const double rt = rq[(size_t)s_cnt[wid][r] * kQ + lane] * norm;
+      const double fr = ar * rt, fi = ai * rt;
+      C o;
+      o.x = (R)fr;
+      o.y = (R)fi;
+      F[(base + r) * kQ + lane] = o;
+      acc2 += fr * fr + fi * fi;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  __shared__ double red[kBWaves][kQ];
+  red[wid][lane] = acc2;
+  __syncthreads();
+  if (wid == 0) {
+    double s = 0;
+    for (int k = 0; k < kBWaves; ++k) s += red[k][lane];
+    part[2 * ((size_t)blockIdx.x * kQ + lane)] = s;
+    part[2 * ((size_t)blockIdx.x * kQ + lane) + 1] = 0.0;
+  }
+}
+
+using SFn = void (*)(SecView, SecView, const double2 *, const double *, const double *, const double *, void *,
+                     double, double *);
+
+template <int NT>
+void launch_sminus(int prec, int grid, hipStream_t st, const SecView &gs, const SecView &tg, const double2 *psi,
+                   const double *rq, const double *eqr, const double *eph, void *F, double norm, double *part) {
+  if (prec == QS3X_FP64)
+    hipLaunchKernelGGL((k_sminus_block<NT, double>), dim3(grid), dim3(kBThreads), 0, st, gs, tg, psi, rq, eqr, eph,
+                       (double2 *)F, norm, part);
+  else
+    hipLaunchKernelGGL((k_sminus_block<NT, float>), dim3(grid), dim3(kBThreads), 0, st, gs, tg, psi, rq, eqr, eph,
+                       (float2 *)F, norm, part);
+}
+
+int cus() {
+  static int c = 0;
+  if (!c) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device()) == hipSuccess) c = p.multiProcessorCount;
+    if (c <= 0) c = 256;
+  }
+  return c;
+}
+
+}  // namespace
+
+}  // namespace qs3x
+
+using namespace qs3x;
+
+extern "C" {
+
+int qs3x_sector_build_all_k(qs3x_model *m, int32_t nod, qs3x_sector **out) {
+  return guarded4([&] {
+    QS3X_REQUIRE(m && out, "qs3x_sector_build_all_k: NULL argument");
+    QS3X_REQUIRE(m->chain, "qs3x_sector_build_all_k: chains only (LY = LZ = 1)");
+    auto *s = new qs3x_sector();
+    s->model = m;
+    s->nod = nod;
+    s->all_k = true;
+    try {
+      build_sector_device(s);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    if (!s->chain) {
+      delete s;
+      throw Error{"qs3x_sector_build_all_k: the chain enumeration does not apply to this N_down / L"};
+    }
+    *out = s;
+  });
+}
+
+int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const double *pos, const double *psi,
+                          int32_t itr_dsf, double rfield, int32_t prec, double *alpha, double *beta2,
+                          double *norm1, double *stats) {
+  return guarded4([&] {
+    QS3X_REQUIRE(gs && K && pos && psi && alpha && beta2 && norm1 && itr_dsf >= 1,
+                 "qs3x_dsf_sminus_batch: bad argument");
+    QS3X_REQUIRE(nq >= 1 && nq <= kQ, "qs3x_dsf_sminus_batch: 1 <= nq <= 64");
+    QS3X_REQUIRE(prec == QS3X_FP64 || prec == QS3X_FP32, "qs3x_dsf_sminus_batch: bad precision");
+    qs3x_model *m = gs->model;
+    QS3X_REQUIRE(m->chain && m->nn_chain, "qs3x_dsf_sminus_batch: nearest-neighbour chains only");
+    QS3X_REQUIRE(gs->chain && gs->mode == kIndexAnchored && gs->d_rclass.p,
+                 "qs3x_dsf_sminus_batch: the ground-state sector must be a chain sector built on the device");
+    const int L = m->NOS, NT = gs->nod + 1;
+    QS3X_REQUIRE(NT >= 2 && NT <= 8, "qs3x_dsf_sminus_batch: target N_down must be 2..8");
+    QS3X_REQUIRE(L < 4096, "qs3x_dsf_sminus_batch: L < 4096");
+    hipStream_t st = stream();
+    const double t0 = now_s();
+    // all-k target sector
+    qs3x_sector tgt;
+    tgt.model = m;
+    tgt.nod = NT;
+    tgt.all_k = true;
+    build_sector_device(&tgt);
+    QS3X_REQUIRE(tgt.chain && tgt.d_bitmap.p, "qs3x_dsf_sminus_batch: the target is not an anchored chain sector");
+    const int64_t dim = tgt.dim;
+    // per-q tables (padding q >= nq: r_q = 0, so those vectors stay zero)
+    std::vector<double> rq((size_t)(NT + 1) * kQ, 0.0), ratio((size_t)(NT + 1) * (NT + 1) * kQ, 0.0),
+        mask((size_t)(NT + 1) * kQ, 0.0), eph((size_t)2 * (L + 1) * kQ, 0.0), eqr((size_t)2 * L * kQ, 0.0);
+    for (int q = 0; q < kQ; ++q) {
+      eph[2 * ((size_t)0 * kQ + q)] = 1.0;
+      if (q >= nq) continue;
+      const double rk[3] = {qs3x_wavevector(K[q], L), 0.0, 0.0};
+      for (int c = 1; c <= NT; ++c) {
+        const double ro = chain_ro(L, c, rk);
+        rq[(size_t)c * kQ + q] = ro > 1.0e-15 ? std::sqrt(ro) : 0.0;
+        mask[(size_t)c * kQ + q] = ro > 1.0e-15 ? 1.0 : 0.0;
+      }
+      for (int ct = 0; ct <= NT; ++ct)
+        for (int cr = 0; cr <= NT; ++cr) {
+          const double a = rq[(size_t)ct * kQ + q], b = rq[(size_t)cr * kQ + q];
+          ratio[((size_t)ct * (NT + 1) + cr) * kQ + q] = b > 0 ? a / b : 0.0;
+        }
+      std::vector<double> ex;
+      compute_explist(m, rk, ex);
+      for (int l = 1; l <= L; ++l) {
+        eph[2 * ((size_t)l * kQ + q)] = ex[2 * (l + 3 * (L + 1))];
+        eph[2 * ((size_t)l * kQ + q) + 1] = ex[2 * (l + 3 * (L + 1)) + 1];
+      }
+      // exp(-i q . R_i), q the transfer from the ground-state sector
+      const double qx = rk[0] - gs->rk[0], qy = -gs->rk[1], qz = -gs->rk[2];
+      for (int i = 0; i < L; ++i) {
+        const double x = qx * pos[3 * i] + qy * pos[3 * i + 1] + qz * pos[3 * i + 2];
+        const std::complex<double> e = std::exp(std::complex<double>(0.0, -x));
+        eqr[2 * ((size_t)i * kQ + q)] = e.real();
+        eqr[2 * ((size_t)i * kQ + q) + 1] = e.imag();
+      }
+    }
+    DevBuf<double> d_rq(rq.size()), d_ratio(ratio.size()), d_mask(mask.size()), d_eph(eph.size()),
+        d_eqr(eqr.size()), d_psi((size_t)2 * gs->dim);
+    // the phase table in the block H.v's precision
+    DevBuf<unsigned char> d_ephR(prec == QS3X_FP64 ? sizeof(double) * eph.size() : sizeof(float) * eph.size());
+    if (prec == QS3X_FP64) {
+      QS3X_HIP(hipMemcpyAsync(d_ephR.p, eph.data(), sizeof(double) * eph.size(), hipMemcpyHostToDevice, st));
+    } else {
+      std::vector<float> ef(eph.begin(), eph.end());
+      QS3X_HIP(hipMemcpyAsync(d_ephR.p, ef.data(), sizeof(float) * ef.size(), hipMemcpyHostToDevice, st));
+      QS3X_HIP(hipStreamSynchronize(st));
+    }
+    d_rq.upload(rq.data(), rq.size());
+    d_ratio.upload(ratio.data(), ratio.size());
+    d_mask.upload(mask.data(), mask.size());
+    d_eph.upload(eph.data(), eph.size());
+    d_eqr.upload(eqr.data(), eqr.size());
+    d_psi.upload(psi, (size_t)2 * gs->dim);
+    const size_t es = prec == QS3X_FP64 ? 16 : 8;
+    DevBuf<unsigned char> U[3];
+    for (auto &u : U) u.alloc(es * (size_t)dim * kQ);
+    const int steps = (int)std::min<int64_t>(itr_dsf, dim);
+    const int NS = steps + 2;
+    // per-q scalar slots [slot][kQ]: arec [NS] | aout [NS] | beta [NS] | scale [NS+1] | norm1, tmp
+    const int A_REC = 0, A_OUT = NS, BETA = 2 * NS, SCALE = 3 * NS, NORM1 = 4 * NS + 2, TMP = 4 * NS + 3;
+    DevBuf<double> sc((size_t)(4 * NS + 8) * kQ);
+    const int gridS = cus() * 2, gridU = cus() * 8;
+    DevBuf<double> part((size_t)2 * kQ * std::max(gridU, gridS * 4));
+    QS3X_HIP(hipStreamSynchronize(st));
+    const double t1 = now_s();
+    // f_q = S^-_q psi; norm1_q = |f_q|^2 (FP64); scale[0] = 1 / sqrt(norm1)
+    {
+      const SecView vg = make_view(gs), vt = make_view(&tgt);
+      const double nrm = 1.0 / std::sqrt((double)L);
+      switch (NT) {
+        case 2: launch_sminus<2>(prec, gridS, st, vg, vt, (const double2 *)d_psi.p, d_rq.p, d_eqr.p, d_eph.p, U[0].p, nrm, part.p); break;
+        case 3: launch_sminus<3>(prec, gridS, st, vg, vt, (const double2 *)d_psi.p, d_rq.p, d_eqr.p, d_eph.p, U[0].p, nrm, part.p); break;
+        case 4: launch_sminus<4>(prec, gridS, st, vg, vt, (const double2 *)d_psi.p, d_rq.p, d_eqr.p, d_eph.p, U[0].p, nrm, part.p); break;
+        case 5: launch_sminus<5>(prec, gridS, st, vg, vt, (const double2 *)d_psi.p, d_rq.p, d_eqr.p, d_eph.p, U[0].p, nrm, part.p); break;
+        case 6: launch_sminus<6>(prec, gridS, st, vg, vt, (const double2 *)d_psi.p, d_rq.p, d_eqr.p, d_eph.p, U[0].p, nrm, part.p); break;
+        case 7: launch_sminus<7>(prec, gridS, st, vg, vt, (const double2 *)d_psi.p, d_rq.p, d_eqr.p, d_eph.p, U[0].p, nrm, part.p); break;
+        default: launch_sminus<8>(prec, gridS, st, vg, vt, (const double2 *)d_psi.p, d_rq.p, d_eqr.p, d_eph.p, U[0].p, nrm, part.p); break;
+      }
+      QS3X_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_finalize_q, dim3(kQ), dim3(kBThreads), 0, st, part.p, gridS, sc.p, 0, NORM1, -1, 0.0);
+      hipLaunchKernelGGL(k_finalize_q, dim3(kQ), dim3(kBThreads), 0, st, part.p, gridS, sc.p, 2, TMP, SCALE, 0.0);
+    }
+    QS3X_HIP(hipStreamSynchronize(st));
+    const double t2 = now_s();
+    // recurrence
+    int kb = 1;
+    while ((1 << kb) <= L) ++kb;
+    const bool key32 = (int64_t)(NT - 1) * kb <= 32;
+    size_t lds = 0;
+    BFn hv = block_kernel(NT, prec, key32, &lds, L, (int)m->bt_jxy2.size());
+    QS3X_REQUIRE(hv, "qs3x_dsf_sminus_batch: no block kernel for this N_down");
+    QS3X_REQUIRE(lds <= 160 * 1024, "qs3x_dsf_sminus_batch: L too large for the block kernel's LDS tables");
+    if (lds > 64 * 1024)
+      QS3X_HIP(hipFuncSetAttribute((const void *)hv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int nbpc = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbpc, hv, kBThreads, lds) != hipSuccess || nbpc < 1) nbpc = 1;
+    const int gridH = (int)std::min<int64_t>((int64_t)cus() * nbpc, ((dim + 63) / 64 + kBWaves - 1) / kBWaves);
+    QS3X_REQUIRE(gridH * kQ * 2 <= (int64_t)part.n, "qs3x_dsf_sminus_batch: partial buffer");
+    BlockHvArgs a{};
+    a.cfg = tgt.d_cfg.p;
+    a.bitmap = tgt.d_bitmap.p;
+    a.dim = dim;
+    a.L = L;
+    a.bits = tgt.bits;
+    a.kbits = kb;
+    a.binom = tgt.d_binom.p;
+    a.kdim = tgt.nbinom_k;
+    a.nn_bt = m->d_nn_bt.p;
+    a.bt = m->d_bt.p;
+    a.nbt = (int)m->bt_jxy2.size();
+    a.shift = prec == QS3X_FP64 ? m->ezz : 0.0;
+    a.sc = sc.p;
+    a.ratio = d_ratio.p;
+    a.rowmask = d_mask.p;
+    a.eph = d_ephR.p;
+    a.dot_partials = part.p;
+    const double sztot_new = (double)L / 2.0 - (double)NT;
+    const double c = (prec == QS3X_FP32) ? m->ezz : 0.0;
+    hipEvent_t e0, e1;
+    QS3X_HIP(hipEventCreate(&e0));
+    QS3X_HIP(hipEventCreate(&e1));
+    float hv_ms_total = 0;
+    for (int i = 0; i < steps; ++i) {
+      void *cur = U[i % 3].p, *prev = U[(i + 2) % 3].p, *w = U[(i + 1) % 3].p;
+      a.x = cur;
+      a.y = w;
+      a.iscale = SCALE + i;
+      if (i == 1) QS3X_HIP(hipEventRecord(e0, st));
+      hipLaunchKernelGGL(hv, dim3(gridH), dim3(kBThreads), lds, st, a);
+      QS3X_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_finalize_q, dim3(kQ), dim3(kBThreads), 0, st, part.p, gridH, sc.p, 1, A_REC + i,
+                         A_OUT + i, c - rfield * sztot_new);
+      if (i + 1 == steps) break;
+      if (prec == QS3X_FP64) {
+        if (i > 0)
+          hipLaunchKernelGGL((k_update3_block<double, true>), dim3(gridU), dim3(kBThreads), 0, st, (double2 *)w,
+                             (const double2 *)cur, (const double2 *)prev, dim, sc.p, A_REC + i, SCALE + i, BETA + i - 1,
+                             SCALE + i - 1, part.p);
+        else
+          hipLaunchKernelGGL((k_update3_block<double, false>), dim3(gridU), dim3(kBThreads), 0, st, (double2 *)w,
+                             (const double2 *)cur, (const double2 *)prev, dim, sc.p, A_REC + i, SCALE + i, -1, -1,
+                             part.p);
+      } else {
+        if (i > 0)
+          hipLaunchKernelGGL((k_update3_block<float, true>), dim3(gridU), dim3(kBThreads), 0, st, (float2 *)w,
+                             (const float2 *)cur, (const float2 *)prev, dim, sc.p, A_REC + i, SCALE + i, BETA + i - 1,
+                             SCALE + i - 1, part.p);
+        else
+          hipLaunchKernelGGL((k_update3_block<float, false>), dim3(gridU), dim3(kBThreads), 0, st, (float2 *)w,
+                             (const float2 *)cur, (const float2 *)prev, dim, sc.p, A_REC + i, SCALE + i, -1, -1,
+                             part.p);
+      }
+      QS3X_HIP(hipGetLastError());
+      hipLaunchKernelGGL(k_finalize_q, dim3(kQ), dim3(kBThreads), 0, st, part.p, gridU, sc.p, 2, BETA + i,
+                         SCALE + i + 1, 0.0);
+    }
+    QS3X_HIP(hipEventRecord(e1, st));
+    QS3X_HIP(hipStreamSynchronize(st));
+    if (steps > 1) QS3X_HIP(hipEventElapsedTime(&hv_ms_total, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const double t3 = now_s();
+    std::vector<double> all(sc.n);
+    sc.download(all.data(), all.size());
+    for (int q = 0; q < nq; ++q) {
+      norm1[q] = all[(size_t)NORM1 * kQ + q];
+      for (int i = 0; i < itr_dsf; ++i) {
+        alpha[(size_t)q * itr_dsf + i] = (i < steps) ? all[(size_t)(A_OUT + i) * kQ + q] : 1.0e300;
+        const double b = (i >= 1 && i < steps) ? all[(size_t)(BETA + i - 1) * kQ + q] : 0.0;
+        beta2[(size_t)q * itr_dsf + i] = b * b;
+      }
+    }
+    if (stats) {
+      stats[0] = t1 - t0;  // all-k sector and tables
+      stats[1] = t2 - t1;  // S^-_q
+      stats[2] = t3 - t2;  // recurrence
+      stats[3] = steps > 1 ? hv_ms_total / (steps - 1) : 0.0;  // ms per block step (H.v + update), steps 2..
+      stats[4] = (double)dim;
+      stats[5] = (double)steps;
+      stats[6] = 0.0;
+      stats[7] = 0.0;
+    }
+  });
+}
+
+}  // extern "C"

@@ -110,6 +110,7 @@ struct qs3x_sector {
   int32_t bits = 0;        // bits per site field in d_cfg
   int32_t mode = 0;        // IndexMode
   bool real_phase = true;  // every explist entry the operator can use is exactly 1
+  bool all_k = false;      // chain sector of every translation representative (batched multi-k work)
   bool chain = false;      // chain fast path usable (LY = LZ = 1 and built on device)
   int64_t index_len = 0;
   int32_t nbinom_n = 0, nbinom_k = 0;  // binomial table C(n, k), n < nbinom_n, k < nbinom_k
@@ -138,6 +139,9 @@ void build_sector_device(qs3x_sector *s);
 void sector_from_lists(qs3x_sector *s, const int64_t *list_s, const double *list_r,
                        const double *explist);
 void compute_explist(const qs3x_model *m, const double rk[3], std::vector<double> &ex);
+// Chain norm ro of a configuration with stabiliser order `count` at wave
+// vector rk (the reference's phase sum, state_lists.f90:84-110).
+double chain_ro(int L, int count, const double rk[3]);
 
 // H.v launch (hv.hip). v0 = H (scale * v1) [- shift for FP32]; if dot_partials
 // is non-null each block writes sum_i conj(scale*v1_i) * v0_i (complex,

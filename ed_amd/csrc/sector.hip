@@ -71,7 +71,7 @@ void compute_explist(const qs3x_model *m, const double rk[3], std::vector<double
 // The reference's norm for a configuration whose stabiliser is the set
 // {T^p, T^2p, ..., T^L} of a chain (p = L / count): ro = |sum e^{-i(pkx i +
 // pky + pkz)}|^2 NOS / count, summed in loop order (state_lists.f90:84-110).
-static double chain_ro(int L, int count, const double rk[3]) {
+double chain_ro(int L, int count, const double rk[3]) {
   if (count <= 0 || L % count) return 0.0;
   const int p = L / count;
   const double p1 = rk[2] * 1.0;
@@ -509,7 +509,8 @@ void build_sector_device(qs3x_sector *s) {
     s->h_rclass.assign(N + 1, 0.0);
     std::vector<double> ro(N + 1, 0.0);
     for (int c = 1; c <= N; ++c) {
-      ro[c] = chain_ro(L, c, s->rk);
+      // an all-k sector keeps every representative (norms are per wave vector)
+      ro[c] = s->all_k ? 1.0 : chain_ro(L, c, s->rk);
       s->h_rclass[c] = std::sqrt(ro[c]);
     }
     DevBuf<double> d_ro(N + 1);
@@ -540,6 +541,8 @@ void build_sector_device(qs3x_sector *s) {
     return;
   }
   // general lattice
+  QS3X_REQUIRE(!s->all_k, "sector: an all-k sector needs the chain enumeration (LY = LZ = 1, C(L-1, N_down-1) "
+                          "< 2^31, packed keys within 64 bits)");
   s->chain = false;
   const int64_t ncand = binom_host(m->NOS, N);
   QS3X_REQUIRE(ncand > 0 && ncand < ((int64_t)1 << 40), "sector: C(NOS, N) too large for the "

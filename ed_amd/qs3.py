@@ -58,6 +58,8 @@ def _load():
         "qs3x_lanczos": [_P, _i32, _i64, _i64, _i64, _dbl, _P, _u64, _P, _P, _P],
         "qs3x_trlanczos": [_P, _i32, _i32, _i32, _i32, _i64, _dbl, _P, _u64, _P, _P, _P],
         "qs3x_solver_trace": [_P, _i64, _P],
+        "qs3x_sector_build_all_k": [_P, _i32, _P],
+        "qs3x_dsf_sminus_batch": [_P, _i32, _P, _P, _P, _i32, _dbl, _i32, _P, _P, _P, _P],
         "qs3x_correlations": [_P, _P, _i64, _P, _P, _P],
         "qs3x_local_mag": [_P, _P, _P],
         "qs3x_apply_sq": [_P, _P, _i32, _P, _P, _P, _P],
@@ -155,6 +157,14 @@ class Sector:
         _check(_lib.qs3x_sector_build(model._h, int(nod), float(rk[0]), float(rk[1]), float(rk[2]),
                                       ctypes.byref(h), _p(d)))
         return cls(model, nod, h, rk)
+
+    @classmethod
+    def build_all_k(cls, model, nod):
+        """Every translation representative of N_down (the union of all k
+        sectors of a chain), for the batched multi-k entry points."""
+        h = ctypes.c_void_p()
+        _check(_lib.qs3x_sector_build_all_k(model._h, int(nod), ctypes.byref(h)))
+        return cls(model, nod, h)
 
     @classmethod
     def from_lists(cls, model, nod, list_s, list_r, explist=None):
@@ -285,6 +295,24 @@ def dsf_recurrence(tgt, f, itr_dsf, rfield=0.0, prec=FP64):
     if np.ndim(f) == 1:
         return alpha[0], beta2[0], float(norm1[0])
     return alpha, beta2, norm1
+
+
+def dsf_sminus_batch(gs, K, pos, psi, itr_dsf, rfield=0.0, prec=FP32):
+    """S^-_q |psi> and the DSF recurrence for up to 64 wave numbers K (target
+    sectors k = 2 pi K / L of N_down + 1) in one block.  Returns alpha, beta2
+    (len(K) x itr_dsf), norm1 (len(K)) and a stats dict."""
+    Ks = np.ascontiguousarray(np.asarray(K, dtype=np.int32).reshape(-1))
+    nq = len(Ks)
+    ps = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(-1))
+    v = _c128(psi)
+    alpha = np.zeros((nq, itr_dsf))
+    beta2 = np.zeros((nq, itr_dsf))
+    norm1 = np.zeros(nq)
+    stats = np.zeros(8)
+    _check(_lib.qs3x_dsf_sminus_batch(gs._h, nq, _p(Ks), _p(ps), _p(v), int(itr_dsf), float(rfield), int(prec),
+                                      _p(alpha), _p(beta2), _p(norm1), _p(stats)))
+    keys = ["seconds_setup", "seconds_sminus", "seconds_recurrence", "ms_per_block_step", "dim_all_k", "steps"]
+    return alpha, beta2, norm1, dict(zip(keys, stats[:6].tolist()))
 
 
 def dsf_analysis(alpha, beta2, norm1, E0, omega, eta=0.04, reference_quirk=True):

@@ -138,6 +138,29 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
                    int64_t maxitr, double conv, const double *start, uint64_t seed,
                    double *ene, double *psi, double *stats);
 
+/* ------------------------------------------------------ batched wave vectors */
+/* A chain sector holding every translation representative of N_down (the
+ * union of all k sectors, rows in ascending rank).  No counterpart in the
+ * reference; its norms are per wave vector, so it serves the batched entry
+ * point below (its list_r is 1 and it is not a sector for qs3x_hv_dev). */
+int qs3x_sector_build_all_k(qs3x_model *m, int32_t nod, qs3x_sector **out);
+
+/* The DSF recurrence for nq <= 64 wave vectors in one block.  Replaces nq
+ * calls of calcu_DSF_wavevector for S^- (get_expectation_values.f90:379-605,
+ * with calcu_Sq_a at :281-377 and allocate_lists_omp_SQ_new at
+ * state_lists.f90:418-486): for each j, f_j = S^-_q |psi> with
+ * q = 2 pi K[j] / L - k_gs is built into the k = 2 pi K[j] / L sector of
+ * N_down + 1, and itr_dsf steps of the recurrence run on all f_j together
+ * (interleaved vectors; hop structure shared by all of them).  gs: the
+ * ground-state chain sector (nearest-neighbour chains); psi: its vector (host,
+ * complex double); pos: 3 x NOS site positions.  Outputs as nq calls of
+ * qs3x_dsf_recurrence: alpha, beta2 (nq x itr_dsf, row-major), norm1 (nq).
+ * stats (8 doubles, may be NULL): [seconds setup, seconds S^-, seconds
+ * recurrence, ms per block step, rows of the all-k sector, steps, 0, 0]. */
+int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const double *pos,
+                          const double *psi, int32_t itr_dsf, double rfield, int32_t prec,
+                          double *alpha, double *beta2, double *norm1, double *stats);
+
 /* Convergence trace of the last qs3x_lanczos / qs3x_trlanczos call (no
  * counterpart in the reference, whose solvers print progress instead):
  * *n rows of [seconds since the solve started, H.v applications so far,

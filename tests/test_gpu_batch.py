@@ -1,0 +1,121 @@
+"""GPU parity of the batched multi-q DSF path against the per-sector path.
+
+qs3x_dsf_sminus_batch builds S^-_q |psi> for many wave numbers K into one
+all-k sector and runs the DSF recurrence (calcu_DSF_wavevector,
+get_expectation_values.f90:379-605) on all of them as one block of
+interleaved vectors.  Its outputs must equal, q by q, those of the per-sector
+path: qs3x_apply_sq into the k = 2 pi K / L sector (pinned against the oracle
+in test_gpu_solvers.py) followed by qs3x_dsf_recurrence.  Tolerances: norm1
+1e-12 relative; alpha, beta^2 1e-9 (FP64) / 1e-4 (FP32) relative to their
+scale over the first steps (late coefficients of a recurrence without
+reorthogonalisation amplify rounding differences in any precision)."""
+import numpy as np
+import pytest
+
+from oracle import qs3 as orc
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _q3():
+    from ed_amd import qs3 as q3
+    return q3
+
+
+def dimer(L, j1, j2, jz1, jz2):
+    return [(i, i % L + 1, j1 if i % 2 else j2, jz1 if i % 2 else jz2) for i in range(1, L + 1)]
+
+
+def test_all_k_sector_is_the_k0_list():
+    """Every representative has ro > 0 at k = 0, so the all-k sector's rows are
+    the k = 0 sector's list_s (state_lists.f90:225-292)."""
+    print("[gpu] all-k sector", flush=True)
+    q3 = _q3()
+    for L, N in ((12, 3), (16, 4), (15, 5), (20, 2)):
+        bonds = orc.chain_bonds(L, [(1, 1.0, 1.0)])
+        om = orc.Model(L, L, 1, 1, bonds)
+        o0 = om.build_sector(N, (0.0, 0.0, 0.0))
+        gm = q3.Model(L, L, 1, 1, bonds)
+        ak = q3.Sector.build_all_k(gm, N)
+        ls, _ = ak.lists()
+        assert np.array_equal(ls, o0.list_s)
+        # the union of the k sectors is the same set
+        union = set()
+        for K in range(L):
+            union |= set(om.build_sector(N, om.rk_from_K(K)).list_s.tolist())
+        assert sorted(union) == ls.tolist()
+
+
+CASES = [
+    # name, L, N_gs, bonds, K list, steps
+    ("chain12_n2", 12, 2, orc.chain_bonds(12, [(1, 1.0, 1.0)]), list(range(12)), 25),
+    ("chain16_n3_xxz", 16, 3, orc.chain_bonds(16, [(1, 1.0, 0.7)]), [0, 1, 4, 8, 8, 15, 3], 25),
+    ("chain14_n3_dimer", 14, 3, dimer(14, 1.0, 0.6, 0.9, 0.4), list(range(14)), 25),
+    ("chain18_n5", 18, 5, orc.chain_bonds(18, [(1, 1.0, 1.0)]), [0, 3, 6, 9, 2], 20),
+]
+
+
+@pytest.mark.parametrize("prec_name", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_dsf_batch_matches_per_sector(case, prec_name):
+    name, L, Ngs, bonds, Ks, steps = case
+    print(f"[gpu] dsf batch {name} {prec_name}", flush=True)
+    q3 = _q3()
+    prec = q3.FP64 if prec_name == "fp64" else q3.FP32
+    gm = q3.Model(L, L, 1, 1, bonds)
+    gsec = q3.Sector.build(gm, Ngs, (0.0, 0.0, 0.0))
+    rng = np.random.default_rng(11)
+    psi = rng.uniform(-1, 1, gsec.dim) + 1j * rng.uniform(-1, 1, gsec.dim)
+    pos = np.array([[i, 0.0, 0.0] for i in range(L)], dtype=float)
+    rfield = 0.2
+    a_b, b_b, n_b, st = q3.dsf_sminus_batch(gsec, Ks, pos, psi, steps, rfield=rfield, prec=prec)
+    assert st["steps"] >= 1
+    tol = 1e-9 if prec == q3.FP64 else 1e-4
+    nchk = 12
+    for j, K in enumerate(Ks):
+        rk = q3.wavevector(K, L)
+        tgt = q3.Sector.build(gm, Ngs + 1, (rk, 0.0, 0.0))
+        if tgt.dim == 0:
+            assert n_b[j] == 0.0
+            continue
+        f = q3.calcu_Sq_a(2, gsec, tgt, np.array([rk, 0.0, 0.0]), pos, psi)
+        n1 = np.vdot(f, f).real
+        assert n_b[j] == pytest.approx(n1, rel=1e-12, abs=1e-300)
+        if n1 < 1e-20:
+            continue
+        s = min(steps, tgt.dim)
+        a_s, b_s, n_s = q3.dsf_recurrence(tgt, f, s, rfield=rfield, prec=prec)
+        # a recurrence longer than half the sector dimension has resolved the
+        # spectrum and loses orthogonality (FP32 first)
+        k = min(nchk, s, max(2, tgt.dim // 2))
+        # an invariant Krylov subspace (e.g. K = 0 of an isotropic chain, where
+        # S^- commutes with H) ends the recurrence: later coefficients are noise
+        for i in range(1, k):
+            if b_s[i] < 1e-10 * max(np.max(b_s[1:i + 1]), 1e-300) or b_s[i] < 1e-24:
+                k = i
+                break
+        sa = max(1.0, np.max(np.abs(a_s[:k])))
+        sb = max(1.0, np.max(np.abs(b_s[:k])))
+        assert np.max(np.abs(a_b[j, :k] - a_s[:k])) <= tol * sa, (K, a_b[j, :k], a_s[:k])
+        assert np.max(np.abs(b_b[j, :k] - b_s[:k])) <= tol * sb, (K, b_b[j, :k], b_s[:k])
+
+
+def test_dsf_batch_fp32_vs_fp64():
+    """FP32 block against FP64 block on the same inputs: early coefficients to
+    FP32 resolution of the shifted operator."""
+    print("[gpu] dsf batch fp32 vs fp64", flush=True)
+    q3 = _q3()
+    L, Ngs = 20, 3
+    bonds = orc.chain_bonds(L, [(1, 1.0, 1.0)])
+    gm = q3.Model(L, L, 1, 1, bonds)
+    gsec = q3.Sector.build(gm, Ngs, (0.0, 0.0, 0.0))
+    E0, psi, _ = q3.lanczos_routines(gsec, maxitr=400, lnc_ene_conv=1e-13, prec=q3.FP64, seed=3)
+    pos = np.array([[i, 0.0, 0.0] for i in range(L)], dtype=float)
+    Ks = list(range(1, L))
+    a64, b64, n64, _ = q3.dsf_sminus_batch(gsec, Ks, pos, psi, 30, prec=q3.FP64)
+    a32, b32, n32, _ = q3.dsf_sminus_batch(gsec, Ks, pos, psi, 30, prec=q3.FP32)
+    assert np.allclose(n32, n64, rtol=1e-12)
+    assert np.max(np.abs(a32[:, :10] - a64[:, :10])) <= 1e-4 * max(1.0, np.max(np.abs(a64[:, :10])))
+    assert np.max(np.abs(b32[:, :10] - b64[:, :10])) <= 1e-4 * max(1.0, np.max(np.abs(b64[:, :10])))
