@@ -83,6 +83,9 @@ int qs3x_set_option(const char *name, double value) {
     } else if (n == "verbose") {
       QS3X_REQUIRE(value >= 0 && value <= 2, "verbose must be 0..2");
       options().verbose = (int)value;
+    } else if (n == "hv_xcd") {
+      QS3X_REQUIRE(value >= 1 && value <= 64, "hv_xcd must be 1..64");
+      options().hv_xcd = (int)value;
     } else if (n == "hv_kernel") {
       QS3X_REQUIRE(value >= 0 && value <= 3, "hv_kernel must be 0..3");
       options().hv_kernel = (int)value;

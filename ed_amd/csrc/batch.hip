@@ -82,6 +82,7 @@ struct BlockHvArgs {
   const double *rowmask;  // [N+1][kQ]: 1 when r_q(c) > 0
   const void *eph;        // [L+1][kQ] complex in precision R: explist_q(ell, 1, 1); entry 0 = 1
   double *dot_partials;   // [grid][kQ] complex
+  int nxcd;               // L2 domains for the chunk schedule (tile_range)
 };
 
 // C(x, q) for 1 <= q <= N - 1 from the LDS table.
@@ -251,7 +252,11 @@ __global__ __launch_bounds__(kBThreads) void k_hv_block(BlockHvArgs a) {
   const C eL1 = cload<R>(a.eph, (size_t)(L - 1) * kQ + lane);
   double dre = 0.0, dim_ = 0.0;
   const int64_t nchunks = (a.dim + 63) / 64;
-  for (int64_t ch = (int64_t)blockIdx.x * kBWaves + wid; ch < nchunks; ch += (int64_t)gridDim.x * kBWaves) {
+  // a block takes kBWaves consecutive chunks at a time, XCD-contiguous ranges
+  const TileRange tr = tile_range((nchunks + kBWaves - 1) / kBWaves, a.nxcd);
+  for (int64_t bt = tr.first; bt < tr.end; bt += tr.stride) {
+    const int64_t ch = bt * kBWaves + wid;
+    if (ch >= nchunks) continue;
     const int64_t base = ch * 64;
     // lanes = rows: hop structure of the chunk
     {
@@ -776,6 +781,7 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
     a.rowmask = d_mask.p;
     a.eph = d_ephR.p;
     a.dot_partials = part.p;
+    a.nxcd = options().hv_xcd;
     const double sztot_new = (double)L / 2.0 - (double)NT;
     const double c = (prec == QS3X_FP32) ? m->ezz : 0.0;
     hipEvent_t e0, e1;

@@ -39,6 +39,11 @@ struct Options {
   // the batched-load chain kernel on; 2 = from the per-hop chain kernel on;
   // 3 = general lattice kernel (translation scan).
   int hv_kernel = 0;
+  // L2 domains (XCDs) for the H.v tile schedule; 1 = plain round-robin.
+  // MI355X has 8 XCDs: 8 halves the measured HBM traffic of the chain H.v at
+  // L = 1000 but leaves its time unchanged (the kernel is latency-bound), so
+  // the default stays 1.
+  int hv_xcd = 1;
   int verbose = 0;            // 1: one stderr line per restart / per 100 Lanczos steps
 };
 Options &options();

@@ -666,6 +666,7 @@ static HvArgs make_args(qs3x_sector *s, void *v0, const void *v1, const double *
   a.ezz = m->ezz;
   a.scale = scale;
   a.dot_partials = partials;
+  a.nxcd = options().hv_xcd;
   return a;
 }
 

@@ -45,7 +45,25 @@ struct HvArgs {
   const double *scale;  // device scalar or null
   double *dot_partials; // 2 per block or null
   unsigned long long *nnz_out;
+  int nxcd;             // tile ranges per L2 domain (1: plain round-robin over blocks)
 };
+
+// Persistent-grid tile schedule.  Workgroups are dispatched round-robin over
+// the nxcd L2 domains (XCDs) of the device, so block b runs on domain b % nxcd;
+// each domain is given one contiguous range of tiles, and its blocks stride
+// through that range.  Rows that are neighbours in configuration space are
+// then processed by the same XCD, whose L2 holds their vector elements.
+struct TileRange {
+  int64_t first, end, stride;
+};
+__device__ __forceinline__ TileRange tile_range(int64_t ntiles, int nxcd) {
+  // every domain that owns a range must have at least one block
+  const int nx = (int)min<int64_t>(nxcd > 0 ? nxcd : 1, (int64_t)gridDim.x);
+  const int x = (int)(blockIdx.x % nx);
+  const int64_t per = ((int64_t)gridDim.x - x + nx - 1) / nx;  // blocks on domain x
+  const int64_t t0 = ntiles * x / nx, t1 = ntiles * (x + 1) / nx;
+  return TileRange{t0 + (int64_t)(blockIdx.x / nx), t1, per};
+}
 
 constexpr int kThreads = 256;
 

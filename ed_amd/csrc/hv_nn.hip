@@ -322,7 +322,8 @@ __global__ __launch_bounds__(kThreads) void k_hv_chain_nn(HvArgs a) {
   double dre = 0.0, dim_ = 0.0;
   const int64_t nsuper = (a.dim + (int64_t)kThreads * RPT - 1) / ((int64_t)kThreads * RPT);
 
-  for (int64_t t = blockIdx.x; t < nsuper; t += gridDim.x) {
+  const TileRange tr = tile_range(nsuper, a.nxcd);
+  for (int64_t t = tr.first; t < tr.end; t += tr.stride) {
     int64_t row[RPT];
     bool valid[RPT], clear[RPT];
     uint64_t w[RPT];

@@ -52,6 +52,9 @@ int qs3x_set_device(int32_t device);
  *                   neighbour chain, batched-load chain, per-hop chain, general
  *                   lattice, in that order); 1, 2, 3: start the list at the
  *                   batched-load chain, per-hop chain or general kernel.
+ *   "hv_xcd"        L2 domains (XCDs) for the H.v tile schedule: each gets one
+ *                   contiguous range of rows (1, the default: plain round-robin
+ *                   over workgroups; 8 for MI355X's XCDs).
  *   "verbose"       0 (default); 1: a stderr line per thick restart and per
  *                   100 Lanczos steps; 2: per convergence check. */
 int qs3x_set_option(const char *name, double value);
