@@ -14,6 +14,7 @@ ham_to_vec_wave_vector timed on a bounded sample of rows).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu]
        python bench.py --dsf [--dsf-steps 200]   (batched 64-q DSF workload, its own JSON line)
+       python bench.py --sectors                 (280-sector L=40 workload, its own JSON line)
 """
 import argparse
 import json
@@ -154,6 +155,65 @@ def dsf_workload(args):
     print(json.dumps(res), flush=True)
 
 
+def sectors_workload(args):
+    """Lowest eigenvalue of every (N_down, k) sector of the periodic XXZ chain
+    L = 40, Jxy = 1, Jz = 0.8, N_down = 1..7, all 40 k (280 sectors, about
+    23.2 M rows): plain Lanczos (FP32-mixed) for all sectors at once, with and
+    without hipGraph replay of the step, against sector-by-sector solves; the
+    error of every E0 against the FP64 group solve."""
+    import torch
+    from ed_amd import qs3 as q3
+    L = 40
+    gm = q3.Model(L, L, 1, 1, q3.chain_bonds(L, [(1, 1.0, 0.8)]))
+    t0 = time.perf_counter()
+    secs, keys = [], []
+    for N in range(1, 8):
+        for K in range(L):
+            s = q3.Sector.build(gm, N, (q3.wavevector(K, L), 0.0, 0.0))
+            if s.dim:
+                secs.append(s)
+                keys.append((N, K))
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    rows = sum(s.dim for s in secs)
+    log(f"{len(secs)} sectors, {rows} rows, built in {t_build:.2f} s")
+    kw = dict(maxitr=400, minitr=20, itrint=5, lnc_ene_conv=args.sectors_conv, seed=1)
+    res = {}
+    for name, prec, graph in (("fp32_graph", q3.FP32, True), ("fp32_nograph", q3.FP32, False),
+                              ("fp64_graph", q3.FP64, True)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        E, its, st = q3.lanczos_group(secs, prec=prec, use_graph=graph, **kw)
+        torch.cuda.synchronize()
+        res[name] = (E, its, st, time.perf_counter() - t0)
+        log(f"group {name}: {res[name][3]:.3f} s, {st}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E1 = []
+    for s in secs:
+        e, _, _ = q3.lanczos_routines(s, prec=q3.FP32, want_vector=False, **kw)
+        E1.append(e)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    E32, it32, st32, t32 = res["fp32_graph"]
+    E64 = res["fp64_graph"][0]
+    rel = np.abs(E32 - E64) / np.maximum(np.abs(E64), 1e-300)
+    out = {"metric": "lowest eigenvalue of 280 (N_down, k) sectors, XXZ chain L=40 Jz=0.8, FP32-mixed",
+           "value": t32, "unit": "s", "higher_is_better": False, "dtype": "complex-fp32 (fp64 alpha/beta)",
+           "data": "synthetic", "config": {"workload": "periodic XXZ chain L=40, Jxy=1, Jz=0.8, N_down=1..7, "
+                                                       "all 40 k", "sectors": len(secs), "rows": int(rows),
+                                           "conv": args.sectors_conv, "maxitr": 400},
+           "seconds_group_fp32_graph": t32, "seconds_group_fp32_no_graph": res["fp32_nograph"][3],
+           "seconds_group_fp64_graph": res["fp64_graph"][3], "seconds_sector_by_sector_fp32": t_seq,
+           "speedup_group_vs_sector_by_sector": t_seq / t32, "group_steps": st32["steps"],
+           "max_iters_fp32": int(np.max(it32)), "fp32_vs_fp64_E0_rel_err_max": float(np.max(rel)),
+           "fp32_vs_fp64_E0_rel_err_median": float(np.median(rel)),
+           "group_vs_sector_by_sector_fp32_max_rel_diff": float(np.max(np.abs(E32 - np.array(E1)) /
+                                                                       np.maximum(np.abs(E32), 1e-300))),
+           "sector_build_seconds": t_build}
+    print(json.dumps(out), flush=True)
+
+
 def load_traffic():
     p = os.path.join(ROOT, "profiles", "hv_chain_L1000_N4_fp32_traffic.json")
     if os.path.exists(p):
@@ -179,6 +239,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true", help="only run a 1 GiB device copy (counter scaling)")
     ap.add_argument("--dsf", action="store_true", help="run the batched 64-q DSF workload instead of the headline")
     ap.add_argument("--dsf-steps", type=int, default=200)
+    ap.add_argument("--sectors", action="store_true", help="run the 280-sector L=40 workload instead of the headline")
+    ap.add_argument("--sectors-conv", type=float, default=1e-10)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X; multi-GPU runs are not part of this build "
@@ -190,6 +252,9 @@ def main():
         return
     if args.dsf:
         dsf_workload(args)
+        return
+    if args.sectors:
+        sectors_workload(args)
         return
 
     import torch

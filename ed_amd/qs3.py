@@ -59,6 +59,7 @@ def _load():
         "qs3x_trlanczos": [_P, _i32, _i32, _i32, _i32, _i64, _dbl, _P, _u64, _P, _P, _P],
         "qs3x_solver_trace": [_P, _i64, _P],
         "qs3x_sector_build_all_k": [_P, _i32, _P],
+        "qs3x_lanczos_group": [_P, _i32, _i32, _i64, _i64, _i64, _dbl, _u64, _i32, _P, _P, _P],
         "qs3x_dsf_sminus_batch": [_P, _i32, _P, _P, _P, _i32, _dbl, _i32, _P, _P, _P, _P],
         "qs3x_correlations": [_P, _P, _i64, _P, _P, _P],
         "qs3x_local_mag": [_P, _P, _P],
@@ -225,6 +226,23 @@ def lanczos_routines(sector, maxitr=2000, minitr=20, itrint=5, lnc_ene_conv=1e-1
     _check(_lib.qs3x_lanczos(sector._h, int(prec), int(maxitr), int(minitr), int(itrint),
                              float(lnc_ene_conv), _p(st), int(seed), _p(ene), _p(psi), _p(it)))
     return float(ene[0]), psi, int(it[0])
+
+
+def lanczos_group(sectors, maxitr=2000, minitr=20, itrint=5, lnc_ene_conv=1e-14, prec=FP64, seed=1,
+                  use_graph=True):
+    """Plain Lanczos energies of many sectors of one chain model at once
+    (lanczos_routines per sector, lanczos.f90:14-152).  Returns
+    (E0 per sector, iterations per sector, stats dict)."""
+    hs = (ctypes.c_void_p * len(sectors))(*[s._h.value if isinstance(s._h, ctypes.c_void_p) else s._h
+                                           for s in sectors])
+    ene = np.zeros(len(sectors))
+    its = np.zeros(len(sectors), dtype=np.int64)
+    stats = np.zeros(8)
+    _check(_lib.qs3x_lanczos_group(hs, len(sectors), int(prec), int(maxitr), int(minitr), int(itrint),
+                                   float(lnc_ene_conv), int(seed), 1 if use_graph else 0, _p(ene), _p(its),
+                                   _p(stats)))
+    keys = ["seconds", "seconds_setup", "steps", "step_launches", "rows_padded", "groups"]
+    return ene, its, dict(zip(keys, stats[:6].tolist()))
 
 
 def my_trlanczos_routines_complex(sector, NOE, NOK, NOM, maxitr=1000, lnc_ene_conv=1e-14, prec=FP64,

@@ -164,6 +164,21 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
                           const double *psi, int32_t itr_dsf, double rfield, int32_t prec,
                           double *alpha, double *beta2, double *norm1, double *stats);
 
+/* Plain Lanczos for nsec sectors of one chain model at once.  Replaces nsec
+ * calls of lanczos_routines (lanczos.f90:14-152; energies only, no second
+ * pass): every sector runs the reference's recurrence from its own
+ * counter-based start vector (seed), is checked every itrint steps past
+ * minitr, and stops at |E_prev/E - 1| < conv, at breakdown, at its dimension
+ * or at maxitr.  Each step is one H.v launch per N_down (all k sectors of it)
+ * plus one update and two segmented reductions for all sectors;
+ * use_graph != 0 replays the step as a captured hipGraph.  ene (nsec):
+ * lowest Ritz values; iters (nsec, may be NULL): steps per sector; stats (8,
+ * may be NULL): [seconds total, seconds setup, group steps, step launches,
+ * padded rows, N_down groups, 0, 0]. */
+int qs3x_lanczos_group(qs3x_sector *const *secs, int32_t nsec, int32_t prec, int64_t maxitr,
+                       int64_t minitr, int64_t itrint, double conv, uint64_t seed, int32_t use_graph,
+                       double *ene, int64_t *iters, double *stats);
+
 /* Convergence trace of the last qs3x_lanczos / qs3x_trlanczos call (no
  * counterpart in the reference, whose solvers print progress instead):
  * *n rows of [seconds since the solve started, H.v applications so far,

@@ -119,3 +119,38 @@ def test_dsf_batch_fp32_vs_fp64():
     assert np.allclose(n32, n64, rtol=1e-12)
     assert np.max(np.abs(a32[:, :10] - a64[:, :10])) <= 1e-4 * max(1.0, np.max(np.abs(a64[:, :10])))
     assert np.max(np.abs(b32[:, :10] - b64[:, :10])) <= 1e-4 * max(1.0, np.max(np.abs(b64[:, :10])))
+
+
+@pytest.mark.parametrize("prec_name,graph", [("fp64", True), ("fp32", True), ("fp64", False)])
+def test_lanczos_group_matches_per_sector(prec_name, graph):
+    """All (N_down, k) sectors of an XXZ chain in one group: each sector's E0
+    equals the per-sector plain Lanczos (same seed, same recurrence) and, in
+    FP64, the dense ground-state energy of the oracle's sector Hamiltonian."""
+    print(f"[gpu] lanczos group {prec_name} graph={graph}", flush=True)
+    q3 = _q3()
+    prec = q3.FP64 if prec_name == "fp64" else q3.FP32
+    L = 12
+    bonds = orc.chain_bonds(L, [(1, 1.0, 0.8)])
+    gm = q3.Model(L, L, 1, 1, bonds)
+    om = orc.Model(L, L, 1, 1, bonds)
+    secs, keys = [], []
+    for N in range(1, 7):
+        for K in range(L):
+            s = q3.Sector.build(gm, N, (q3.wavevector(K, L), 0.0, 0.0))
+            if s.dim:
+                secs.append(s)
+                keys.append((N, K))
+    conv = 1e-12
+    E, its, st = q3.lanczos_group(secs, maxitr=500, minitr=10, itrint=5, lnc_ene_conv=conv, prec=prec, seed=5,
+                                  use_graph=graph)
+    assert st["groups"] == 6
+    tol = 1e-9 if prec == q3.FP64 else 2e-6
+    for (N, K), s, e in zip(keys, secs, E):
+        e1, _, _ = q3.lanczos_routines(s, maxitr=500, minitr=10, itrint=5, lnc_ene_conv=conv, prec=prec, seed=5,
+                                       want_vector=False)
+        assert abs(e - e1) <= tol * max(1.0, abs(e1)), (N, K, e, e1)
+        if prec == q3.FP64:
+            osec = om.build_sector(N, om.rk_from_K(K))
+            H = om.full_hamiltonian(osec)
+            ed = np.linalg.eigvalsh(H)[0]
+            assert abs(e - ed) <= 1e-9 * max(1.0, abs(ed)), (N, K, e, ed)
