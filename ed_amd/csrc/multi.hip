@@ -17,7 +17,8 @@
 // * the slot of step i is read from a device step counter, so one captured
 //   hipGraph is the whole step and is replayed between convergence checks;
 // * a sector stops (its tiles are skipped) when the host's check finds it
-//   converged, at Krylov exhaustion (step = dim) or at maxitr.
+//   converged or broken down, at its step bound (dim in FP64, as the
+//   reference; 2 dim + 8 in FP32, see qs3x_lanczos) or at maxitr.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -472,7 +473,9 @@ int qs3x_lanczos_group(qs3x_sector *const *secs, int32_t nsec, int32_t prec, int
     std::vector<int64_t> cap(S);
     int64_t mitr = 1;
     for (int j = 0; j < S; ++j) {
-      cap[j] = std::min<int64_t>(maxitr, secs[order[j]]->dim);
+      // dim steps (the reference's bound) in FP64; FP32 may run to 2 dim + 8 (see qs3x_lanczos)
+      const int64_t dj = secs[order[j]]->dim;
+      cap[j] = std::min<int64_t>(maxitr, prec == QS3X_FP32 ? 2 * dj + 8 : dj);
       mitr = std::max(mitr, cap[j]);
     }
     DevBuf<SecDesc> d_desc(S);

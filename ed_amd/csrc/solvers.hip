@@ -106,7 +106,11 @@ int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, i
     hipStream_t st = stream();
     const int64_t dim = s->dim;
     QS3X_REQUIRE(dim >= 1, "qs3x_lanczos: empty sector");
-    const int64_t mitr = std::min<int64_t>(maxitr, dim);
+    // The reference stops at dim steps (exact arithmetic spans the sector).
+    // FP32 vectors lose orthogonality long before, so the FP32 recurrence may
+    // run past dim (up to 2 dim + 8 steps): ghosts of converged Ritz values
+    // never lie below the lowest eigenvalue, and the change test still stops it.
+    const int64_t mitr = std::min<int64_t>(maxitr, prec == QS3X_FP32 ? 2 * dim + 8 : dim);
     const size_t es = elem_bytes(prec);
     DevBuf<unsigned char> U[3];
     for (auto &u : U) u.alloc(es * dim);

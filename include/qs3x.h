@@ -115,6 +115,8 @@ int qs3x_hv_dev(qs3x_sector *s, int32_t prec, void *v0_dev, const void *v1_dev, 
  * lancz_eigen_val (lanczos.f90:14-152): three-term recurrence without
  * reorthogonalisation, lowest Ritz value every itrint steps past minitr,
  * stop when |E_prev/E - 1| < conv; optional second pass for the eigenvector.
+ * FP64 stops at dim steps as the reference does; the FP32-mixed recurrence
+ * may run to 2 dim + 8 steps (lost orthogonality needs them).
  * start: host start vector (complex double, length dim) or NULL for the
  * counter-based generator keyed by seed.  ene[0] = E0; psi (host, length
  * dim, may be NULL) receives the normalised ground state; iters the number of
