@@ -232,12 +232,15 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--NOK", type=int, default=4)
-    ap.add_argument("--NOM", type=int, default=60)
-    ap.add_argument("--reorth", type=int, default=1, help="thick-restart reorthogonalisation: 0 full, 1 locked+local")
+    ap.add_argument("--NOM", type=int, default=480)
+    ap.add_argument("--reorth", type=int, default=2,
+                    help="thick-restart reorthogonalisation: 0 full, 1 locked+local, 2 locked at restart only")
     ap.add_argument("--cgs", type=int, default=1, help="Gram-Schmidt passes per thick-restart step")
     ap.add_argument("--fp32-floor", type=float, default=1e-7)
+    ap.add_argument("--check-every", type=int, default=5,
+                    help="thick restart: convergence check every n steps inside cycles (0: at restarts)")
     ap.add_argument("--conv-plain", type=float, default=1e-11)
-    ap.add_argument("--conv-mixed", type=float, default=1e-10)
+    ap.add_argument("--conv-mixed", type=float, default=1e-11)
     ap.add_argument("--calibrate-copy", action="store_true", help="only run a 1 GiB device copy (counter scaling)")
     ap.add_argument("--dsf", action="store_true", help="run the batched 64-q DSF workload instead of the headline")
     ap.add_argument("--dsf-steps", type=int, default=200)
@@ -326,6 +329,7 @@ def main():
         q3.set_option("tr_reorth", args.reorth)
         q3.set_option("tr_cgs_passes", args.cgs)
         q3.set_option("fp32_floor", args.fp32_floor)
+        q3.set_option("tr_check_every", args.check_every)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ene, _, st = q3.my_trlanczos_routines_complex(sec, NOE=1, NOK=args.NOK, NOM=args.NOM, maxitr=400,
@@ -344,7 +348,7 @@ def main():
                  "rel_err_fp32_mixed": abs(E32 / E0_EXACT - 1), "fp32_mixed_stats": st,
                  "fp32_mixed_settings": {"NOK": args.NOK, "NOM": args.NOM, "tr_reorth": args.reorth,
                                          "cgs_passes": args.cgs, "fp32_floor": args.fp32_floor,
-                                         "conv": args.conv_mixed},
+                                         "conv": args.conv_mixed, "check_every": args.check_every},
                  "fp64_plain_settings": {"conv": args.conv_plain, "itrint": 5, "minitr": 20},
                  "speedup_fp32_mixed_vs_fp64_plain": t64 / t32 if t32 > 0 else None,
                  # first Ritz value within 1e-9 of the Bethe-ansatz E0 (solver trace; seconds, H.v)

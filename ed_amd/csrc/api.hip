@@ -75,7 +75,7 @@ int qs3x_set_option(const char *name, double value) {
     QS3X_REQUIRE(name, "qs3x_set_option: NULL name");
     const std::string n(name);
     if (n == "tr_reorth") {
-      QS3X_REQUIRE(value == 0 || value == 1, "tr_reorth must be 0 or 1");
+      QS3X_REQUIRE(value >= 0 && value <= 2, "tr_reorth must be 0, 1 or 2");
       options().tr_reorth = (int)value;
     } else if (n == "tr_cgs_passes") {
       QS3X_REQUIRE(value >= 1 && value <= 3, "tr_cgs_passes must be 1..3");
@@ -83,6 +83,12 @@ int qs3x_set_option(const char *name, double value) {
     } else if (n == "verbose") {
       QS3X_REQUIRE(value >= 0 && value <= 2, "verbose must be 0..2");
       options().verbose = (int)value;
+    } else if (n == "rot_vec") {
+      QS3X_REQUIRE(value == 0 || value == 1, "rot_vec must be 0 or 1");
+      options().rot_vec = (int)value;
+    } else if (n == "tr_check_every") {
+      QS3X_REQUIRE(value >= 0 && value <= 1000, "tr_check_every must be 0..1000");
+      options().tr_check_every = (int)value;
     } else if (n == "hv_xcd") {
       QS3X_REQUIRE(value >= 1 && value <= 64, "hv_xcd must be 1..64");
       options().hv_xcd = (int)value;

@@ -31,9 +31,12 @@ int vec_maxpy(int prec, const void *const *V, const double *scales, const int *s
 
 // Tunables (qs3x_set_option): thick-restart reorthogonalisation and the FP32 phase.
 struct Options {
-  int tr_reorth = 0;          // 0: every previous basis vector; 1: locked Ritz vectors + last two
+  int tr_reorth = 0;          // 0: every previous basis vector; 1: locked Ritz vectors + last two;
+                              // 2: as 1 for the first step of a restart cycle, three-term after it
   int tr_cgs_passes = 2;      // classical Gram-Schmidt passes per step
   double fp32_floor = 1e-6;   // |delta theta| that ends the FP32 phase of the mixed thick restart
+  int rot_vec = 1;            // Ritz rotation: 16-byte loads (aligned slots) instead of 4/8-byte loads
+  int tr_check_every = 0;     // FP32 thick-restart stage: convergence check every n steps inside cycles (0: at restarts)
   // H.v kernel choice: 0 = fastest available (nearest-neighbour chain, then
   // batched-load chain, then per-hop chain, then general lattice); 1 = from
   // the batched-load chain kernel on; 2 = from the per-hop chain kernel on;
@@ -53,5 +56,13 @@ double tridiag_eigenvalue(const double *d, const double *e, int n, int k);
 double tridiag_lowest(const double *d, const double *e, int n);
 void tridiag_vector(const double *d, const double *e, int n, double lam, double *z);
 void sym_eig(int n, double *A, double *w, double *V);
+// Thick-restart projection: k locked values w (couplings c to row k), then a
+// tridiagonal tail a[0..m-k), b[0..m-k-1).
+struct Arrow {
+  int k, m;
+  const double *w, *c, *a, *b;
+};
+double arrow_eigenvalue(const Arrow &A, int idx);
+void arrow_lowest_pairs(const Arrow &A, int nv, double *lam, double *Z);
 
 }  // namespace qs3x

@@ -18,6 +18,8 @@
 #include <limits>
 #include <vector>
 
+#include "blas.h"
+
 namespace qs3x {
 
 // number of eigenvalues of the symmetric tridiagonal (d[0..n-1], off-diagonal
@@ -183,9 +185,155 @@ void sym_eig(int n, double *A, double *w, double *V) {
   std::copy(V2.begin(), V2.end(), V);
 }
 
+// ------------------------------------------------ arrowhead + tridiagonal
+// The thick-restart projection of size m: rows 0..k-1 are the locked Ritz
+// values w_j coupled only to row k (c_j); rows k..m-1 are tridiagonal
+// (a[i-k] diagonal, b[i-k] coupling of i and i+1).  k = 0 is a tridiagonal.
+//
+// Inertia: eliminating the tail bottom-up (i = m-1 .. k+1), then the locked
+// rows, then row k, is an LDL^T factorisation of a symmetric permutation of
+// A - x I, so its negative pivots count the eigenvalues below x:
+//   d_{m-1} = a_{m-1} - x,  d_i = a_i - x - b_i^2 / d_{i+1},
+//   p_j = w_j - x,          d_k = a_k - x - b_k^2 / d_{k+1} - sum_j c_j^2 / p_j.
+static int arrow_count(const Arrow &A, double x, double pivmin) {
+  const int k = A.k, m = A.m;
+  auto guard = [&](double q) { return std::fabs(q) < pivmin ? -pivmin : q; };
+  int c = 0;
+  double dn = 0.0;  // pivot of row i + 1
+  for (int i = m - 1; i >= k; --i) {
+    double q = A.a[i - k] - x;
+    if (i + 1 < m) q -= A.b[i - k] * A.b[i - k] / dn;
+    if (i == k)
+      for (int j = 0; j < k; ++j) q -= A.c[j] * A.c[j] / guard(A.w[j] - x);
+    q = guard(q);
+    if (q < 0) ++c;
+    dn = q;
+  }
+  for (int j = 0; j < k; ++j)
+    if (guard(A.w[j] - x) < 0) ++c;
+  return c;
+}
+
+static void arrow_bounds(const Arrow &A, double &lo, double &hi, double &pivmin) {
+  const int k = A.k, m = A.m;
+  lo = std::numeric_limits<double>::max();
+  hi = -lo;
+  double smax = 0.0;
+  double ck = 0.0;
+  for (int j = 0; j < k; ++j) {
+    const double r = std::fabs(A.c[j]);
+    lo = std::min(lo, A.w[j] - r);
+    hi = std::max(hi, A.w[j] + r);
+    ck += r;
+    smax = std::max(smax, A.c[j] * A.c[j]);
+  }
+  for (int i = k; i < m; ++i) {
+    double r = (i > k ? std::fabs(A.b[i - k - 1]) : 0.0) + (i + 1 < m ? std::fabs(A.b[i - k]) : 0.0);
+    if (i == k) r += ck;
+    lo = std::min(lo, A.a[i - k] - r);
+    hi = std::max(hi, A.a[i - k] + r);
+    if (i + 1 < m) smax = std::max(smax, A.b[i - k] * A.b[i - k]);
+  }
+  const double eps = std::numeric_limits<double>::epsilon();
+  pivmin = std::numeric_limits<double>::min() * std::max(1.0, smax);
+  const double tn = std::max(std::fabs(lo), std::fabs(hi));
+  lo -= 2 * eps * tn + pivmin;
+  hi += 2 * eps * tn + pivmin;
+}
+
+double arrow_eigenvalue(const Arrow &A, int idx) {
+  double lo, hi, pivmin;
+  arrow_bounds(A, lo, hi, pivmin);
+  const double eps = std::numeric_limits<double>::epsilon();
+  for (int it = 0; it < 300; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (hi - lo <= 2 * eps * std::max(std::fabs(lo), std::fabs(hi)) + pivmin || mid == lo || mid == hi) break;
+    if (arrow_count(A, mid, pivmin) > idx) hi = mid;
+    else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+// Solve (A - lam I) x = r in place by the elimination order of arrow_count.
+static void arrow_solve(const Arrow &A, double lam, double tiny, std::vector<double> &x) {
+  const int k = A.k, m = A.m;
+  auto guard = [&](double q) { return std::fabs(q) < tiny ? (q >= 0 ? tiny : -tiny) : q; };
+  std::vector<double> d(m), g(m);
+  for (int i = m - 1; i >= k; --i) {
+    double q = A.a[i - k] - lam, r = x[i];
+    if (i + 1 < m) {
+      q -= A.b[i - k] * A.b[i - k] / d[i + 1];
+      r -= A.b[i - k] * g[i + 1] / d[i + 1];
+    }
+    if (i == k)
+      for (int j = 0; j < k; ++j) {
+        const double p = guard(A.w[j] - lam);
+        q -= A.c[j] * A.c[j] / p;
+        r -= A.c[j] * x[j] / p;
+      }
+    d[i] = guard(q);
+    g[i] = r;
+  }
+  // back-substitution: row k, then the locked rows, then the tail downwards
+  x[k] = g[k] / d[k];
+  for (int j = 0; j < k; ++j) x[j] = (x[j] - A.c[j] * x[k]) / guard(A.w[j] - lam);
+  for (int i = k + 1; i < m; ++i) x[i] = (g[i] - A.b[i - k - 1] * x[i - 1]) / d[i];
+}
+
+// Lowest nv eigenpairs: lam[0..nv) ascending, Z row-major m x nv (column j =
+// eigenvector j, unit norm, largest-magnitude component positive).  Inverse
+// iteration; vectors of (nearly) equal eigenvalues are orthogonalised.
+void arrow_lowest_pairs(const Arrow &A, int nv, double *lam, double *Z) {
+  const int m = A.m;
+  double lo, hi, pivmin;
+  arrow_bounds(A, lo, hi, pivmin);
+  const double anorm = std::max(std::fabs(lo), std::fabs(hi));
+  const double eps = std::numeric_limits<double>::epsilon();
+  const double tiny = eps * std::max(anorm, 1e-300);
+  std::vector<std::vector<double>> V;
+  for (int v = 0; v < nv; ++v) {
+    lam[v] = arrow_eigenvalue(A, v);
+    std::vector<double> x(m);
+    for (int i = 0; i < m; ++i) x[i] = 1.0 + 0.37 * std::sin(1.3 * (i + 1) + 0.7 * (v + 1));
+    for (int it = 0; it < 5; ++it) {
+      arrow_solve(A, lam[v], tiny, x);
+      // orthogonalise against the vectors of close eigenvalues (twice)
+      for (int pass = 0; pass < 2; ++pass)
+        for (int u = 0; u < v; ++u) {
+          if (std::fabs(lam[u] - lam[v]) > 1e-3 * std::max(anorm, 1e-300)) continue;
+          double dot = 0.0;
+          for (int i = 0; i < m; ++i) dot += V[u][i] * x[i];
+          for (int i = 0; i < m; ++i) x[i] -= dot * V[u][i];
+        }
+      double nr = 0.0;
+      for (int i = 0; i < m; ++i) nr += x[i] * x[i];
+      nr = std::sqrt(nr);
+      if (!(nr > 0)) {
+        for (int i = 0; i < m; ++i) x[i] = (i == v % m) ? 1.0 : 0.0;
+        continue;
+      }
+      for (int i = 0; i < m; ++i) x[i] /= nr;
+    }
+    int imax = 0;
+    for (int i = 1; i < m; ++i)
+      if (std::fabs(x[i]) > std::fabs(x[imax])) imax = i;
+    const double sg = x[imax] < 0 ? -1.0 : 1.0;
+    for (int i = 0; i < m; ++i) x[i] *= sg;
+    for (int i = 0; i < m; ++i) Z[(size_t)i * nv + v] = x[i];
+    V.push_back(std::move(x));
+  }
+}
+
 }  // namespace qs3x
 
 extern "C" {
+// Test hook: lowest nv eigenpairs of the arrowhead + tridiagonal matrix (k
+// locked values w, couplings c; tail diagonal a (m - k), off-diagonal b (m - k - 1)).
+void qs3x_test_arrow_pairs(int k, int m, const double *w, const double *c, const double *a, const double *b,
+                           int nv, double *lam, double *Z) {
+  qs3x::Arrow A{k, m, w, c, a, b};
+  qs3x::arrow_lowest_pairs(A, nv, lam, Z);
+}
 // Exposed for tests of the host eigensolvers (not part of include/qs3x.h's
 // solver contract; used by tests/test_cpu_eig.py).
 double qs3x_test_tridiag_eigenvalue(const double *d, const double *e, int n, int k) {

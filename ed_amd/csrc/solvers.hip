@@ -289,6 +289,19 @@ struct TR {
     finalize(part.p, nb, sc.p, 2, Bt() + i, S() + i + 1, 0.0, st);
   }
 
+  // One Lanczos step from slot i into slot i+1 by the three-term recurrence
+  // (no Gram-Schmidt): w = H v_i - alpha_i v_i - beta_{i-1} v_{i-1}, with
+  // alpha from the fused dot of the H.v and beta_{i-1} from the previous step.
+  void step3(int i) {
+    int nb = 0;
+    launch_hv(s, prec, slot(i + 1), slot(i), sc.p + S() + i, part.p, &nb, st);
+    ++hv_count;
+    finalize(part.p, nb, sc.p, 1, A() + i, TMP(), shift, st);
+    nb = vec_update3(prec, slot(i + 1), slot(i), i > 0 ? slot(i - 1) : slot(i), dim, sc.p, A() + i, S() + i,
+                     i > 0 ? Bt() + i - 1 : -1, i > 0 ? S() + i - 1 : -1, part.p, st);
+    finalize(part.p, nb, sc.p, 2, Bt() + i, S() + i + 1, 0.0, st);
+  }
+
   // alpha'[i0..i1), beta[i0..i1) to host
   void read_ab(int i0, int i1, std::vector<double> &a, std::vector<double> &b) {
     std::vector<double> all(sc.n);
@@ -297,13 +310,10 @@ struct TR {
     b.assign(all.begin() + Bt() + i0, all.begin() + Bt() + i1);
   }
 
-  // slots 0..nok-1 := V[0..NOM-1] C[:, 0..nok-1] (C row-major NOM x NOM on host)
-  void rotate(const std::vector<double> &C) {
-    std::vector<double> cc((size_t)NOM * NOK);
-    for (int j = 0; j < NOM; ++j)
-      for (int k = 0; k < NOK; ++k) cc[(size_t)j * NOK + k] = C[(size_t)j * NOM + k];
-    Cdev.upload(cc.data(), cc.size());
-    ritz_rotate(prec, basis.p, dim, sc.p + S(), NOM, NOK, Cdev.p, st);
+  // slots 0..NOK-1 := V[0..m-1] Z (Z row-major m x NOK on host, m <= NOM)
+  void rotate(const std::vector<double> &Z, int m) {
+    Cdev.upload(Z.data(), (size_t)m * NOK);
+    ritz_rotate(prec, basis.p, dim, sc.p + S(), m, NOK, Cdev.p, st);
     std::vector<double> ones(NOK, 1.0);
     QS3X_HIP(hipMemcpyAsync(sc.p + S(), ones.data(), sizeof(double) * NOK, hipMemcpyHostToDevice, st));
     QS3X_HIP(hipStreamSynchronize(st));
@@ -327,6 +337,36 @@ struct TRResult {
 // abs_floor > 0 (FP32 phase): converged when the nwant lowest Ritz values
 // moved less than abs_floor (or tol relative) over two consecutive restarts;
 // abs_floor = 0: the reference's test over all NOK values (eigen_solver.f90:150-162).
+// Lowest nok eigenpairs of the projection (k locked values w with couplings
+// cb to row k, tridiagonal tail a, b; m rows): Jacobi on the dense matrix for
+// small m (the path of the reference's zheevr call), the structured Sturm /
+// inverse-iteration solver of eig.cpp otherwise.  Z is row-major m x nok.
+void projected_pairs(int k, int m, const std::vector<double> &w, const std::vector<double> &cb,
+                     const std::vector<double> &a, const std::vector<double> &b, int nok, std::vector<double> &lam,
+                     std::vector<double> &Z) {
+  lam.assign(nok, 0.0);
+  Z.assign((size_t)m * nok, 0.0);
+  if (m <= 96) {
+    std::vector<double> T((size_t)m * m, 0.0), C((size_t)m * m), ww(m);
+    for (int j = 0; j < k; ++j) {
+      T[(size_t)j * m + j] = w[j];
+      T[(size_t)j * m + k] = T[(size_t)k * m + j] = cb[j];
+    }
+    for (int i = k; i < m; ++i) {
+      T[(size_t)i * m + i] = a[i - k];
+      if (i + 1 < m) T[(size_t)i * m + i + 1] = T[(size_t)(i + 1) * m + i] = b[i - k];
+    }
+    sym_eig(m, T.data(), ww.data(), C.data());
+    for (int j = 0; j < nok; ++j) {
+      lam[j] = ww[j];
+      for (int i = 0; i < m; ++i) Z[(size_t)i * nok + j] = C[(size_t)i * m + j];
+    }
+    return;
+  }
+  Arrow A{k, m, k ? w.data() : nullptr, k ? cb.data() : nullptr, a.data(), b.data()};
+  arrow_lowest_pairs(A, nok, lam.data(), Z.data());
+}
+
 TRResult tr_cycles(TR &t, int64_t maxitr, double tol, double abs_floor, std::vector<double> &theta,
                    std::vector<double> &resid, int nwant = 0) {
   const int NOK = t.NOK, NOM = t.NOM;
@@ -336,18 +376,14 @@ TRResult tr_cycles(TR &t, int64_t maxitr, double tol, double abs_floor, std::vec
   for (int i = 0; i < NOM; ++i) t.step(i, 0);
   std::vector<double> a, b;
   t.read_ab(0, NOM, a, b);
-  std::vector<double> T((size_t)NOM * NOM, 0.0), C((size_t)NOM * NOM), w(NOM);
-  for (int i = 0; i < NOM; ++i) {
-    T[(size_t)i * NOM + i] = a[i];
-    if (i + 1 < NOM) T[(size_t)i * NOM + i + 1] = T[(size_t)(i + 1) * NOM + i] = b[i];
-  }
+  std::vector<double> w, C, none;
   double beta_last = b[NOM - 1];
-  sym_eig(NOM, T.data(), w.data(), C.data());
+  projected_pairs(0, NOM, none, none, a, b, NOK, w, C);
   std::vector<double> prev(a.begin(), a.begin() + NOK);  // calpha(1:NOK) = alpha (eigen_solver.f90:293-294)
   int flag = 0;
   int64_t tritr = 0;
   for (tritr = 1; tritr <= maxitr; ++tritr) {
-    t.rotate(C);
+    t.rotate(C, NOM);
     double mc = 0.0;
     for (int j = 0; j < ncheck; ++j) {
       const double e = w[j] + t.shift, pe = prev[j] + t.shift;
@@ -369,27 +405,103 @@ TRResult tr_cycles(TR &t, int64_t maxitr, double tol, double abs_floor, std::vec
     if (tritr == maxitr) break;
     t.copy_slot(NOK, NOM);
     for (int j = 0; j < NOK; ++j) prev[j] = w[j];
-    std::vector<double> cb(NOK);
-    for (int j = 0; j < NOK; ++j) cb[j] = beta_last * C[(size_t)(NOM - 1) * NOM + j];
+    std::vector<double> cb(NOK), wl(w.begin(), w.begin() + NOK);
+    for (int j = 0; j < NOK; ++j) cb[j] = beta_last * C[(size_t)(NOM - 1) * NOK + j];
     for (int i = NOK; i < NOM; ++i) t.step(i, 1);
     t.read_ab(NOK, NOM, a, b);
-    std::fill(T.begin(), T.end(), 0.0);
-    for (int j = 0; j < NOK; ++j) {
-      T[(size_t)j * NOM + j] = w[j];
-      T[(size_t)j * NOM + NOK] = T[(size_t)NOK * NOM + j] = cb[j];
-    }
-    for (int i = NOK; i < NOM; ++i) {
-      T[(size_t)i * NOM + i] = a[i - NOK];
-      if (i + 1 < NOM) T[(size_t)i * NOM + i + 1] = T[(size_t)(i + 1) * NOM + i] = b[i - NOK];
-    }
     beta_last = b[NOM - 1 - NOK];
-    sym_eig(NOM, T.data(), w.data(), C.data());
+    projected_pairs(NOK, NOM, wl, cb, a, b, NOK, w, C);
   }
   r.restarts = (int)std::min<int64_t>(tritr, maxitr);
-  theta.resize(NOM);
-  for (int j = 0; j < NOM; ++j) theta[j] = w[j] + t.shift;
+  theta.resize(NOK);
+  for (int j = 0; j < NOK; ++j) theta[j] = w[j] + t.shift;
   resid.resize(NOK);
-  for (int j = 0; j < NOK; ++j) resid[j] = std::fabs(beta_last * C[(size_t)(NOM - 1) * NOM + j]);
+  for (int j = 0; j < NOK; ++j) resid[j] = std::fabs(beta_last * C[(size_t)(NOM - 1) * NOK + j]);
+  return r;
+}
+
+// Thick restart with convergence checks inside the cycles (an extension of
+// eigen_solver.f90:91-216, used by the FP32 stage of the mixed solver when
+// option tr_check_every > 0): every itrint steps the nwant lowest eigenvalues
+// of the projection so far (structured Sturm bisection, O(m) per count) are
+// compared with the previous check; when they moved less than tol (relative)
+// or abs_floor (absolute; 0 disables), the cycle ends there.  Then (or at NOM)
+// the NOK lowest Ritz pairs are rotated into slots 0..NOK-1 as in tr_cycles.
+// In the first cycle, with tr_reorth >= 1, each step is the three-term
+// recurrence (no Gram-Schmidt); restart cycles keep the locked + local CGS of
+// step() (tr_reorth 1) or use it only for their first step (tr_reorth 2).
+TRResult tr_cycles_checked(TR &t, int64_t maxitr, double tol, double abs_floor, int itrint,
+                           std::vector<double> &theta, std::vector<double> &resid, int nwant) {
+  const int NOK = t.NOK, NOM = t.NOM;
+  const int nw = std::max(1, std::min(nwant, NOK));
+  TRResult r;
+  t.set_norm_scale(0);
+  std::vector<double> wl, cb, a, b, lam, Z, prev;
+  double beta_last = 0.0;
+  int k0 = 0;
+  int64_t cycle = 0;
+  for (cycle = 1; cycle <= maxitr; ++cycle) {
+    const int i0 = k0;
+    int m_end = NOM;
+    bool conv = false;
+    for (int i = i0; i < NOM; ++i) {
+      // tr_reorth 1: three-term first cycle; 2: also every restart-cycle step
+      // after the first (the only one coupled to the locked vectors in exact arithmetic)
+      if ((k0 == 0 && t.reorth >= 1) || (t.reorth == 2 && i > k0)) t.step3(i);
+      else t.step(i, k0 == 0 ? 0 : 1);
+      const int m = i + 1;
+      if ((m - i0) % itrint != 0 || m < NOK + 1) continue;
+      t.read_ab(i0, m, a, b);
+      Arrow A{k0, m, k0 ? wl.data() : nullptr, k0 ? cb.data() : nullptr, a.data(), b.data()};
+      std::vector<double> ev(nw);
+      for (int j = 0; j < nw; ++j) ev[j] = arrow_eigenvalue(A, j);
+      double mc = prev.empty() ? 1.0 : 0.0;
+      for (int j = 0; j < nw && !prev.empty(); ++j) {
+        const double e = ev[j] + t.shift, pe = prev[j] + t.shift;
+        double ch = std::fabs(pe / e - 1.0);
+        if (abs_floor > 0) ch = std::min(ch, std::fabs(pe - e) / abs_floor * tol);
+        mc = std::max(mc, ch);
+      }
+      prev = ev;
+      trace_point(t.t_origin, t.hv_base + t.hv_count, ev[0] + t.shift, t.prec);
+      if (options().verbose > 1)
+        std::fprintf(stderr, "[qs3x] cycle %lld step %d: theta0=%.15g change %.3e\n", (long long)cycle, m,
+                     ev[0] + t.shift, mc);
+      r.max_change = mc;
+      if (mc < tol) {
+        conv = true;
+        m_end = m;
+        break;
+      }
+    }
+    t.read_ab(i0, m_end, a, b);
+    beta_last = b[m_end - 1 - i0];
+    projected_pairs(k0, m_end, wl, cb, a, b, NOK, lam, Z);
+    t.rotate(Z, m_end);
+    if (options().verbose)
+      std::fprintf(stderr, "[qs3x] cycle %lld (%s): %d steps, theta0=%.15g, %lld H.v%s\n", (long long)cycle,
+                   t.prec == QS3X_FP32 ? "fp32" : "fp64", m_end - i0, lam[0] + t.shift, (long long)t.hv_count,
+                   conv ? ", converged" : "");
+    if (conv) {
+      r.converged = true;
+      break;
+    }
+    if (cycle == maxitr) break;
+    // restart: the residual vector becomes slot NOK, the locked Ritz values
+    // couple to it through beta_last * z_{m-1, j}
+    t.copy_slot(NOK, m_end);
+    wl.assign(lam.begin(), lam.begin() + NOK);
+    cb.resize(NOK);
+    for (int j = 0; j < NOK; ++j) cb[j] = beta_last * Z[(size_t)(m_end - 1) * NOK + j];
+    k0 = NOK;
+    prev.clear();
+  }
+  r.restarts = (int)std::min<int64_t>(cycle, maxitr);
+  theta.resize(NOK);
+  for (int j = 0; j < NOK; ++j) theta[j] = lam[j] + t.shift;
+  const int m_last = (int)(Z.size() / NOK);
+  resid.resize(NOK);
+  for (int j = 0; j < NOK; ++j) resid[j] = std::fabs(beta_last * Z[(size_t)(m_last - 1) * NOK + j]);
   return r;
 }
 
@@ -552,7 +664,10 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
       // FP32 phase: until the NOE lowest Ritz values stop moving at the FP32
       // resolution of the shifted spectrum
       const double floor32 = options().fp32_floor;
-      TRResult r = tr_cycles(t32s, maxitr, std::max(conv, 1e-30), floor32, theta, resid, NOE);
+      const int every = options().tr_check_every;
+      TRResult r = every > 0
+                       ? tr_cycles_checked(t32s, maxitr, std::max(conv, 1e-30), 0.0, every, theta, resid, NOE)
+                       : tr_cycles(t32s, maxitr, std::max(conv, 1e-30), floor32, theta, resid, NOE);
       restarts += r.restarts;
       hv32 = t32s.hv_count;
       QS3X_HIP(hipStreamSynchronize(st));
@@ -560,7 +675,10 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
       // FP64 Rayleigh-Ritz on the NOK FP32 Ritz vectors.  The FP64 stage
       // always reorthogonalises fully (CGS2): started next to an eigenvector,
       // local orthogonalisation loses the projection property in one cycle.
-      t64.init(s, QS3X_FP64, NOK, NOM, 0, 2);
+      // the FP64 stage holds the Rayleigh-Ritz block and, if needed, an FP64
+      // thick restart of at most 64 vectors (the FP32 basis may be far longer)
+      const int NOM64 = std::max(std::min(NOM, 64), 2 * NOK + NOE);
+      t64.init(s, QS3X_FP64, NOK, NOM64, 0, 2);
       t64.t_origin = t0;
       t64.hv_base = hv32;
       for (int j = 0; j < NOK; ++j)

@@ -44,7 +44,9 @@ int qs3x_set_device(int32_t device);
  *   "tr_reorth"     0 (default): the thick restart reorthogonalises every new
  *                   Lanczos vector against all previous basis vectors, as
  *                   eigen_solver.f90:233-239 does; 1: against the locked Ritz
- *                   vectors and the last two Lanczos vectors only.
+ *                   vectors and the last two Lanczos vectors only; 2 (with
+ *                   tr_check_every > 0): as 1 for the first step of a restart
+ *                   cycle, the three-term recurrence for the others.
  *   "tr_cgs_passes" classical Gram-Schmidt passes per step (default 2).
  *   "fp32_floor"    Ritz-value change (absolute) that ends the FP32 phase of
  *                   the mixed thick restart (default 1e-6).
@@ -52,6 +54,13 @@ int qs3x_set_device(int32_t device);
  *                   neighbour chain, batched-load chain, per-hop chain, general
  *                   lattice, in that order); 1, 2, 3: start the list at the
  *                   batched-load chain, per-hop chain or general kernel.
+ *   "tr_check_every" FP32 stage of the mixed thick restart: 0 (default) checks
+ *                   convergence at restarts as the reference does; n > 0
+ *                   checks the projection every n steps inside a cycle and
+ *                   ends the cycle there (with tr_reorth 1 its first cycle is
+ *                   the plain three-term recurrence).  Allows NOM up to 511.
+ *   "rot_vec"       Ritz rotation with 16-byte basis loads where the slots are
+ *                   16-byte aligned (1, default) or element loads (0).
  *   "hv_xcd"        L2 domains (XCDs) for the H.v tile schedule: each gets one
  *                   contiguous range of rows (1, the default: plain round-robin
  *                   over workgroups; 8 for MI355X's XCDs).

@@ -96,3 +96,38 @@ def test_start_vector_formula_is_deterministic():
     assert np.array_equal(a[500:], b)
     assert np.all(np.abs(a.real) < 1) and np.all(np.abs(a.imag) < 1)
     assert abs(np.mean(a.real)) < 0.1 and abs(np.mean(a.imag)) < 0.1
+
+
+def test_arrowhead_lowest_pairs(lib):
+    """Structured eigensolver of the thick-restart projection (locked Ritz
+    values w coupled to row k, tridiagonal tail) against numpy's eigh, including
+    a pure tridiagonal (k = 0), a degenerate locked block and a large size."""
+    lib.qs3x_test_arrow_pairs.restype = None
+    lib.qs3x_test_arrow_pairs.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4 + [ctypes.c_int] + \
+        [ctypes.c_void_p] * 2
+    rng = np.random.default_rng(5)
+    for k, m, nv, degen in ((0, 40, 4, False), (4, 60, 4, False), (8, 300, 8, False), (4, 50, 4, True),
+                            (2, 3, 2, False), (0, 1, 1, False), (1, 2, 1, False)):
+        w = np.sort(rng.normal(size=k)) if k else np.zeros(1)
+        if degen and k >= 2:
+            w[1] = w[0]
+        c = rng.normal(size=max(k, 1))
+        a = rng.normal(size=m - k) if m > k else np.zeros(1)
+        b = rng.uniform(0.1, 1.0, size=max(m - k - 1, 1))
+        A = np.zeros((m, m))
+        for j in range(k):
+            A[j, j] = w[j]
+            A[j, k] = A[k, j] = c[j]
+        for i in range(k, m):
+            A[i, i] = a[i - k]
+            if i + 1 < m:
+                A[i, i + 1] = A[i + 1, i] = b[i - k]
+        lam = np.zeros(nv)
+        Z = np.zeros((m, nv))
+        lib.qs3x_test_arrow_pairs(k, m, w.ctypes.data, c.ctypes.data, a.ctypes.data, b.ctypes.data, nv,
+                                  lam.ctypes.data, Z.ctypes.data)
+        ev = np.linalg.eigvalsh(A)
+        assert np.allclose(lam, ev[:nv], atol=1e-12 * max(1, np.max(np.abs(ev)))), (k, m, lam, ev[:nv])
+        res = A @ Z - Z * lam
+        assert np.max(np.abs(res)) < 1e-9, (k, m, np.max(np.abs(res)))
+        assert np.allclose(Z.T @ Z, np.eye(nv), atol=1e-9)

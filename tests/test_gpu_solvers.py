@@ -90,17 +90,46 @@ def test_thick_restart(prec_name):
         assert r < 1e-5, r
 
 
-def test_ritz_rotation_mfma():
-    """The FP64 MFMA rotation equals numpy's V[:, :nom] @ C for both storages."""
-    print("[gpu] ritz rotation", flush=True)
+@pytest.mark.parametrize("nom", [40, 150])
+def test_thick_restart_checked_fp32(nom):
+    """FP32 stage with convergence checks inside the cycles (tr_check_every),
+    three-term first cycle (tr_reorth 1), long cycles (NOM 150 uses the
+    structured projected eigensolver), then the FP64 Rayleigh-Ritz step."""
+    print(f"[gpu] thick restart checked fp32 NOM={nom}", flush=True)
     q3 = _q3()
+    om, osec, gm, gsec, rk = _setup(20, 5, [(1, 1.0, 1.0)], K=0)
+    w = np.linalg.eigvalsh(om.full_hamiltonian(osec))
+    q3.set_option("tr_check_every", 5)
+    q3.set_option("tr_reorth", 1)
+    q3.set_option("tr_cgs_passes", 1)
+    try:
+        ene, psi, st = q3.my_trlanczos_routines_complex(gsec, NOE=1, NOK=4, NOM=nom, maxitr=200,
+                                                         lnc_ene_conv=1e-12, prec=q3.FP32, seed=3)
+    finally:
+        q3.set_option("tr_check_every", 0)
+        q3.set_option("tr_reorth", 0)
+        q3.set_option("tr_cgs_passes", 2)
+    assert abs(ene[0] - w[0]) <= 1e-9 * abs(w[0]), (ene[0], w[0], st)
+    H = om.full_hamiltonian(osec)
+    r = np.linalg.norm(H @ psi[:, 0] - ene[0] * psi[:, 0])
+    assert r < 1e-4, r
+
+
+@pytest.mark.parametrize("rot_vec", [1, 0])
+def test_ritz_rotation_mfma(rot_vec):
+    """The FP64 MFMA rotation equals numpy's V[:, :nom] @ C for both storages,
+    with 16-byte basis loads (aligned slots: every FP64 case, even dims in
+    FP32) and with element loads."""
+    print(f"[gpu] ritz rotation rot_vec={rot_vec}", flush=True)
+    q3 = _q3()
+    q3.set_option("rot_vec", rot_vec)
     L = q3.lib()
     L.qs3x_test_ritz_rotate.restype = ctypes.c_int
     L.qs3x_test_ritz_rotate.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
                                         ctypes.c_int32, ctypes.c_void_p]
     rng = np.random.default_rng(9)
     for dt, tol in ((np.complex128, 1e-13), (np.complex64, 2e-6)):
-        for dim, nom, nok in ((1000, 12, 5), (777, 30, 17), (5, 8, 4), (4099, 41, 33)):
+        for dim, nom, nok in ((1000, 12, 5), (777, 30, 17), (5, 8, 4), (4099, 41, 33), (4098, 64, 64), (130, 96, 48)):
             V = (rng.normal(size=(nom + 1, dim)) + 1j * rng.normal(size=(nom + 1, dim))).astype(dt)
             C = rng.normal(size=(nom, nok))
             ref = (C.T @ V[:nom].astype(np.complex128))
@@ -112,6 +141,7 @@ def test_ritz_rotation_mfma():
             got = d.cpu().numpy()
             assert np.max(np.abs(got[:nok] - ref)) <= tol * np.max(np.abs(ref))
             assert np.array_equal(got[nok:], V[nok:])  # untouched columns
+    q3.set_option("rot_vec", 1)
 
 
 @pytest.mark.parametrize("geom", ["chain", "torus"])
