@@ -7,7 +7,7 @@ namespace qs3x {
 
 constexpr int kVecThreads = 256;
 constexpr int kMaxGroup = 8;    // basis vectors per pass of the multi-dot kernel
-constexpr int kMaxBasis = 512;  // largest Krylov basis (NOM + 1)
+constexpr int kMaxBasis = 1024;  // largest Krylov basis (NOM + 1)
 constexpr int kMaxBlocks = 4096;  // upper bound of every reduction grid (vec_grid, H.v)
 
 int vec_grid(int64_t n);

@@ -75,23 +75,21 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_mfma(R *base, int64_t M 
 }
 
 // Same product with 16-byte loads: lane (li, lk) reads SUB = 16 / sizeof(R)
-// consecutive rows row0 + SUB li .. + SUB - 1 of basis vector k0 + lk, and
+// consecutive rows row0 + SUB li .. + SUB - 1 of basis vector k + lk, and
 // each of the SUB values is the A operand of its own MFMA (the tile's row i =
-// li stands for physical row row0 + SUB li + s).  A wavefront then covers
-// 16 SUB rows per K step and each load instruction reads 4 segments of
-// 16 x 16 bytes.  Needs 16-byte aligned slots (M * sizeof(R) a multiple of 16).
-template <class R, int NT>
+// li stands for physical row row0 + SUB li + s).  A wavefront covers 16 SUB
+// rows and issues the loads of U K-steps (4U basis vectors) before their
+// MFMAs, so that U 16-byte loads per lane are in flight.  C (scaled) is staged
+// compactly as [nom][nok] (B = 0 outside), which keeps the LDS small enough
+// for several blocks per CU.  Needs 16-byte aligned slots (M * sizeof(R) a
+// multiple of 16).
+template <class R, int NT, int U>
 __global__ __launch_bounds__(kRotThreads) void k_rotate_mfma_v(R *base, int64_t M, const double *scales, int nom,
                                                                int nok, const double *C) {
   constexpr int SUB = 16 / (int)sizeof(R);
   using V4 = typename std::conditional<sizeof(R) == 4, float4, double2>::type;
-  extern __shared__ double cs[];  // [nomp][NT*16]
-  const int nomp = (nom + 3) & ~3;
-  const int ncol = NT * 16;
-  for (int q = threadIdx.x; q < nomp * ncol; q += blockDim.x) {
-    int k = q / ncol, n = q % ncol;
-    cs[q] = (k < nom && n < nok) ? C[(size_t)k * nok + n] * (scales ? scales[k] : 1.0) : 0.0;
-  }
+  extern __shared__ double cs[];  // [nom][nok]
+  for (int q = threadIdx.x; q < nom * nok; q += blockDim.x) cs[q] = C[q] * (scales ? scales[q / nok] : 1.0);
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -103,31 +101,41 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_mfma_v(R *base, int64_t 
        tile += (int64_t)gridDim.x * waves_per_block) {
     const int64_t row0 = tile * rows_per_tile;
     const int64_t r = row0 + SUB * li;
+    const bool rin = r < M;
     d4 acc[SUB][NT];
 #pragma unroll
     for (int q = 0; q < SUB; ++q)
 #pragma unroll
       for (int t = 0; t < NT; ++t) acc[q][t] = d4{0.0, 0.0, 0.0, 0.0};
-    for (int k0 = 0; k0 < nomp; k0 += 4) {
-      const int k = k0 + lk;
-      V4 a;
-      if (k < nom && r < M) a = *(const V4 *)(base + (size_t)k * M + r);
-      else a = V4{};
-      double av[SUB];
-      if constexpr (SUB == 4) {
-        av[0] = a.x;
-        av[1] = a.y;
-        av[2] = a.z;
-        av[3] = a.w;
-      } else {
-        av[0] = a.x;
-        av[1] = a.y;
+    for (int k0 = 0; k0 < nom; k0 += 4 * U) {
+      V4 a[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + 4 * u + lk;
+        if (k < nom && rin) a[u] = *(const V4 *)(base + (size_t)k * M + r);
+        else a[u] = V4{};
       }
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const double b = cs[k * ncol + t * 16 + li];
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + 4 * u + lk;
+        double av[SUB];
+        if constexpr (SUB == 4) {
+          av[0] = a[u].x;
+          av[1] = a[u].y;
+          av[2] = a[u].z;
+          av[3] = a[u].w;
+        } else {
+          av[0] = a[u].x;
+          av[1] = a[u].y;
+        }
 #pragma unroll
-        for (int q = 0; q < SUB; ++q) acc[q][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], b, acc[q][t], 0, 0, 0);
+        for (int t = 0; t < NT; ++t) {
+          const int col = t * 16 + li;
+          const double b = (k < nom && col < nok) ? cs[k * nok + col] : 0.0;
+#pragma unroll
+          for (int q = 0; q < SUB; ++q)
+            acc[q][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], b, acc[q][t], 0, 0, 0);
+        }
       }
     }
 #pragma unroll
@@ -155,22 +163,23 @@ void ritz_rotate(int prec, void *base, int64_t dim, const double *scales_dev, in
   const int waves = kRotThreads / 64;
   int grid = (int)std::min<int64_t>((ntiles + waves - 1) / waves, (int64_t)vec_grid(dim) * 2);
   grid = std::max(grid, 1);
-  const size_t lds = sizeof(double) * (size_t)((nom + 3) & ~3) * NT * 16;
-  QS3X_REQUIRE(lds <= 64 * 1024, "ritz_rotate: NOM x NOK too large for LDS staging");
+  size_t lds = sizeof(double) * (size_t)((nom + 3) & ~3) * NT * 16;
   const size_t rsz = prec == QS3X_FP64 ? sizeof(double) : sizeof(float);
   const bool vec = ((uintptr_t)base % 16 == 0) && ((M * rsz) % 16 == 0) && options().rot_vec;
   if (vec) {
+    lds = sizeof(double) * (size_t)nom * nok;
     const int64_t vt = (M + 16 * (16 / (int64_t)rsz) - 1) / (16 * (16 / (int64_t)rsz));
     grid = (int)std::max<int64_t>(1, std::min<int64_t>((vt + waves - 1) / waves, (int64_t)vec_grid(dim) * 2));
   }
-#define ROT(R, NTV)                                                                                      \
-  do {                                                                                                   \
-    if (vec)                                                                                             \
-      hipLaunchKernelGGL((k_rotate_mfma_v<R, NTV>), dim3(grid), dim3(kRotThreads), lds, st, (R *)base, M, \
-                         scales_dev, nom, nok, C_dev);                                                   \
-    else                                                                                                 \
-      hipLaunchKernelGGL((k_rotate_mfma<R, NTV>), dim3(grid), dim3(kRotThreads), lds, st, (R *)base, M,   \
-                         scales_dev, nom, nok, C_dev);                                                   \
+  QS3X_REQUIRE(lds <= 64 * 1024, "ritz_rotate: NOM x NOK too large for LDS staging");
+#define ROT(R, NTV)                                                                                           \
+  do {                                                                                                        \
+    if (vec)                                                                                                  \
+      hipLaunchKernelGGL((k_rotate_mfma_v<R, NTV, 4>), dim3(grid), dim3(kRotThreads), lds, st, (R *)base, M, \
+                         scales_dev, nom, nok, C_dev);                                                        \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_rotate_mfma<R, NTV>), dim3(grid), dim3(kRotThreads), lds, st, (R *)base, M,        \
+                         scales_dev, nom, nok, C_dev);                                                        \
   } while (0)
   if (prec == QS3X_FP64) {
     switch (NT) {

@@ -225,7 +225,14 @@ struct TR {
     dim = s->dim;
     st = stream();
     shift = (prec == QS3X_FP32) ? s->model->ezz : 0.0;
-    basis.alloc((size_t)(NOM + 1) * dim * elem_bytes(prec));
+    {
+      const double ta = now_s();
+      basis.alloc((size_t)(NOM + 1) * dim * elem_bytes(prec));
+      if (options().verbose)
+        std::fprintf(stderr, "[qs3x] thick-restart basis: %d x %lld %s elements, %.1f GB, allocated in %.3f s\n",
+                     NOM + 1, (long long)dim, prec == QS3X_FP32 ? "complex-float" : "complex-double",
+                     1e-9 * (double)(NOM + 1) * (double)dim * (double)elem_bytes(prec), now_s() - ta);
+    }
     sc.alloc((size_t)3 * NOM + 8);
     const int nb = vec_grid(dim);
     part.alloc((size_t)2 * std::max(nb, 1) * (NOM + 2));
@@ -425,7 +432,8 @@ TRResult tr_cycles(TR &t, int64_t maxitr, double tol, double abs_floor, std::vec
 // option tr_check_every > 0): every itrint steps the nwant lowest eigenvalues
 // of the projection so far (structured Sturm bisection, O(m) per count) are
 // compared with the previous check; when they moved less than tol (relative)
-// or abs_floor (absolute; 0 disables), the cycle ends there.  Then (or at NOM)
+// or abs_floor (absolute; 0 disables) and the extrapolated remaining change
+// is below tol too, the cycle ends there.  Then (or at NOM)
 // the NOK lowest Ritz pairs are rotated into slots 0..NOK-1 as in tr_cycles.
 // In the first cycle, with tr_reorth >= 1, each step is the three-term
 // recurrence (no Gram-Schmidt); restart cycles keep the locked + local CGS of
@@ -437,7 +445,7 @@ TRResult tr_cycles_checked(TR &t, int64_t maxitr, double tol, double abs_floor, 
   TRResult r;
   t.set_norm_scale(0);
   std::vector<double> wl, cb, a, b, lam, Z, prev;
-  double beta_last = 0.0;
+  double beta_last = 0.0, mc_prev = -1.0;
   int k0 = 0;
   int64_t cycle = 0;
   for (cycle = 1; cycle <= maxitr; ++cycle) {
@@ -455,7 +463,8 @@ TRResult tr_cycles_checked(TR &t, int64_t maxitr, double tol, double abs_floor, 
       Arrow A{k0, m, k0 ? wl.data() : nullptr, k0 ? cb.data() : nullptr, a.data(), b.data()};
       std::vector<double> ev(nw);
       for (int j = 0; j < nw; ++j) ev[j] = arrow_eigenvalue(A, j);
-      double mc = prev.empty() ? 1.0 : 0.0;
+      const bool first_check = prev.empty();
+      double mc = first_check ? 1.0 : 0.0;
       for (int j = 0; j < nw && !prev.empty(); ++j) {
         const double e = ev[j] + t.shift, pe = prev[j] + t.shift;
         double ch = std::fabs(pe / e - 1.0);
@@ -468,7 +477,21 @@ TRResult tr_cycles_checked(TR &t, int64_t maxitr, double tol, double abs_floor, 
         std::fprintf(stderr, "[qs3x] cycle %lld step %d: theta0=%.15g change %.3e\n", (long long)cycle, m,
                      ev[0] + t.shift, mc);
       r.max_change = mc;
+      // Stop when the change is below tol and so is the remaining error
+      // estimated from the last two changes (geometric tail
+      // change * rho / (1 - rho), rho = change / previous change), or when
+      // the change is below tol / 10 (fast convergence or the Ritz values
+      // have reached the working precision's resolution).
+      bool stop = false;
       if (mc < tol) {
+        if (mc < 0.1 * tol) stop = true;
+        else if (mc_prev > 0.0) {
+          const double rho = mc / mc_prev;
+          stop = rho < 1.0 && mc * rho / (1.0 - rho) < tol;
+        }
+      }
+      mc_prev = first_check ? -1.0 : mc;
+      if (stop) {
         conv = true;
         m_end = m;
         break;
@@ -495,6 +518,7 @@ TRResult tr_cycles_checked(TR &t, int64_t maxitr, double tol, double abs_floor, 
     for (int j = 0; j < NOK; ++j) cb[j] = beta_last * Z[(size_t)(m_end - 1) * NOK + j];
     k0 = NOK;
     prev.clear();
+    mc_prev = -1.0;
   }
   r.restarts = (int)std::min<int64_t>(cycle, maxitr);
   theta.resize(NOK);

@@ -57,8 +57,11 @@ int qs3x_set_device(int32_t device);
  *   "tr_check_every" FP32 stage of the mixed thick restart: 0 (default) checks
  *                   convergence at restarts as the reference does; n > 0
  *                   checks the projection every n steps inside a cycle and
- *                   ends the cycle there (with tr_reorth 1 its first cycle is
- *                   the plain three-term recurrence).  Allows NOM up to 511.
+ *                   ends the cycle there, once the change is below conv and
+ *                   so is the remaining change extrapolated from the last two
+ *                   (with tr_reorth >= 1 its first cycle is the plain
+ *                   three-term recurrence).  NOM is at most 1023 (and
+ *                   NOM x NOK at most 8192 for the rotation's staging).
  *   "rot_vec"       Ritz rotation with 16-byte basis loads where the slots are
  *                   16-byte aligned (1, default) or element loads (0).
  *   "hv_xcd"        L2 domains (XCDs) for the H.v tile schedule: each gets one
