@@ -50,7 +50,7 @@ def cpu_baseline(sec, budget_s=12.0):
     v = (rng.uniform(-1, 1, sec.dim) + 1j * rng.uniform(-1, 1, sec.dim)).astype(np.complex128)
     threads = orc.lib().orc_max_threads()
     chunk = 4096
-    starts = np.linspace(0, sec.dim - chunk, 64).astype(np.int64)
+    starts = np.linspace(0, sec.dim - chunk, 256).astype(np.int64)
     rows = 0
     t0 = time.perf_counter()
     for s in starts:
