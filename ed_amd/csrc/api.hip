@@ -83,6 +83,9 @@ int qs3x_set_option(const char *name, double value) {
     } else if (n == "verbose") {
       QS3X_REQUIRE(value >= 0 && value <= 2, "verbose must be 0..2");
       options().verbose = (int)value;
+    } else if (n == "fp64_refine") {
+      QS3X_REQUIRE(value >= 0 && value <= 32, "fp64_refine must be 0..32");
+      options().fp64_refine = (int)value;
     } else if (n == "rot_vec") {
       QS3X_REQUIRE(value == 0 || value == 1, "rot_vec must be 0 or 1");
       options().rot_vec = (int)value;

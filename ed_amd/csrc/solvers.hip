@@ -654,6 +654,64 @@ RRResult rayleigh_ritz(TR &t, int k, int nwant) {
   return r;
 }
 
+// Final FP64 refinement of p accepted Ritz vectors (slots src0 .. src0+p-1,
+// src0 >= p): Rayleigh-Ritz on the block Krylov space span{Y, HY, ..., H^S Y}.
+// Each new block vector is H applied to the one p slots back, orthonormalised
+// at once (CGS2) against every earlier slot, so the basis stays well
+// conditioned.  The space ends early when a new vector lies in the span of the
+// earlier ones (norm after CGS2 below 1e-10 of its norm before).  Slots 0..k-1
+// hold the basis; the refined vectors land in slots 2k + j (see rayleigh_ritz).
+struct RefineResult {
+  RRResult rr;
+  int k;
+};
+
+RefineResult refine_block_krylov(TR &t, int src0, int p, int S) {
+  hipStream_t st = t.st;
+  const int64_t dim = t.dim;
+  int kmax = (S + 1) * p;
+  while (kmax > p && (2 * kmax + p > t.NOM + 1 || kmax > dim)) kmax -= p;
+  QS3X_REQUIRE(src0 >= p && 2 * kmax + p <= t.NOM + 1, "refine_block_krylov: basis too small");
+  for (int j = 0; j < p; ++j) t.copy_slot(j, src0 + j);
+  std::vector<const void *> hp(kmax);
+  std::vector<int> hs(kmax);
+  for (int j = 0; j < kmax; ++j) {
+    hp[j] = t.slot(j);
+    hs[j] = j;
+  }
+  DevBuf<const void *> P(kmax);
+  P.upload(hp.data(), kmax);
+  DevBuf<int> X(kmax);
+  X.upload(hs.data(), kmax);
+  const void *const *PV = (const void *const *)P.p;
+  int k = kmax;
+  const std::vector<double> one(1, 1.0);
+  for (int i = p; i < kmax; ++i) {
+    launch_hv(t.s, QS3X_FP64, t.slot(i), t.slot(i - p), t.sc.p + t.S() + i - p, nullptr, nullptr, st);
+    ++t.hv_count;
+    int nb = vec_dot(QS3X_FP64, t.slot(i), QS3X_FP64, t.slot(i), dim, t.part.p, st);
+    finalize(t.part.p, nb, t.sc.p, 3, t.TMP() + 1, -1, 0.0, st);
+    for (int pass = 0; pass < 2; ++pass) {
+      vec_mdot(QS3X_FP64, PV, t.sc.p + t.S(), X.p, i, t.slot(i), dim, t.part.p, t.h.p, st);
+      vec_maxpy(QS3X_FP64, PV, t.sc.p + t.S(), X.p, i, t.h.p, t.slot(i), dim, nullptr, st);
+    }
+    QS3X_HIP(hipMemcpyAsync(t.sc.p + t.S() + i, one.data(), sizeof(double), hipMemcpyHostToDevice, st));
+    nb = vec_dot(QS3X_FP64, t.slot(i), QS3X_FP64, t.slot(i), dim, t.part.p, st);
+    finalize(t.part.p, nb, t.sc.p, 2, t.TMP(), t.S() + i, 0.0, st);
+    double nrm[2];
+    QS3X_HIP(hipMemcpyAsync(nrm, t.sc.p + t.TMP(), 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    QS3X_HIP(hipStreamSynchronize(st));
+    if (!(nrm[0] > 1e-10 * nrm[1])) {
+      k = i - (i % p);  // keep whole blocks
+      break;
+    }
+  }
+  RefineResult out;
+  out.k = std::max(k, p);
+  out.rr = rayleigh_ritz(t, out.k, p);
+  return out;
+}
+
 }  // namespace
 
 extern "C" {
@@ -715,32 +773,56 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
       // Rayleigh-Ritz value) is within conv |theta|, or the FP32 phase met its
       // convergence test and the FP64 Rayleigh quotient agrees with the FP32
       // Ritz value to conv |theta| (FP32 rounding has not moved the answer).
+      auto bound_ok = [&](const RRResult &q, int j) {
+        double gap = 1e300;
+        for (int i = 0; i < (int)q.theta.size(); ++i)
+          if (i != j) gap = std::min(gap, std::fabs(q.theta[i] - q.theta[j]));
+        const double bound = q.resid[j] * q.resid[j] / std::max(gap, 1e-300);
+        return std::make_pair(bound <= conv * std::fabs(q.theta[j]), bound);
+      };
       bool ok = true;
       for (int j = 0; j < NOE; ++j) {
-        double gap = 1e300;
-        for (int i = 0; i < NOK; ++i)
-          if (i != j) gap = std::min(gap, std::fabs(rr.theta[i] - rr.theta[j]));
-        const double bound = rr.resid[j] * rr.resid[j] / std::max(gap, 1e-300);
-        const double lim = conv * std::fabs(rr.theta[j]);
-        const bool agree = r.converged && std::fabs(rr.theta[j] - theta[j]) <= lim;
+        const auto b = bound_ok(rr, j);
+        const bool agree = r.converged && std::fabs(rr.theta[j] - theta[j]) <= conv * std::fabs(rr.theta[j]);
         if (options().verbose)
           std::fprintf(stderr, "[qs3x] pair %d: fp32 %.15g fp64 RR %.15g residual %.3e bound %.3e\n", j, theta[j],
-                       rr.theta[j], rr.resid[j], bound);
-        if (!(bound <= lim || agree)) ok = false;
+                       rr.theta[j], rr.resid[j], b.second);
+        if (!(b.first || agree)) ok = false;
       }
       resid = rr.resid;
       theta = rr.theta;
+      rr_j0 = 2 * NOK;
+      const int S = options().fp64_refine;
+      if (S > 0) {
+        // final FP64 residual correction of the Ritz vectors: Rayleigh-Ritz
+        // on their block Krylov space.  Values not accepted above are
+        // accepted when the refined pairs pass the residual bound.
+        RefineResult rf = refine_block_krylov(t64, rr_j0, NOE, S);
+        bool ok2 = true;
+        for (int j = 0; j < NOE; ++j) {
+          theta[j] = rf.rr.theta[j];
+          resid[j] = rf.rr.resid[j];
+          const auto b = bound_ok(rf.rr, j);
+          if (!b.first) ok2 = false;
+          if (options().verbose)
+            std::fprintf(stderr, "[qs3x] refined pair %d: %.15g residual %.3e bound %.3e (block Krylov, %d vectors)\n",
+                         j, theta[j], resid[j], b.second, rf.k);
+        }
+        ok = ok || ok2;
+        rr_j0 = 2 * rf.k;
+        trace_point(t0, hv32 + t64.hv_count, theta[0], QS3X_FP64);
+      }
       if (ok) {
         refined = true;
-        rr_j0 = 2 * NOK;
       } else {
         // fall back to FP64 thick restart from the refined vectors
         DevBuf<double> acc((size_t)2 * dim);
         QS3X_HIP(hipMemsetAsync(acc.p, 0, 16 * dim, st));
         for (int j = 0; j < NOE; ++j)
-          vec_axpy_acc(QS3X_FP64, acc.p, t64.slot(2 * NOK + j), dim, 1.0, nullptr, -1, st);
+          vec_axpy_acc(QS3X_FP64, acc.p, t64.slot(rr_j0 + j), dim, 1.0, nullptr, -1, st);
         vec_copy_scale(QS3X_FP64, t64.slot(0), QS3X_FP64, acc.p, dim, nullptr, -1, 1.0, st);
         QS3X_HIP(hipStreamSynchronize(st));
+        rr_j0 = 0;  // the FP64 thick restart leaves its Ritz vectors in slots 0..NOK-1
       }
     } else {
       t64.init(s, QS3X_FP64, NOK, NOM);

@@ -62,6 +62,13 @@ int qs3x_set_device(int32_t device);
  *                   (with tr_reorth >= 1 its first cycle is the plain
  *                   three-term recurrence).  NOM is at most 1023 (and
  *                   NOM x NOK at most 8192 for the rotation's staging).
+ *   "fp64_refine"   FP32-mixed thick restart: after the FP64 Rayleigh-Ritz
+ *                   values are accepted, Rayleigh-Ritz on the block Krylov
+ *                   space {Y, HY, ..., H^s Y} of the NOE accepted vectors, in
+ *                   FP64 (s = value, default 4; 0 skips this refinement).
+ *                   It costs s NOE + (s + 1) NOE FP64 H.v and needs
+ *                   NOE (2 s + 3) <= max(min(NOM, 64), 2 NOK + NOE) + 1
+ *                   (s is lowered to fit).
  *   "rot_vec"       Ritz rotation with 16-byte basis loads where the slots are
  *                   16-byte aligned (1, default) or element loads (0).
  *   "hv_xcd"        L2 domains (XCDs) for the H.v tile schedule: each gets one
