@@ -233,6 +233,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--NOK", type=int, default=4)
     ap.add_argument("--NOM", type=int, default=480)
+    ap.add_argument("--NOM64", type=int, default=380,
+                    help="basis of the FP64 thick-restart leg (0 skips it); 381 complex-double vectors are 252 GB")
     ap.add_argument("--reorth", type=int, default=2,
                     help="thick-restart reorthogonalisation: 0 full, 1 locked+local, 2 locked at restart only")
     ap.add_argument("--cgs", type=int, default=1, help="Gram-Schmidt passes per thick-restart step")
@@ -344,30 +346,64 @@ def main():
         log(f"FP32-mixed thick restart: E0={E32:.15f} in {t32:.2f} s, stats {st}, rel err {abs(E32 / E0_EXACT - 1):.2e}")
         solve = {"time_to_E0_fp64_plain_s": t64, "E0_fp64_plain": E64, "iters_fp64_plain": it64,
                  "rel_err_fp64_plain": abs(E64 / E0_EXACT - 1),
-                 "time_to_E0_fp32_mixed_s": t32, "E0_fp32_mixed": E32,
+                 # solver time without the Krylov basis allocation (reported apart)
+                 "time_to_E0_fp32_mixed_s": st["seconds"], "wall_fp32_mixed_s": t32,
+                 "basis_alloc_fp32_mixed_s": st.get("seconds_alloc"), "E0_fp32_mixed": E32,
                  "rel_err_fp32_mixed": abs(E32 / E0_EXACT - 1), "fp32_mixed_stats": st,
                  "fp32_mixed_settings": {"NOK": args.NOK, "NOM": args.NOM, "tr_reorth": args.reorth,
                                          "cgs_passes": args.cgs, "fp32_floor": args.fp32_floor,
                                          "conv": args.conv_mixed, "check_every": args.check_every},
                  "fp64_plain_settings": {"conv": args.conv_plain, "itrint": 5, "minitr": 20},
-                 "speedup_fp32_mixed_vs_fp64_plain": t64 / t32 if t32 > 0 else None,
+                 "speedup_fp32_mixed_vs_fp64_plain": t64 / st["seconds"] if st["seconds"] > 0 else None,
                  # first Ritz value within 1e-9 of the Bethe-ansatz E0 (solver trace; seconds, H.v)
                  "time_to_1e-9_fp64_plain": hit64,
                  "plain_fp32_mixed": {"E0": E32p, "iters": it32p, "seconds": t32p,
                                       "rel_err": abs(E32p / E0_EXACT - 1), "time_to_1e-9": hit32p},
                  "time_to_1e-9_fp32_mixed_any": hit32_any,
                  "time_to_1e-9_fp32_mixed_fp64_value": hit32_ref}
-        # time to the first Ritz value within 1e-9 of E0: FP64 plain Lanczos over
-        # each FP32-mixed solver (same start vector, same recurrence for plain)
+        # the FP64 thick restart with the same cycle settings (its basis is
+        # limited by memory: NOM64 + 1 complex-double vectors)
+        tr64_hit, tr64_s, E64tr = None, None, None
+        ene64 = None
+        if args.NOM64 > 0:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            try:
+                ene64, _, st64 = q3.my_trlanczos_routines_complex(sec, NOE=1, NOK=args.NOK, NOM=args.NOM64,
+                                                                   maxitr=400, lnc_ene_conv=args.conv_mixed,
+                                                                   prec=q3.FP64, seed=1, want_vectors=False)
+            except q3.QS3XError as e:  # e.g. the basis does not fit next to other allocations
+                log(f"FP64 thick restart skipped: {e}")
+            torch.cuda.synchronize()
+            tr64_s = time.perf_counter() - t0
+        if ene64 is not None:
+            E64tr = float(ene64[0])
+            tr64_hit = first_within(q3.solver_trace(), E0_EXACT, 1e-9)
+            log(f"FP64 thick restart (NOM {args.NOM64}): E0={E64tr:.15f} in {tr64_s:.2f} s, stats {st64}, "
+                f"first within 1e-9 {tr64_hit}")
+            solve["fp64_thick_restart"] = {"E0": E64tr, "seconds": st64["seconds"], "wall_s": tr64_s,
+                                           "basis_alloc_s": st64.get("seconds_alloc"),
+                                           "rel_err": abs(E64tr / E0_EXACT - 1), "time_to_1e-9": tr64_hit,
+                                           "NOM": args.NOM64, "stats": st64}
+        # time to the first Ritz value within 1e-9 of E0: the faster FP64 path
+        # (plain Lanczos or thick restart) over each FP32-mixed solver
         ratio = {}
-        if hit64:
+        den = [h for h in (hit64, tr64_hit) if h]
+        if den:
+            d = min(h[0] for h in den)
+            solve["time_to_1e-9_fp64_best"] = d
             if hit32p:
-                ratio["fp32_plain"] = hit64[0] / hit32p[0]
+                ratio["fp32_plain"] = d / hit32p[0]
             if hit32_any:
-                ratio["fp32_thick_restart"] = hit64[0] / hit32_any[0]
+                ratio["fp32_thick_restart"] = d / hit32_any[0]
             if hit32_ref:
-                ratio["fp32_thick_restart_fp64_value"] = hit64[0] / hit32_ref[0]
-        solve["speedup_to_1e-9_vs_fp64_plain"] = ratio
+                ratio["fp32_thick_restart_fp64_value"] = d / hit32_ref[0]
+        solve["speedup_to_1e-9_vs_fp64_best"] = ratio
+        # the same times with the sector build included
+        solve["time_to_1e-9_including_build"] = {
+            k: (h[0] + t_build if h else None)
+            for k, h in (("fp64_plain", hit64), ("fp64_thick_restart", tr64_hit), ("fp32_plain", hit32p),
+                         ("fp32_thick_restart", hit32_any), ("fp32_thick_restart_fp64_value", hit32_ref))}
         log(f"first within 1e-9: plain FP64 {hit64}, mixed (any) {hit32_any}, mixed (FP64 value) {hit32_ref}")
 
     cpu = None

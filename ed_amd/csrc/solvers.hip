@@ -206,6 +206,7 @@ struct TR {
   int64_t hv_count = 0;
   int64_t hv_base = 0;   // H.v applications of earlier phases (for the trace)
   double t_origin = 0;   // start of the solve (for the trace)
+  double alloc_s = 0;    // seconds spent allocating the basis in init()
   int cgs_passes = 2;
   int reorth = 0;
 
@@ -228,6 +229,7 @@ struct TR {
     {
       const double ta = now_s();
       basis.alloc((size_t)(NOM + 1) * dim * elem_bytes(prec));
+      alloc_s = now_s() - ta;
       if (options().verbose)
         std::fprintf(stderr, "[qs3x] thick-restart basis: %d x %lld %s elements, %.1f GB, allocated in %.3f s\n",
                      NOM + 1, (long long)dim, prec == QS3X_FP32 ? "complex-float" : "complex-double",
@@ -728,7 +730,10 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
     QS3X_REQUIRE(!(dim < NOE || dim < NOK || dim < NOM), "qs3x_trlanczos: dim < NOM (stop 999)");
     QS3X_REQUIRE(NOM + 1 <= kMaxBasis, "qs3x_trlanczos: NOM too large");
     hipStream_t st = stream();
-    const double t0 = now_s();
+    // Solve time and the trace exclude the Krylov basis allocations (their
+    // cost depends on what the process freed before: 0 to about 4.5 s for
+    // 160-250 GB); they are reported separately in stats[7].
+    double t0 = now_s(), t_alloc = 0.0;
     std::vector<double> theta, resid;
     double t32 = 0.0;
     int64_t hv32 = 0, hv64 = 0;
@@ -741,6 +746,8 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
     if (prec == QS3X_FP32) {
       TR t32s;
       t32s.init(s, QS3X_FP32, NOK, NOM);
+      t0 += t32s.alloc_s;
+      t_alloc += t32s.alloc_s;
       t32s.t_origin = t0;
       load_start(QS3X_FP32, t32s.slot(0), dim, start, seed, st);
       // FP32 phase: until the NOE lowest Ritz values stop moving at the FP32
@@ -761,6 +768,8 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
       // thick restart of at most 64 vectors (the FP32 basis may be far longer)
       const int NOM64 = std::max(std::min(NOM, 64), 2 * NOK + NOE);
       t64.init(s, QS3X_FP64, NOK, NOM64, 0, 2);
+      t0 += t64.alloc_s;
+      t_alloc += t64.alloc_s;
       t64.t_origin = t0;
       t64.hv_base = hv32;
       for (int j = 0; j < NOK; ++j)
@@ -826,11 +835,19 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
       }
     } else {
       t64.init(s, QS3X_FP64, NOK, NOM);
+      t0 += t64.alloc_s;
+      t_alloc += t64.alloc_s;
       t64.t_origin = t0;
       load_start(QS3X_FP64, t64.slot(0), dim, start, seed, st);
     }
     if (!refined) {
-      TRResult r = tr_cycles(t64, maxitr, conv, 0.0, theta, resid);
+      // FP64 solve (or the fallback of the mixed one): with tr_check_every > 0
+      // the FP64 solve checks inside its cycles as the FP32 stage does; the
+      // fallback keeps the reference's restart-level test
+      const int every = options().tr_check_every;
+      TRResult r = (prec == QS3X_FP64 && every > 0)
+                       ? tr_cycles_checked(t64, maxitr, std::max(conv, 1e-30), 0.0, every, theta, resid, NOE)
+                       : tr_cycles(t64, maxitr, conv, 0.0, theta, resid);
       restarts += r.restarts;
     }
     hv64 = t64.hv_count;
@@ -856,7 +873,7 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
       stats[4] = t32;
       stats[5] = mr;
       stats[6] = refined ? 1.0 : 0.0;  // FP32 Ritz vectors accepted after the FP64 Rayleigh-Ritz step
-      stats[7] = 0.0;
+      stats[7] = t_alloc;  // seconds spent allocating Krylov bases (not in stats[3] or the trace)
     }
   });
 }

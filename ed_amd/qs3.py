@@ -255,8 +255,9 @@ def my_trlanczos_routines_complex(sector, NOE, NOK, NOM, maxitr=1000, lnc_ene_co
     stats = np.zeros(8)
     _check(_lib.qs3x_trlanczos(sector._h, int(prec), int(NOE), int(NOK), int(NOM), int(maxitr),
                                float(lnc_ene_conv), _p(st), int(seed), _p(ene), _p(psi), _p(stats)))
-    keys = ["restarts", "hv_fp32", "hv_fp64", "seconds", "seconds_fp32", "max_residual", "rr_accepted"]
-    return ene, (psi.T.copy() if psi is not None else None), dict(zip(keys, stats[:7].tolist()))
+    keys = ["restarts", "hv_fp32", "hv_fp64", "seconds", "seconds_fp32", "max_residual", "rr_accepted",
+            "seconds_alloc"]
+    return ene, (psi.T.copy() if psi is not None else None), dict(zip(keys, stats.tolist()))
 
 
 def solver_trace():

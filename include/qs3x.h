@@ -54,7 +54,8 @@ int qs3x_set_device(int32_t device);
  *                   neighbour chain, batched-load chain, per-hop chain, general
  *                   lattice, in that order); 1, 2, 3: start the list at the
  *                   batched-load chain, per-hop chain or general kernel.
- *   "tr_check_every" FP32 stage of the mixed thick restart: 0 (default) checks
+ *   "tr_check_every" thick restart (the FP64 solve and the FP32 stage of the
+ *                   mixed one; not its FP64 fallback): 0 (default) checks
  *                   convergence at restarts as the reference does; n > 0
  *                   checks the projection every n steps inside a cycle and
  *                   ends the cycle there, once the change is below conv and
@@ -152,12 +153,16 @@ int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, i
  * in FP64 until the FP64 criterion holds.  ene (length NOE) receives the
  * lowest Ritz values; psi (host, dim x NOE column-major, may be NULL) the
  * Ritz vectors; stats (may be NULL, 8 doubles): [restarts, hv_fp32, hv_fp64,
- * seconds_total, seconds_fp32, max_residual, rr_accepted, 0].  In FP32 mode the
- * FP32 Ritz vectors go through one FP64 Rayleigh-Ritz step (NOK FP64 H.v); the
- * result is accepted (rr_accepted = 1) when, for every wanted pair, r^2/gap <=
- * conv |theta|, or the FP32 restarts met the convergence test and the FP64
- * Rayleigh quotient equals the FP32 Ritz value to conv |theta|; otherwise an
- * FP64 thick restart (full reorthogonalisation) continues from them. */
+ * seconds_total, seconds_fp32, max_residual, rr_accepted, seconds_alloc];
+ * seconds_total, seconds_fp32 and the solver trace exclude seconds_alloc, the
+ * time spent allocating the Krylov bases.  In FP32 mode the FP32 Ritz vectors
+ * go through one FP64 Rayleigh-Ritz step (NOK FP64 H.v) and, with option
+ * fp64_refine > 0, the FP64 block-Krylov refinement; the result is accepted
+ * (rr_accepted = 1) when, for every wanted pair, r^2/gap <= conv |theta| before
+ * or after the refinement, or the FP32 restarts met the convergence test and
+ * the FP64 Rayleigh quotient equals the FP32 Ritz value to conv |theta|;
+ * otherwise an FP64 thick restart (full reorthogonalisation) continues from
+ * the refined vectors. */
 int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32_t NOM,
                    int64_t maxitr, double conv, const double *start, uint64_t seed,
                    double *ene, double *psi, double *stats);

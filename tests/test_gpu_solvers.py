@@ -90,21 +90,24 @@ def test_thick_restart(prec_name):
         assert r < 1e-5, r
 
 
-@pytest.mark.parametrize("nom", [40, 150])
-def test_thick_restart_checked_fp32(nom):
-    """FP32 stage with convergence checks inside the cycles (tr_check_every),
-    three-term first cycle (tr_reorth 1), long cycles (NOM 150 uses the
-    structured projected eigensolver), then the FP64 Rayleigh-Ritz step."""
-    print(f"[gpu] thick restart checked fp32 NOM={nom}", flush=True)
+@pytest.mark.parametrize("nom,reorth,prec_name", [(40, 1, "fp32"), (150, 1, "fp32"), (150, 2, "fp32"),
+                                                   (150, 2, "fp64")])
+def test_thick_restart_checked(nom, reorth, prec_name):
+    """Convergence checks inside the cycles (tr_check_every), three-term first
+    cycle (tr_reorth 1; 2: three-term steps in restart cycles too), long
+    cycles (NOM 150 uses the structured projected eigensolver); in FP32 then
+    the FP64 Rayleigh-Ritz step and refinement."""
+    print(f"[gpu] thick restart checked NOM={nom} reorth={reorth} {prec_name}", flush=True)
     q3 = _q3()
+    prec = q3.FP64 if prec_name == "fp64" else q3.FP32
     om, osec, gm, gsec, rk = _setup(20, 5, [(1, 1.0, 1.0)], K=0)
     w = np.linalg.eigvalsh(om.full_hamiltonian(osec))
     q3.set_option("tr_check_every", 5)
-    q3.set_option("tr_reorth", 1)
+    q3.set_option("tr_reorth", reorth)
     q3.set_option("tr_cgs_passes", 1)
     try:
         ene, psi, st = q3.my_trlanczos_routines_complex(gsec, NOE=1, NOK=4, NOM=nom, maxitr=200,
-                                                         lnc_ene_conv=1e-12, prec=q3.FP32, seed=3)
+                                                         lnc_ene_conv=1e-12, prec=prec, seed=3)
     finally:
         q3.set_option("tr_check_every", 0)
         q3.set_option("tr_reorth", 0)
