@@ -323,4 +323,99 @@ int qs3x_time_hv(qs3x_sector *s, int32_t prec, int32_t n_rep, double *ms) {
   });
 }
 
+int qs3x_hv_autotune(qs3x_sector *s, int32_t prec, int32_t n_rep, int32_t *best, double *best_ms,
+                     int32_t max_results, int32_t *n_results, double *results) {
+  return guarded([&] {
+    QS3X_REQUIRE(s && n_rep > 0, "qs3x_hv_autotune: bad argument");
+    QS3X_REQUIRE(prec == QS3X_FP64 || prec == QS3X_FP32, "qs3x_hv_autotune: bad precision");
+    qs3x_sector::HvTune &tn = s->tune[prec == QS3X_FP64 ? 0 : 1];
+    tn.set = false;
+    if (s->dim == 0) {
+      if (n_results) *n_results = 0;
+      return;
+    }
+    const size_t es = (prec == QS3X_FP64) ? 16 : 8;
+    DevBuf<unsigned char> a(es * s->dim), b(es * s->dim);
+    hipStream_t st = stream();
+    vec_random(prec, a.p, s->dim, 12345, st);
+    hipEvent_t e0, e1;
+    QS3X_HIP(hipEventCreate(&e0));
+    QS3X_HIP(hipEventCreate(&e1));
+    // Candidates: every H.v kernel that applies (the launcher falls through
+    // to the next one when a kernel does not apply, so duplicates are timed
+    // and dropped by the order of the list), two tile schedules (round robin,
+    // one contiguous range per XCD), three grid sizes (resident, 1/2, 1/4).
+    std::vector<qs3x_sector::HvTune> cand;
+    for (int k = 0; k <= 3; ++k)
+      for (int x : {1, 8})
+        for (int d : {1, 2, 4}) {
+          qs3x_sector::HvTune c;
+          c.set = true;
+          c.kernel = k;
+          c.xcd = x;
+          c.grid_div = d;
+          cand.push_back(c);
+        }
+    double best_t = 1e300;
+    qs3x_sector::HvTune best_c;
+    int nres = 0;
+    auto timed = [&](int reps) {
+      QS3X_HIP(hipEventRecord(e0, st));
+      for (int r = 0; r < reps; ++r) launch_hv(s, prec, b.p, a.p, nullptr, nullptr, nullptr, st);
+      QS3X_HIP(hipEventRecord(e1, st));
+      QS3X_HIP(hipEventSynchronize(e1));
+      float t = 0.f;
+      QS3X_HIP(hipEventElapsedTime(&t, e0, e1));
+      return (double)t / reps;
+    };
+    // A candidate whose first (timed) launch is over 3x the best so far is not
+    // repeated (its time is recorded negated); when that happens to a
+    // kernel's first configuration, its other configurations are not run
+    // (time 0).  Slow kernels (e.g. the general-lattice kernel on a long
+    // chain) then cost one launch each.
+    int skip_kernel = -1;
+    for (const auto &c : cand) {
+      double ms = 0.0;
+      if (c.kernel != skip_kernel) {
+        tn = c;
+        const double first = timed(1);
+        if (first > 3.0 * best_t) {
+          ms = -first;
+          if (c.xcd == 1 && c.grid_div == 1) skip_kernel = c.kernel;
+        } else {
+          ms = timed(n_rep);
+          if (ms < best_t) {
+            best_t = ms;
+            best_c = c;
+          }
+        }
+      }
+      if (results && nres < max_results) {
+        results[4 * nres] = c.kernel;
+        results[4 * nres + 1] = c.xcd;
+        results[4 * nres + 2] = c.grid_div;
+        results[4 * nres + 3] = ms;
+      }
+      ++nres;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    tn = best_c;
+    if (best) {
+      best[0] = best_c.kernel;
+      best[1] = best_c.xcd;
+      best[2] = best_c.grid_div;
+    }
+    if (best_ms) *best_ms = best_t;
+    if (n_results) *n_results = std::min(nres, max_results);
+  });
+}
+
+int qs3x_hv_tune_clear(qs3x_sector *s) {
+  return guarded([&] {
+    QS3X_REQUIRE(s, "qs3x_hv_tune_clear: NULL sector");
+    s->tune[0].set = s->tune[1].set = false;
+  });
+}
+
 }  // extern "C"

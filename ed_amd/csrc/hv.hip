@@ -696,16 +696,21 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
     return;
   }
   HvArgs a = make_args(s, v0, v1, scale_dev, dot_partials);
-  int grid = persistent_grid(a.ntiles);
+  const qs3x_sector::HvTune &tn = s->tune[prec == QS3X_FP64 ? 0 : 1];
+  if (tn.set) a.nxcd = tn.xcd;
+  // a tuned configuration divides the resident grid (never enlarges it, so the
+  // callers' partial-sum buffers keep their size)
+  auto tuned = [&](int g) { return tn.set ? std::max(1, g / tn.grid_div) : g; };
+  int grid = tuned(persistent_grid(a.ntiles));
   if (nblocks_out) *nblocks_out = grid;
   const bool normlist = s->d_rclass.p == nullptr;  // sectors from caller lists carry list_r only
-  const int pick = options().hv_kernel;
+  const int pick = tn.set ? tn.kernel : options().hv_kernel;
   if (pick <= 0 && use_chain(s, prec) && !normlist && s->model->nn_chain && s->d_bitmap.p) {
     const size_t lds = chain_nn_lds_bytes(s->nod, s->model->NOS, (int)s->model->bt_jxy2.size(), prec);
     const bool uniform = s->model->bt_jxy2.size() == 1;
     KFn f = (lds <= 64 * 1024) ? chain_nn_kernel(s->nod, prec, s->real_phase, uniform, key32(s)) : nullptr;
     if (f) {
-      grid = resident_grid(f, lds, a.ntiles);
+      grid = tuned(resident_grid(f, lds, a.ntiles));
       if (nblocks_out) *nblocks_out = grid;
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), lds, st, a);
       QS3X_HIP(hipGetLastError());
@@ -715,7 +720,7 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   if (pick <= 1 && use_chain(s, prec) && !normlist) {
     KFn f = chain_mlp_kernel(s->nod, prec, s->real_phase, s->model->max_deg, key32(s));
     if (f) {
-      grid = resident_grid(f, chain_lds_bytes(s, prec), a.ntiles);
+      grid = tuned(resident_grid(f, chain_lds_bytes(s, prec), a.ntiles));
       if (nblocks_out) *nblocks_out = grid;
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, prec), st, a);
       QS3X_HIP(hipGetLastError());
@@ -725,7 +730,7 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   if (pick <= 2 && use_chain(s, prec) && !(normlist && prec != QS3X_FP64)) {
     KFn f = chain_kernel(s->nod, prec, s->real_phase, normlist, false, key32(s));
     if (f) {
-      grid = resident_grid(f, chain_lds_bytes(s, prec), a.ntiles);
+      grid = tuned(resident_grid(f, chain_lds_bytes(s, prec), a.ntiles));
       if (nblocks_out) *nblocks_out = grid;
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, prec), st, a);
       QS3X_HIP(hipGetLastError());

@@ -129,6 +129,14 @@ struct qs3x_sector {
   int64_t nnz = -1;
   // scratch for reductions
   qs3x::DevBuf<double> d_partials;
+  // H.v launch configuration chosen by qs3x_hv_autotune, per precision
+  // (0: FP64, 1: FP32); unset entries fall back to the options.
+  struct HvTune {
+    bool set = false;
+    int kernel = 0;  // first kernel tried, as option hv_kernel
+    int xcd = 1;     // L2 domains of the tile schedule, as option hv_xcd
+    int grid_div = 1;  // the resident grid is divided by this
+  } tune[2];
 };
 
 namespace qs3x {

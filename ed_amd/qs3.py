@@ -55,6 +55,8 @@ def _load():
         "qs3x_ham_to_vec_wave_vector": [_P, _P, _P],
         "qs3x_hv_dev": [_P, _i32, _P, _P, _P],
         "qs3x_time_hv": [_P, _i32, _i32, _P],
+        "qs3x_hv_autotune": [_P, _i32, _i32, _P, _P, _i32, _P, _P],
+        "qs3x_hv_tune_clear": [_P],
         "qs3x_lanczos": [_P, _i32, _i64, _i64, _i64, _dbl, _P, _u64, _P, _P, _P],
         "qs3x_trlanczos": [_P, _i32, _i32, _i32, _i32, _i64, _dbl, _P, _u64, _P, _P, _P],
         "qs3x_solver_trace": [_P, _i64, _P],
@@ -212,6 +214,20 @@ class Sector:
         ms = np.zeros(1)
         _check(_lib.qs3x_time_hv(self._h, int(prec), int(n_rep), _p(ms)))
         return float(ms[0])
+
+    def hv_autotune(self, prec=FP64, n_rep=5):
+        """qs3x_hv_autotune: times every launch candidate and keeps the fastest
+        for this sector and precision.  Returns (best (kernel, xcd, grid
+        divisor), best ms, array of rows (kernel, xcd, divisor, ms))."""
+        best = np.zeros(3, dtype=np.int32)
+        ms = np.zeros(1)
+        res = np.zeros((64, 4))
+        n = np.zeros(1, dtype=np.int32)
+        _check(_lib.qs3x_hv_autotune(self._h, int(prec), int(n_rep), _p(best), _p(ms), 64, _p(n), _p(res)))
+        return tuple(int(x) for x in best), float(ms[0]), res[:int(n[0])].copy()
+
+    def hv_tune_clear(self):
+        _check(_lib.qs3x_hv_tune_clear(self._h))
 
 
 # --------------------------------------------------------------- solvers

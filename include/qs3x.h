@@ -259,6 +259,22 @@ int qs3x_continued_fraction(int32_t nlev, const double *alpha, const double *bet
 /* Times n_rep device H . v applications (events on the library stream) and
  * returns the mean milliseconds per application. */
 int qs3x_time_hv(qs3x_sector *s, int32_t prec, int32_t n_rep, double *ms);
+/* Launch autotuner for the sector's H . v (no counterpart in the reference):
+ * times n_rep applications of every candidate configuration -- first kernel
+ * tried (as option hv_kernel 0..3), tile schedule (hv_xcd 1 or 8), grid (the
+ * resident grid divided by 1, 2 or 4) -- on a random vector, and keeps the
+ * fastest for this sector and precision; every later H . v of the sector
+ * (solvers included) uses it.  best (3 ints, may be NULL): kernel, xcd,
+ * grid divisor; best_ms its time; results (4 doubles per candidate, up to
+ * max_results): kernel, xcd, divisor, ms, where ms < 0 is the single timed
+ * launch of a candidate over 3x slower than the best so far and ms = 0 a
+ * candidate not run (its kernel's first configuration was that slow).  All
+ * candidates compute the same H . v (each kernel is checked against the
+ * oracle by the test suite). */
+int qs3x_hv_autotune(qs3x_sector *s, int32_t prec, int32_t n_rep, int32_t *best, double *best_ms,
+                     int32_t max_results, int32_t *n_results, double *results);
+/* Forgets the autotuned configurations of the sector (both precisions). */
+int qs3x_hv_tune_clear(qs3x_sector *s);
 
 #ifdef __cplusplus
 }

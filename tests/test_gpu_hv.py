@@ -144,3 +144,33 @@ def test_start_vector_matches_host_formula():
     # exercised through lanczos with a tiny sector: covered in test_gpu_solvers
     v = q3.start_vector(1000, 5)
     assert np.all(np.abs(v.real) <= 1) and np.all(np.abs(v.imag) <= 1)
+
+
+@pytest.mark.parametrize("case", [c for c in CASES if c[0] in ("chain12_n4_k3_j1j2", "chain16_n8_k4", "chain10_n5_k5",
+                                                             "torus6x6_n3_k12")], ids=lambda c: c[0])
+def test_hv_autotuned_parity(case):
+    """qs3x_hv_autotune keeps the fastest launch configuration per sector and
+    precision; H.v through that configuration still equals the oracle's (FP64
+    1e-12, FP32 2e-6), and clearing the tuning restores the default."""
+    print(f"[gpu] hv autotuned {case[0]}", flush=True)
+    q3 = _q3()
+    om, osec, gm, gsec = _build(case)
+    rng = np.random.default_rng(13)
+    v = rng.uniform(-1, 1, osec.dim) + 1j * rng.uniform(-1, 1, osec.dim)
+    ref = om.hv(osec, v)
+    best64, ms64, res64 = gsec.hv_autotune(q3.FP64, n_rep=2)
+    assert len(res64) == 24 and ms64 > 0 and best64[1] in (1, 8) and best64[2] in (1, 2, 4)
+    assert ms64 == pytest.approx(min(r[3] for r in res64 if r[3] > 0))
+    got = gsec.ham_to_vec_wave_vector(v)
+    assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 1e-12
+    best32, ms32, _ = gsec.hv_autotune(q3.FP32, n_rep=2)
+    ref32 = ref - gm.calcu_FM_energy() * v
+    dv = torch.from_numpy(v.astype(np.complex64)).cuda()
+    out = torch.empty_like(dv)
+    gsec.hv_dev(out.data_ptr(), dv.data_ptr(), q3.FP32)
+    torch.cuda.synchronize()
+    got32 = out.cpu().numpy().astype(np.complex128)
+    assert np.max(np.abs(got32 - ref32)) / np.max(np.abs(ref32)) < 2e-6
+    gsec.hv_tune_clear()
+    got = gsec.ham_to_vec_wave_vector(v)
+    assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 1e-12
