@@ -231,7 +231,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-tune", action="store_true", help="skip the H.v launch autotuner")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the H.v launch autotuner first (the timed H.v and the solves then use its choice)")
     ap.add_argument("--NOK", type=int, default=4)
     ap.add_argument("--NOM", type=int, default=480)
     ap.add_argument("--NOM64", type=int, default=380,
@@ -282,13 +283,13 @@ def main():
     # launch autotuner: kernel, tile schedule and grid per precision (the
     # timed H.v and the solvers below use the chosen configuration)
     tune = {}
-    if not args.no_tune:
+    if args.tune:
         log("autotuning the H.v launch (kernel, tile schedule, grid) ...")
         for name, prec in (("fp32", q3.FP32), ("fp64", q3.FP64)):
             best, bms, table = sec.hv_autotune(prec, n_rep=5)
-            tune[name] = {"kernel": best[0], "xcd": best[1], "grid_div": best[2], "ms": bms,
+            tune[name] = {"kernel": best[0], "xcd": best[1], "grid_quarters": best[2], "ms": bms,
                           "candidates": [[int(r[0]), int(r[1]), int(r[2]), round(float(r[3]), 4)] for r in table]}
-            log(f"autotune {name}: kernel {best[0]} xcd {best[1]} grid/{best[2]}: {bms:.4f} ms "
+            log(f"autotune {name}: kernel {best[0]} xcd {best[1]} grid {best[2]}/4: {bms:.4f} ms "
                 f"(default config {float(table[0][3]):.4f} ms)")
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
@@ -427,18 +428,18 @@ def main():
             log(f"cpu baseline unavailable: {e}")
 
     f32 = res["fp32"]
-    # the committed counter profile covers the default launch configuration
-    # (nearest-neighbour kernel, round-robin tiles, resident grid)
-    t32 = tune.get("fp32", {"kernel": 0, "xcd": 1, "grid_div": 1})
-    default_cfg = (t32["kernel"], t32["xcd"], t32["grid_div"]) == (0, 1, 1)
-    traffic = load_traffic() if default_cfg else None
+    # the committed counter profile names the launch configuration it covers
+    t32 = tune.get("fp32", {"kernel": 0, "xcd": 8, "grid_quarters": 4})
+    traffic = load_traffic()
+    if traffic and traffic.get("launch_config") != {"xcd": t32["xcd"], "grid_quarters": t32["grid_quarters"]}:
+        traffic = None
     kname = ("k_hv_chain_nn<4, float, real-phase, uniform, uint32>" if t32["kernel"] == 0 else
              f"H.v kernel family {t32['kernel']} (option hv_kernel)")
     roof = {"bound": "hbm", "achieved": f32["alg_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": f32["alg_GBs"] / HBM_PEAK_GBS,
             "traffic": (traffic or {}).get("bytes_per_launch"),
             "alg_bytes_per_launch": f32["alg_bytes"], "kernel": kname,
-            "launch_config": {"xcd": t32["xcd"], "grid_div": t32["grid_div"]},
+            "launch_config": {"xcd": t32["xcd"], "grid_quarters": t32["grid_quarters"]},
             "avg_launch_ms": f32["ms"]}
     out = {
         "metric": METRIC,

@@ -344,16 +344,18 @@ int qs3x_hv_autotune(qs3x_sector *s, int32_t prec, int32_t n_rep, int32_t *best,
     // Candidates: every H.v kernel that applies (the launcher falls through
     // to the next one when a kernel does not apply, so duplicates are timed
     // and dropped by the order of the list), two tile schedules (round robin,
-    // one contiguous range per XCD), three grid sizes (resident, 1/2, 1/4).
+    // one contiguous range per XCD), four grid sizes (4/4, 3/4, 2/4, 1/4 of
+    // the resident grid: fewer resident blocks can keep more of a large
+    // vector's gathers in cache).
     std::vector<qs3x_sector::HvTune> cand;
     for (int k = 0; k <= 3; ++k)
-      for (int x : {1, 8})
-        for (int d : {1, 2, 4}) {
+      for (int x : {8, 1})
+        for (int q : {4, 3, 2, 1}) {
           qs3x_sector::HvTune c;
           c.set = true;
           c.kernel = k;
           c.xcd = x;
-          c.grid_div = d;
+          c.grid_q = q;
           cand.push_back(c);
         }
     double best_t = 1e300;
@@ -381,10 +383,12 @@ int qs3x_hv_autotune(qs3x_sector *s, int32_t prec, int32_t n_rep, int32_t *best,
         const double first = timed(1);
         if (first > 3.0 * best_t) {
           ms = -first;
-          if (c.xcd == 1 && c.grid_div == 1) skip_kernel = c.kernel;
+          if (c.xcd == 8 && c.grid_q == 4) skip_kernel = c.kernel;
         } else {
           ms = timed(n_rep);
-          if (ms < best_t) {
+          // a later candidate must win by 2 %, so that run-to-run noise does
+          // not move the choice away from the earlier (default) configurations
+          if (ms < 0.98 * best_t) {
             best_t = ms;
             best_c = c;
           }
@@ -393,7 +397,7 @@ int qs3x_hv_autotune(qs3x_sector *s, int32_t prec, int32_t n_rep, int32_t *best,
       if (results && nres < max_results) {
         results[4 * nres] = c.kernel;
         results[4 * nres + 1] = c.xcd;
-        results[4 * nres + 2] = c.grid_div;
+        results[4 * nres + 2] = c.grid_q;
         results[4 * nres + 3] = ms;
       }
       ++nres;
@@ -404,7 +408,7 @@ int qs3x_hv_autotune(qs3x_sector *s, int32_t prec, int32_t n_rep, int32_t *best,
     if (best) {
       best[0] = best_c.kernel;
       best[1] = best_c.xcd;
-      best[2] = best_c.grid_div;
+      best[2] = best_c.grid_q;
     }
     if (best_ms) *best_ms = best_t;
     if (n_results) *n_results = std::min(nres, max_results);

@@ -47,7 +47,7 @@ struct Options {
   // MI355X has 8 XCDs: 8 halves the measured HBM traffic of the chain H.v at
   // L = 1000 but leaves its time unchanged (the kernel is latency-bound), so
   // the default stays 1.
-  int hv_xcd = 1;
+  int hv_xcd = 8;
   int verbose = 0;            // 1: one stderr line per restart / per 100 Lanczos steps
 };
 Options &options();

@@ -700,7 +700,7 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   if (tn.set) a.nxcd = tn.xcd;
   // a tuned configuration divides the resident grid (never enlarges it, so the
   // callers' partial-sum buffers keep their size)
-  auto tuned = [&](int g) { return tn.set ? std::max(1, g / tn.grid_div) : g; };
+  auto tuned = [&](int g) { return tn.set ? std::max(1, g * tn.grid_q / 4) : g; };
   int grid = tuned(persistent_grid(a.ntiles));
   if (nblocks_out) *nblocks_out = grid;
   const bool normlist = s->d_rclass.p == nullptr;  // sectors from caller lists carry list_r only

@@ -109,8 +109,10 @@ struct NnCtx {
 // Index reads, gathers, accumulation, store and dot partial of RPT rows whose
 // H hop slots hold (anchored rank or -1, coefficient, phase index).  Rows with
 // store[k] false are skipped (their slots must be -1).  All index reads of the
-// RPT rows are issued before any gather.
-template <int N, class R, bool REALPH, int RPT>
+// RPT rows are issued before any gather.  DIRECT1 (clear rows): the hops of
+// spin 1 (slots 2, 3) land on anchored rank crow +- 1, which is itself a
+// representative, so their rows are row +- 1 and need no index read.
+template <int N, class R, bool REALPH, int RPT, bool DIRECT1 = false>
 __device__ __forceinline__ void finish_rows(const HvArgs &a, const int64_t *row, const typename CT<R>::T *xin,
                                             const bool *store, const int (*hc)[2 * N], const R (*hcf)[2 * N],
                                             const int (*hei)[2 * N], const R *anti, R scale, double &dre,
@@ -122,12 +124,17 @@ __device__ __forceinline__ void finish_rows(const HvArgs &a, const int64_t *row,
 #pragma unroll
   for (int k = 0; k < RPT; ++k)
 #pragma unroll
-    for (int h = 0; h < H; ++h) wd[k][h] = a.bitmap[(uint32_t)(hc[k][h] >= 0 ? hc[k][h] : 0) >> 5];
+    for (int h = 0; h < H; ++h)
+      if (!(DIRECT1 && (h == 2 || h == 3))) wd[k][h] = a.bitmap[(uint32_t)(hc[k][h] >= 0 ? hc[k][h] : 0) >> 5];
   int hid[RPT][H];
 #pragma unroll
   for (int k = 0; k < RPT; ++k)
 #pragma unroll
     for (int h = 0; h < H; ++h) {
+      if (DIRECT1 && (h == 2 || h == 3)) {
+        hid[k][h] = hc[k][h] >= 0 ? (int)row[k] + (h == 2 ? 1 : -1) : -1;
+        continue;
+      }
       const uint32_t b = (uint32_t)hc[k][h] & 31u;
       const bool in = hc[k][h] >= 0 && ((wd[k][h].x >> b) & 1u);
       hid[k][h] = in ? (int)wd[k][h].y + __popc(wd[k][h].x & ((1u << b) - 1u)) : -1;
@@ -354,9 +361,9 @@ __global__ __launch_bounds__(kThreads) void k_hv_chain_nn(HvArgs a) {
     }
     if constexpr (REALPH) {
       int hz[RPT][H];
-      finish_rows<N, R, REALPH, RPT>(a, row, xin, clear, hc, hcf, hz, anti, scale, dre, dim_);
+      finish_rows<N, R, REALPH, RPT, true>(a, row, xin, clear, hc, hcf, hz, anti, scale, dre, dim_);
     } else {
-      finish_rows<N, R, REALPH, RPT>(a, row, xin, clear, hc, hcf, hei, anti, scale, dre, dim_);
+      finish_rows<N, R, REALPH, RPT, true>(a, row, xin, clear, hc, hcf, hei, anti, scale, dre, dim_);
     }
     // queue the rows that need the key search (order: slice, then lane)
 #pragma unroll

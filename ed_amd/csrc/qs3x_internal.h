@@ -135,7 +135,7 @@ struct qs3x_sector {
     bool set = false;
     int kernel = 0;  // first kernel tried, as option hv_kernel
     int xcd = 1;     // L2 domains of the tile schedule, as option hv_xcd
-    int grid_div = 1;  // the resident grid is divided by this
+    int grid_q = 4;  // quarters of the resident grid launched (4: all of it)
   } tune[2];
 };
 

@@ -218,7 +218,7 @@ class Sector:
     def hv_autotune(self, prec=FP64, n_rep=5):
         """qs3x_hv_autotune: times every launch candidate and keeps the fastest
         for this sector and precision.  Returns (best (kernel, xcd, grid
-        divisor), best ms, array of rows (kernel, xcd, divisor, ms))."""
+        quarters), best ms, array of rows (kernel, xcd, quarters, ms))."""
         best = np.zeros(3, dtype=np.int32)
         ms = np.zeros(1)
         res = np.zeros((64, 4))

@@ -73,8 +73,8 @@ int qs3x_set_device(int32_t device);
  *   "rot_vec"       Ritz rotation with 16-byte basis loads where the slots are
  *                   16-byte aligned (1, default) or element loads (0).
  *   "hv_xcd"        L2 domains (XCDs) for the H.v tile schedule: each gets one
- *                   contiguous range of rows (1, the default: plain round-robin
- *                   over workgroups; 8 for MI355X's XCDs).
+ *                   contiguous range of rows (8, the default: MI355X's XCDs;
+ *                   1: plain round robin over workgroups).
  *   "verbose"       0 (default); 1: a stderr line per thick restart and per
  *                   100 Lanczos steps; 2: per convergence check. */
 int qs3x_set_option(const char *name, double value);
@@ -261,12 +261,13 @@ int qs3x_continued_fraction(int32_t nlev, const double *alpha, const double *bet
 int qs3x_time_hv(qs3x_sector *s, int32_t prec, int32_t n_rep, double *ms);
 /* Launch autotuner for the sector's H . v (no counterpart in the reference):
  * times n_rep applications of every candidate configuration -- first kernel
- * tried (as option hv_kernel 0..3), tile schedule (hv_xcd 1 or 8), grid (the
- * resident grid divided by 1, 2 or 4) -- on a random vector, and keeps the
- * fastest for this sector and precision; every later H . v of the sector
+ * tried (as option hv_kernel 0..3), tile schedule (hv_xcd 8 or 1), grid (4,
+ * 3, 2 or 1 quarters of the resident grid) -- on a random vector, and keeps the
+ * fastest (a later candidate must beat the best so far by 2 %) for this
+ * sector and precision; every later H . v of the sector
  * (solvers included) uses it.  best (3 ints, may be NULL): kernel, xcd,
- * grid divisor; best_ms its time; results (4 doubles per candidate, up to
- * max_results): kernel, xcd, divisor, ms, where ms < 0 is the single timed
+ * grid quarters; best_ms its time; results (4 doubles per candidate, up to
+ * max_results): kernel, xcd, quarters, ms, where ms < 0 is the single timed
  * launch of a candidate over 3x slower than the best so far and ms = 0 a
  * candidate not run (its kernel's first configuration was that slow).  All
  * candidates compute the same H . v (each kernel is checked against the

@@ -159,8 +159,8 @@ def test_hv_autotuned_parity(case):
     v = rng.uniform(-1, 1, osec.dim) + 1j * rng.uniform(-1, 1, osec.dim)
     ref = om.hv(osec, v)
     best64, ms64, res64 = gsec.hv_autotune(q3.FP64, n_rep=2)
-    assert len(res64) == 24 and ms64 > 0 and best64[1] in (1, 8) and best64[2] in (1, 2, 4)
-    assert ms64 == pytest.approx(min(r[3] for r in res64 if r[3] > 0))
+    assert len(res64) == 32 and ms64 > 0 and best64[1] in (1, 8) and best64[2] in (1, 2, 3, 4)
+    assert ms64 <= min(r[3] for r in res64 if r[3] > 0) / 0.98
     got = gsec.ham_to_vec_wave_vector(v)
     assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 1e-12
     best32, ms32, _ = gsec.hv_autotune(q3.FP32, n_rep=2)
