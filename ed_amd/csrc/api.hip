@@ -237,6 +237,23 @@ int qs3x_sector_build(qs3x_model *m, int32_t nod, double rkx, double rky, double
   });
 }
 
+int qs3x_sector_build_u1(qs3x_model *m, int32_t nod, qs3x_sector **out, int64_t *dim) {
+  return guarded([&] {
+    QS3X_REQUIRE(m && out, "qs3x_sector_build_u1: NULL argument");
+    auto *s = new qs3x_sector();
+    s->model = m;
+    s->nod = nod;
+    try {
+      build_sector_u1(s);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+    if (dim) *dim = s->dim;
+  });
+}
+
 int qs3x_sector_from_lists(qs3x_model *m, int32_t nod, int64_t dim, const int64_t *list_s,
                            const double *list_r, const double *explist, qs3x_sector **out) {
   return guarded([&] {
@@ -269,6 +286,12 @@ int qs3x_sector_dim(const qs3x_sector *s, int64_t *dim) {
 int qs3x_sector_lists(const qs3x_sector *s, int64_t *list_s, double *list_r) {
   return guarded([&] {
     QS3X_REQUIRE(s, "qs3x_sector_lists: NULL sector");
+    if (is_u1(s)) {
+      // the U(1)-only block: list_s(i) = i, norms 1 (rows are the ranks)
+      for (int64_t i = 0; list_s && i < s->dim; ++i) list_s[i] = i + 1;
+      for (int64_t i = 0; list_r && i < s->dim; ++i) list_r[i] = 1.0;
+      return;
+    }
     if (list_s && s->dim) s->d_list_s.download(list_s, s->dim);
     if (list_r && s->dim) s->d_list_r.download(list_r, s->dim);
   });

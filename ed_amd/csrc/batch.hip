@@ -648,6 +648,7 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
                           int32_t itr_dsf, double rfield, int32_t prec, double *alpha, double *beta2,
                           double *norm1, double *stats) {
   return guarded4([&] {
+    QS3X_REQUIRE(!is_u1(gs), "qs3x_dsf_sminus_batch: not available for U(1)-only sectors");
     QS3X_REQUIRE(gs && K && pos && psi && alpha && beta2 && norm1 && itr_dsf >= 1,
                  "qs3x_dsf_sminus_batch: bad argument");
     QS3X_REQUIRE(nq >= 1 && nq <= kQ, "qs3x_dsf_sminus_batch: 1 <= nq <= 64");

@@ -415,6 +415,88 @@ __device__ __forceinline__ int shift_site(int s, int i, int j, int k, int LX, in
   return 1 + x + LX * (y + LY * z);
 }
 
+// U(1)-only H.v (source_only_u1/ham2vec_v3.f90:35-65).  Rows are all
+// C(NOS, N) configurations in colex order: row r holds the configuration of
+// 0-based rank r = sum_q C(p_q, q) over its 0-based down sites p_1 < ... < p_N
+// (list_fly / inv_list of source_only_u1/state_lists.f90, shifted to 0-based
+// sites and ranks).  A hop's target row is the rank of the flipped
+// configuration: no representative, norm or phase.  The occupation is one
+// 64-bit word (NOS <= 64), so membership tests are bit tests and the target
+// rank is a sum over the set bits of the flipped word.
+template <class R, bool COUNT>
+__global__ __launch_bounds__(kThreads) void k_hv_u1(HvArgs a, int N) {
+  using C = typename CT<R>::T;
+  const C *__restrict__ v1 = (const C *)a.v1;
+  C *__restrict__ v0 = (C *)a.v0;
+  const R scale = a.scale ? (R)(*a.scale) : (R)1;
+  const int64_t *__restrict__ B = a.binom;  // B[n * kdim + k] = C(n, k)
+  const int kd = a.kdim;
+  double dre = 0.0, dim_ = 0.0;
+  unsigned long long nnz = 0;
+  for (int64_t tile = blockIdx.x; tile < a.ntiles; tile += gridDim.x) {
+    const int64_t row = tile * kThreads + threadIdx.x;
+    if (row >= a.dim) continue;
+    // unrank: for q = N..1 the largest p with C(p, q) <= r (C(p, q) rises with p)
+    uint64_t occ = 0;
+    int p[kMaxNod];
+    int64_t r = row;
+    int hi = a.NOS - 1;
+    for (int q = N; q >= 1; --q) {
+      int lo = q - 1, h = hi;
+      while (lo < h) {
+        const int mid = (lo + h + 1) >> 1;
+        if (B[(int64_t)mid * kd + q] <= r) lo = mid;
+        else h = mid - 1;
+      }
+      p[q - 1] = lo;
+      r -= B[(int64_t)lo * kd + q];
+      occ |= 1ull << lo;
+      hi = lo - 1;
+    }
+    C x = v1[row];
+    x.x *= scale;
+    x.y *= scale;
+    R anti = 0, accx = 0, accy = 0;
+    for (int m = 0; m < N; ++m) {
+      const int s1 = p[m] + 1;  // 1-based site
+      for (int e = a.adj_ptr[s1]; e < a.adj_ptr[s1 + 1]; ++e) {
+        const int ent = a.adj[e];
+        const int t = (ent & 0xffff) - 1, btp = ent >> 16;
+        if ((occ >> t) & 1ull) continue;  // parallel bond
+        anti += (R)a.bt[2 * btp + 1];
+        if constexpr (COUNT) {
+          ++nnz;
+          continue;
+        }
+        uint64_t w = occ ^ (1ull << p[m]) ^ (1ull << t);
+        int64_t id = 0;
+        for (int q = 1; w; ++q, w &= w - 1) id += B[(int64_t)__builtin_ctzll(w) * kd + q];
+        const R cf = (R)a.bt[2 * btp];
+        const C y = v1[id];
+        accx += cf * y.x;
+        accy += cf * y.y;
+      }
+    }
+    if constexpr (!COUNT) {
+      const R dg = (sizeof(R) == 8 ? (R)a.ezz : (R)0) - (R)2 * anti;
+      C out;
+      out.x = dg * x.x + accx * scale;
+      out.y = dg * x.y + accy * scale;
+      v0[row] = out;
+      if (a.dot_partials) {
+        dre += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
+        dim_ += (double)x.x * (double)out.y - (double)x.y * (double)out.x;
+      }
+    }
+  }
+  if constexpr (COUNT) {
+    for (int d = 32; d >= 1; d >>= 1) nnz += __shfl_down(nnz, d, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(a.nnz_out, nnz);
+  } else {
+    if (a.dot_partials) block_reduce_store(dre, dim_, a.dot_partials);
+  }
+}
+
 template <class R, bool REALPH, bool COUNT>
 __global__ __launch_bounds__(kThreads) void k_hv_general(HvArgs a, int N) {
   using C = typename CT<R>::T;
@@ -703,6 +785,14 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   auto tuned = [&](int g) { return tn.set ? std::max(1, g * tn.grid_q / 4) : g; };
   int grid = tuned(persistent_grid(a.ntiles));
   if (nblocks_out) *nblocks_out = grid;
+  if (s->mode == kIndexU1) {
+    if (prec == QS3X_FP64)
+      hipLaunchKernelGGL((k_hv_u1<double, false>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
+    else
+      hipLaunchKernelGGL((k_hv_u1<float, false>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
+    QS3X_HIP(hipGetLastError());
+    return;
+  }
   const bool normlist = s->d_rclass.p == nullptr;  // sectors from caller lists carry list_r only
   const int pick = tn.set ? tn.kernel : options().hv_kernel;
   if (pick <= 0 && use_chain(s, prec) && !normlist && s->model->nn_chain && s->d_bitmap.p) {
@@ -761,8 +851,11 @@ int64_t count_nnz(qs3x_sector *s) {
   const int grid = persistent_grid(a.ntiles);
   hipStream_t st = stream();
   const bool normlist = s->d_rclass.p == nullptr;
-  KFn f = use_chain(s, QS3X_FP64) ? chain_kernel(s->nod, QS3X_FP64, true, normlist, true, false) : nullptr;
-  if (f)
+  KFn f = (s->mode != kIndexU1 && use_chain(s, QS3X_FP64))
+              ? chain_kernel(s->nod, QS3X_FP64, true, normlist, true, false) : nullptr;
+  if (s->mode == kIndexU1)
+    hipLaunchKernelGGL((k_hv_u1<double, true>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
+  else if (f)
     hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, QS3X_FP64), st, a);
   else
     hipLaunchKernelGGL((k_hv_general<double, true, true>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);

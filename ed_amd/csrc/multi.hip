@@ -443,6 +443,7 @@ int qs3x_lanczos_group(qs3x_sector *const *secs, int32_t nsec, int32_t prec, int
       order[i] = i;
       qs3x_sector *s = secs[i];
       QS3X_REQUIRE(s && s->model == m, "qs3x_lanczos_group: sectors of different models");
+      QS3X_REQUIRE(!is_u1(s), "qs3x_lanczos_group: not available for U(1)-only sectors");
       QS3X_REQUIRE(s->chain && s->mode == kIndexAnchored && s->d_rclass.p && s->d_bitmap.p && !s->all_k,
                    "qs3x_lanczos_group: chain sectors built on the device only");
       QS3X_REQUIRE(s->nod >= 1 && s->nod <= 8, "qs3x_lanczos_group: N_down must be 1..8");

@@ -374,6 +374,7 @@ extern "C" {
 int qs3x_correlations(qs3x_sector *s, const double *psi, int64_t npairs, const int32_t *pairs, double *szsz,
                       double *spsm) {
   return guarded3([&] {
+    QS3X_REQUIRE(!is_u1(s), "qs3x_correlations: not available for U(1)-only sectors");
     QS3X_REQUIRE(s && psi && pairs && szsz && spsm, "qs3x_correlations: NULL argument");
     qs3x_model *m = s->model;
     const int NOS = m->NOS;
@@ -447,6 +448,7 @@ int qs3x_correlations(qs3x_sector *s, const double *psi, int64_t npairs, const i
 
 int qs3x_local_mag(qs3x_sector *s, const double *psi, double *expe_sz) {
   return guarded3([&] {
+    QS3X_REQUIRE(!is_u1(s), "qs3x_local_mag: not available for U(1)-only sectors");
     QS3X_REQUIRE(s && psi && expe_sz, "qs3x_local_mag: NULL argument");
     qs3x_model *m = s->model;
     const int NOS = m->NOS;
@@ -492,6 +494,7 @@ int qs3x_local_mag(qs3x_sector *s, const double *psi, double *expe_sz) {
 int qs3x_apply_sq(qs3x_sector *gs, qs3x_sector *tgt, int32_t spsmsz, const double *q, const double *pos,
                   const double *psi, double *f) {
   return guarded3([&] {
+    QS3X_REQUIRE(!is_u1(gs) && !is_u1(tgt), "qs3x_apply_sq: not available for U(1)-only sectors");
     QS3X_REQUIRE(gs && tgt && q && pos && psi && f, "qs3x_apply_sq: NULL argument");
     QS3X_REQUIRE(gs->model == tgt->model, "qs3x_apply_sq: sectors of different models");
     QS3X_REQUIRE(spsmsz >= 1 && spsmsz <= 3, "qs3x_apply_sq: spsmsz must be 1, 2 or 3");

@@ -99,7 +99,8 @@ struct qs3x_model {
 enum IndexMode : int32_t {
   kIndexAnchored = 0,  // chain: dense table over configurations containing site 1
   kIndexFullRank = 1,  // dense table over all C(NOS, N) ranks
-  kIndexSearch = 2     // binary search in list_s
+  kIndexSearch = 2,    // binary search in list_s
+  kIndexU1 = 3         // U(1)-only block: row = colex rank - 1, no lists (source_only_u1)
 };
 
 struct qs3x_sector {
@@ -144,6 +145,11 @@ namespace qs3x {
 // Host helpers (sector.hip)
 int64_t binom_host(int64_t n, int64_t k);  // exact C(n,k) with the reference's recurrence
 void build_sector_device(qs3x_sector *s);
+// U(1)-only block of all C(NOS, N) configurations (source_only_u1/main.f90:23-29)
+void build_sector_u1(qs3x_sector *s);
+// Entry points that need the translation-symmetric lists reject U(1)-only
+// sectors, which support H.v, the Lanczos solvers and the DSF recurrence.
+inline bool is_u1(const qs3x_sector *s) { return s && s->mode == kIndexU1; }
 void sector_from_lists(qs3x_sector *s, const int64_t *list_s, const double *list_r,
                        const double *explist);
 void compute_explist(const qs3x_model *m, const double rk[3], std::vector<double> &ex);

@@ -489,6 +489,22 @@ static bool chain_fast_ok(const qs3x_model *m, int nod) {
 
 static void build_bitmap(qs3x_sector *s);
 
+void build_sector_u1(qs3x_sector *s) {
+  qs3x_model *m = s->model;
+  const int N = s->nod;
+  QS3X_REQUIRE(N >= 1 && N <= kMaxNod, "U(1) sector: N_down must be in 1..16");
+  QS3X_REQUIRE(N <= m->NOS, "U(1) sector: N_down > NOS");
+  QS3X_REQUIRE(m->NOS <= 64, "U(1) sector: NOS must be at most 64 (one 64-bit occupation word per row)");
+  s->rk[0] = s->rk[1] = s->rk[2] = 0.0;
+  upload_common(s, N + 1);
+  s->mode = kIndexU1;
+  s->chain = false;
+  s->real_phase = true;
+  // THS = combination(NOS, NOD) (source_only_u1/main.f90:27)
+  s->dim = binom_host(m->NOS, N);
+  QS3X_REQUIRE(s->dim > 0, "U(1) sector: C(NOS, N_down) overflows int64");
+}
+
 void build_sector_device(qs3x_sector *s) {
   qs3x_model *m = s->model;
   const int N = s->nod;

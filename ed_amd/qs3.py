@@ -48,6 +48,7 @@ def _load():
         "qs3x_fm_energy": [_P, _P],
         "qs3x_sector_build": [_P, _i32, _dbl, _dbl, _dbl, _P, _P],
         "qs3x_sector_from_lists": [_P, _i32, _i64, _P, _P, _P, _P],
+        "qs3x_sector_build_u1": [_P, _i32, _P, _P],
         "qs3x_sector_destroy": [_P],
         "qs3x_sector_dim": [_P, _P],
         "qs3x_sector_lists": [_P, _P, _P],
@@ -160,6 +161,15 @@ class Sector:
         _check(_lib.qs3x_sector_build(model._h, int(nod), float(rk[0]), float(rk[1]), float(rk[2]),
                                       ctypes.byref(h), _p(d)))
         return cls(model, nod, h, rk)
+
+    @classmethod
+    def build_u1(cls, model, nod):
+        """qs3x_sector_build_u1: the U(1)-only block of every configuration
+        with nod down spins (source_only_u1), rows in colex rank order."""
+        h = ctypes.c_void_p()
+        d = np.zeros(1, dtype=np.int64)
+        _check(_lib.qs3x_sector_build_u1(model._h, int(nod), ctypes.byref(h), _p(d)))
+        return cls(model, nod, h)
 
     @classmethod
     def build_all_k(cls, model, nod):

@@ -105,6 +105,13 @@ double qs3x_wavevector(int32_t K, int32_t L);
  * THS (the sector dimension) is returned through dim. */
 int qs3x_sector_build(qs3x_model *m, int32_t nod, double rkx, double rky, double rkz,
                       qs3x_sector **out, int64_t *dim);
+/* U(1)-only block: every configuration with nod down spins, rows in colex
+ * rank order (row i = rank i + 1), no translation symmetry.  Replaces the
+ * list setup of the U(1)-only code (source_only_u1/main.f90:23-29, THS =
+ * combination(NOS, NOD)); H . v follows source_only_u1/ham2vec_v3.f90:35-65.
+ * Such a sector supports H . v, the Lanczos solvers and the DSF recurrence;
+ * qs3x_sector_lists returns list_s(i) = i and unit norms.  NOS <= 64. */
+int qs3x_sector_build_u1(qs3x_model *m, int32_t nod, qs3x_sector **out, int64_t *dim);
 /* Same sector from caller-supplied lists (e.g. the reference's own list_s /
  * list_r / explist), for drop-in use behind ham_to_vec_wave_vector: list_s
  * ascending 1-based ranks, list_r norms, explist (LX+1)(LY+1)(LZ+1) complex

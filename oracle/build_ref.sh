@@ -50,3 +50,20 @@ $FC -shared -fopenmp -o "$OUT/libqs3ref.so" \
   "$OUT/input_param.o" "$OUT/my_l1_blas.o" "$OUT/state_lists.o" "$OUT/ham2vec_v3.o" \
   "$OUT/get_expectation_values.o" "$OUT/ref_harness.o" -latomic
 echo "built $OUT/libqs3ref.so"
+
+# The U(1)-only modules (source_only_u1/) reuse the module names, so they get
+# their own module directory and library.
+SRC1="$REF/source_only_u1"
+OUT1="$OUT/u1"
+if [ -d "$SRC1" ]; then
+  mkdir -p "$OUT1"
+  FFLAGS1="-O2 -fopenmp -fPIC -fdefault-integer-8 -module-dir $OUT1 -I $OUT1"
+  for f in input_param my_l1_blas state_lists ham2vec_v3; do
+    $FC $FFLAGS1 -c "$SRC1/$f.f90" -o "$OUT1/$f.o"
+  done
+  $FC $FFLAGS1 -c "$HERE/ref_harness_u1.f90" -o "$OUT1/ref_harness_u1.o"
+  $FC -shared -fopenmp -o "$OUT/libqs3ref_u1.so" \
+    "$OUT1/input_param.o" "$OUT1/my_l1_blas.o" "$OUT1/state_lists.o" "$OUT1/ham2vec_v3.o" \
+    "$OUT1/ref_harness_u1.o"
+  echo "built $OUT/libqs3ref_u1.so"
+fi

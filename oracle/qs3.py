@@ -33,6 +33,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle_qs3.so")
 _REF_PATH = os.path.join(_HERE, "_ref", "libqs3ref.so")
+_REF_U1_PATH = os.path.join(_HERE, "_ref", "libqs3ref_u1.so")
 
 i64 = ctypes.c_int64
 dbl = ctypes.c_double
@@ -72,6 +73,7 @@ def lib():
         L.orc_build_sector_chain.restype = i64
         L.orc_build_sector_chain.argtypes = [P, i64, dbl, dbl, dbl, P, P]
         L.orc_ham_to_vec.argtypes = [P, i64, i64, P, P, P, P, P, i64, i64]
+        L.orc_u1_ham_to_vec.argtypes = [P, i64, P, P, i64, i64]
         L.orc_count_hops.restype = i64
         L.orc_count_hops.argtypes = [P, i64, i64, P, i64, i64]
         L.orc_mk_list_ij.argtypes = [P, P, P, P]
@@ -214,6 +216,21 @@ class Model:
         r0, r1 = (0, sec.dim) if rows is None else rows
         lib().orc_ham_to_vec(self._h, sec.nod, sec.dim, _ptr(sec.list_s), _ptr(sec.list_r),
                              _ptr(sec.explist), _ptr(v0), _ptr(v1), r0, r1)
+        return v0
+
+    def u1_dim(self, nod):
+        """THS of the U(1)-only code: combination(NOS, NOD) (source_only_u1/main.f90:27)."""
+        return int(lib().orc_f_combination(self.NOS, nod))
+
+    def hv_u1(self, nod, v, rows=None):
+        """U(1)-only ham_to_vec_wave_vector (source_only_u1/ham2vec_v3.f90:35-65):
+        rows are colex ranks 1..C(NOS, nod).  The model needs ncol >= nod."""
+        if self.ncol < nod:
+            raise ValueError("hv_u1 needs a model with ncol >= nod")
+        v1 = np.ascontiguousarray(v, dtype=np.complex128)
+        v0 = np.zeros_like(v1)
+        r0, r1 = (0, len(v1)) if rows is None else rows
+        lib().orc_u1_ham_to_vec(self._h, nod, _ptr(v0), _ptr(v1), r0, r1)
         return v0
 
     def count_hops(self, sec, rows=None):
@@ -549,6 +566,37 @@ def continued_fraction(alpha, beta2, norm1, E0, omegas, eta, reference_quirk=Tru
 # ---------------------------------------------------------------------------
 # The reference's own modules (oracle/_ref/libqs3ref.so)
 # ---------------------------------------------------------------------------
+class RefLibU1:
+    """ctypes wrapper around oracle/ref_harness_u1.f90: the reference's own
+    U(1)-only modules (source_only_u1/).  One model at a time."""
+
+    def __init__(self):
+        if not os.path.exists(_REF_U1_PATH):
+            raise FileNotFoundError(_REF_U1_PATH)
+        self.L = ctypes.CDLL(_REF_U1_PATH)
+        self.THS = 0
+
+    @staticmethod
+    def available():
+        return os.path.exists(_REF_U1_PATH)
+
+    def setup(self, NOS, nod, bonds):
+        b = np.asarray(bonds, dtype=np.float64).reshape(-1, 4)
+        self._keep = [np.ascontiguousarray(b[:, c].astype(t)) for c, t in
+                      ((0, np.int64), (1, np.int64), (2, np.float64), (3, np.float64))]
+        pi, pj, jxy, jz = self._keep
+        t = np.zeros(1, dtype=np.int64)
+        self.L.ref_u1_setup(i64(NOS), i64(nod), i64(len(b)), _ptr(pi), _ptr(pj), _ptr(jxy), _ptr(jz), _ptr(t))
+        self.THS = int(t[0])
+        return self.THS
+
+    def hv(self, v):
+        v1 = np.ascontiguousarray(v, dtype=np.complex128)
+        v0 = np.zeros_like(v1)
+        self.L.ref_u1_hv(_ptr(v0), _ptr(v1))
+        return v0
+
+
 class RefLib:
     """Thin ctypes wrapper around oracle/ref_harness.f90.  The Fortran modules
     keep global state: one model / sector at a time."""

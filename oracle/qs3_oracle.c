@@ -386,6 +386,34 @@ void orc_ham_to_vec(const orc_model *m, i64 nod, i64 lv, const i64 *list_s,
   }
 }
 
+/* U(1)-only H.v (source_only_u1/ham2vec_v3.f90:35-65): the rows are all
+ * C(NOS, nod) configurations, row i (1-based) being colex rank i
+ * (list_fly(i), source_only_u1/state_lists.f90); a hop's target row is
+ * inv_list of the flipped configuration (no representative, no norms, no
+ * phases).  Needs ncol >= nod.  Rows [r0, r1). */
+void orc_u1_ham_to_vec(const orc_model *m, i64 nod, double complex *v0, const double complex *v1, i64 r0,
+                       i64 r1) {
+#pragma omp parallel for schedule(dynamic, 256)
+  for (i64 i = r0; i < r1; ++i) {
+    i64 st[64], ni[64];
+    double complex acc = 0.0;
+    orc_list_fly(m, i + 1, nod, st);
+    for (i64 b = 0; b < m->NOxxz; ++b) {
+      int f1 = member(st, nod, m->p1[b]);
+      int f2 = member(st, nod, m->p2[b]);
+      if (f1 != f2) {
+        acc = acc - m->sJ3[b] * v1[i];
+        orc_j_flip(m->p1[b], m->p2[b], st, nod, ni);
+        i64 id = orc_inv_list(m, ni, nod);
+        acc = acc + m->sJ2[b] * v1[id - 1];
+      } else {
+        acc = acc + m->sJ3[b] * v1[i];
+      }
+    }
+    v0[i] = acc;
+  }
+}
+
 /* Number of hop terms with a target inside the sector (nnz of the off-
  * diagonal part), for the algorithmic byte count. Rows [r0, r1). */
 i64 orc_count_hops(const orc_model *m, i64 nod, i64 lv, const i64 *list_s, i64 r0, i64 r1) {

@@ -252,3 +252,22 @@ def test_sq_sum_rule():
         # <S-_q^dagger S-_q> = (1/L) sum_ij e^{iq(R_i - R_j)} <S+_i S-_j>
         val = np.sum(np.exp(1j * q[0] * (pos[:, 0][:, None] - pos[:, 0][None, :])) * C) / L
         assert np.vdot(f, f).real == pytest.approx(val.real, abs=1e-10)
+
+
+@pytest.mark.parametrize("L,N,fam", [(10, 4, [(1, 1.0, 1.0)]), (9, 3, [(1, 1.0, 0.6), (2, 0.4, -0.3)]),
+                                     (8, 1, [(1, 0.7, 1.3)])])
+def test_u1_hv_equals_dense_block(L, N, fam):
+    """The oracle's U(1)-only H.v (rows = colex ranks) is the bit-string N_down
+    block above, row for row once rank r is mapped to its configuration
+    unrank(r): every matrix element, not only the spectrum."""
+    bonds = orc.chain_bonds(L, fam)
+    m = orc.Model(L, L, 1, 1, bonds)
+    Hd = _dense_sz_block(L, N, bonds)
+    confs = list(itertools.combinations(range(1, L + 1), N))
+    idx = {c: i for i, c in enumerate(confs)}
+    D = m.u1_dim(N)
+    assert D == len(confs)
+    perm = np.array([idx[tuple(int(x) for x in m.unrank(r, N))] for r in range(1, D + 1)])
+    assert sorted(perm.tolist()) == list(range(D))  # the ranks enumerate every configuration once
+    Hu = np.stack([m.hv_u1(N, np.eye(D)[:, j].astype(complex)) for j in range(D)], axis=1)
+    assert np.allclose(Hu, Hd[np.ix_(perm, perm)], atol=1e-14)

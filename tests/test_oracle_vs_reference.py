@@ -155,3 +155,29 @@ def test_correlations_local_mag_sq(case):
         assert np.max(np.abs(fr - fo)) <= 1e-13 * max(1.0, np.max(np.abs(fr)))
         w = rng.uniform(-1, 1, Dn) + 1j * rng.uniform(-1, 1, Dn)
         assert np.max(np.abs(ref.hv_new(w) - m.hv(new, w))) <= 1e-13 * np.max(np.abs(ref.hv_new(w)))
+
+
+U1_CASES = [
+    (12, 4, [(1, 1.0, 1.0)]),
+    (12, 5, [(1, 1.0, 0.7), (2, 0.5, 0.5)]),
+    (14, 7, [(1, 1.0, 0.8)]),
+    (10, 1, [(1, 1.0, 1.0)]),
+]
+
+
+@pytest.mark.skipif(not orc.RefLibU1.available(), reason="oracle/_ref U(1)-only library not built")
+@pytest.mark.parametrize("L,N,fam", U1_CASES)
+def test_u1_hv_against_reference(L, N, fam):
+    """The oracle's U(1)-only H.v equals the reference's own
+    source_only_u1/ham2vec_v3.f90 on the full C(L, N) block, and the same
+    block on a 4x3 torus (bond order and the p1 < p2 swap included)."""
+    for NOS, bonds in ((L, orc.chain_bonds(L, fam)), (12, orc.square_bonds(4, 3, 1.0))):
+        ref = orc.RefLibU1()
+        ths = ref.setup(NOS, N, bonds)
+        om = orc.Model(NOS, NOS, 1, 1, bonds, ncol=N + 1)
+        assert ths == om.u1_dim(N)
+        rng = np.random.default_rng(3)
+        v = rng.uniform(-1, 1, ths) + 1j * rng.uniform(-1, 1, ths)
+        r = ref.hv(v)
+        o = om.hv_u1(N, v)
+        assert np.max(np.abs(o - r)) <= 1e-13 * max(1.0, np.max(np.abs(r)))
