@@ -119,7 +119,7 @@ def dsf_workload(args):
     a32, b32, n32, st32, t32 = out["fp32"]
     a64, b64, n64, st64, t64 = out["fp64"]
     om = np.linspace(0.0, 5.0, 2001)
-    errs = []
+    errs, errs_wide, coef = [], [], []
     for j in range(64):
         if n64[j] <= 0:
             continue
@@ -127,7 +127,16 @@ def dsf_workload(args):
         s32 = q3.dsf_analysis(a32[j], b32[j], n32[j], E0, om, 0.02)
         scale = max(np.max(np.abs(s64)), 1e-300)
         errs.append(np.max(np.abs(s32 - s64)) / scale)
-    errs = np.array(errs)
+        # the same spectra at eta = 0.1 (wider than the spacing of the
+        # 200 Lanczos poles, so pole-position shifts average out)
+        w64 = q3.dsf_analysis(a64[j], b64[j], n64[j], E0, om, 0.1)
+        w32 = q3.dsf_analysis(a32[j], b32[j], n32[j], E0, om, 0.1)
+        errs_wide.append(np.max(np.abs(w32 - w64)) / max(np.max(np.abs(w64)), 1e-300))
+        # the first 10 coefficients (they fix the first 20 spectral moments)
+        k = 10
+        coef.append(max(np.max(np.abs(a32[j, :k] - a64[j, :k])) / max(1.0, np.max(np.abs(a64[j, :k]))),
+                        np.max(np.abs(b32[j, :k] - b64[j, :k])) / max(1e-300, np.max(np.abs(b64[j, :k])))))
+    errs, errs_wide, coef = np.array(errs), np.array(errs_wide), np.array(coef)
     # per-sector path for a sample of q, FP32 (sector build + S^-_q + recurrence)
     sample = [1, 17, 40]
     tps = []
@@ -151,7 +160,12 @@ def dsf_workload(args):
            "ms_per_block_step_fp64": st64["ms_per_block_step"], "seconds_per_sector_fp32_extrapolated": per_sector,
            "per_sector_sample_seconds": tps, "speedup_batch_vs_per_sector": per_sector / t32,
            "fp32_vs_fp64_S_rel_err_median": float(np.median(errs)), "fp32_vs_fp64_S_rel_err_max": float(np.max(errs)),
-           "fp32_vs_fp64_S_rel_err_q_counted": int(len(errs)), "ground_state_seconds": t_gs, "E0": E0}
+           "fp32_vs_fp64_S_rel_err_q_counted": int(len(errs)),
+           "fp32_vs_fp64_S_eta0.1_rel_err_median": float(np.median(errs_wide)),
+           "fp32_vs_fp64_S_eta0.1_rel_err_max": float(np.max(errs_wide)),
+           "fp32_vs_fp64_first10_coef_rel_err_median": float(np.median(coef)),
+           "fp32_vs_fp64_first10_coef_rel_err_max": float(np.max(coef)),
+           "ground_state_seconds": t_gs, "E0": E0}
     print(json.dumps(res), flush=True)
 
 
