@@ -128,6 +128,57 @@ int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, i
     double E = 0.0, tmp = 0.0;
     int64_t ite = 0;
     bool done = false;
+    // The convergence test (lanczos.f90:90-110) runs one check interval
+    // behind the device: at a check point the alpha/beta so far are copied
+    // to pinned host memory and an event recorded, the next steps are
+    // launched, and the previous check is evaluated once its event has
+    // completed.  The device never waits for the host's tridiagonal solve;
+    // it runs at most itrint steps past the step that met the test, and the
+    // result (E, iteration count, eigenvector pass) is that of the step that
+    // met it, as in the reference.
+    struct Pending {
+      int64_t ite = 0;
+      double *host = nullptr;
+      hipEvent_t ev = nullptr;
+    } pend[2];
+    for (auto &p : pend) {
+      QS3X_HIP(hipHostMalloc((void **)&p.host, sizeof(double) * 2 * (size_t)mitr, hipHostMallocDefault));
+      QS3X_HIP(hipEventCreateWithFlags(&p.ev, hipEventDisableTiming));
+    }
+    struct PendingFree {
+      Pending *p;
+      ~PendingFree() {
+        for (int k = 0; k < 2; ++k) {
+          if (p[k].host) (void)hipHostFree(p[k].host);
+          if (p[k].ev) (void)hipEventDestroy(p[k].ev);
+        }
+      }
+    } pend_free{pend};
+    int npend = 0, slot = 0;
+    int64_t ite_done = 0;  // step count of the check that ended the solve
+    auto evaluate = [&](Pending &p) {
+      QS3X_HIP(hipEventSynchronize(p.ev));
+      const int64_t n = p.ite;
+      ha.assign(p.host, p.host + n);
+      hb.assign(p.host + mitr, p.host + mitr + n);
+      if (n == 1) {  // dim == 1 (lanczos.f90:95-101)
+        E = ha[0];
+        done = true;
+        ite_done = n;
+        return;
+      }
+      E = tridiag_lowest(ha.data(), hb.data(), (int)n);
+      const double cv = std::fabs(tmp / E - 1.0);
+      const bool breakdown = !(hb[n - 1] > 1e-13 * std::fabs(E) + 1e-300);
+      if (n == maxitr || n == mitr || cv < conv || breakdown) {
+        done = true;
+        ite_done = n;
+      }
+      trace_point(t_start, n, E, prec);
+      if (options().verbose && (done || options().verbose > 1 || n % 100 == 0))
+        std::fprintf(stderr, "[qs3x] lanczos step %lld E=%.15g change %.3e\n", (long long)n, E, cv);
+      tmp = E;
+    };
     for (int64_t i = 0; i < mitr && !done; ++i) {
       void *cur = U[i % 3].p, *prev = U[(i + 2) % 3].p, *w = U[(i + 1) % 3].p;
       launch_hv(s, prec, w, cur, sc.p + SCALE + i, part.p, &nb, st);
@@ -137,25 +188,27 @@ int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, i
       finalize(part.p, nb, sc.p, 2, (int)(BETA + i), (int)(SCALE + i + 1), 0.0, st);
       ite = i + 1;
       if (ite == mitr || (ite >= minitr && ite % itrint == 0)) {
-        std::vector<double> all((size_t)4 * mitr + 4);
-        sc.download(all.data(), all.size());
-        ha.assign(all.begin() + A_OUT, all.begin() + A_OUT + ite);
-        hb.assign(all.begin() + BETA, all.begin() + BETA + ite);
-        if (ite == 1) {  // dim == 1 (lanczos.f90:95-101)
-          E = ha[0];
-          done = true;
-          break;
+        Pending &p = pend[slot];
+        p.ite = ite;
+        QS3X_HIP(hipMemcpyAsync(p.host, sc.p + A_OUT, sizeof(double) * ite, hipMemcpyDeviceToHost, st));
+        QS3X_HIP(hipMemcpyAsync(p.host + mitr, sc.p + BETA, sizeof(double) * ite, hipMemcpyDeviceToHost, st));
+        QS3X_HIP(hipEventRecord(p.ev, st));
+        slot ^= 1;
+        if (++npend == 2) {  // evaluate the previous check while this one is in flight
+          evaluate(pend[slot]);
+          --npend;
         }
-        E = tridiag_lowest(ha.data(), hb.data(), (int)ite);
-        const double cv = std::fabs(tmp / E - 1.0);
-        const bool breakdown = !(hb[ite - 1] > 1e-13 * std::fabs(E) + 1e-300);
-        if (ite == maxitr || cv < conv || breakdown) done = true;
-        trace_point(t_start, ite, E, prec);
-        if (options().verbose && (done || options().verbose > 1 || ite % 100 == 0))
-          std::fprintf(stderr, "[qs3x] lanczos step %lld E=%.15g change %.3e\n", (long long)ite, E, cv);
-        tmp = E;
+        if (ite == mitr && !done) {  // last step: evaluate it now
+          evaluate(pend[slot ^ 1]);
+          npend = 0;
+        }
       }
     }
+    if (!done && npend == 1) evaluate(pend[slot ^ 1]);
+    if (ite_done > 0) ite = ite_done;
+    // coefficients of the accepted step count (the extra steps past it are dropped)
+    ha.resize(ite);
+    hb.resize(ite);
     ene[0] = E;
     ene[1] = 99.0;  // the reference's sentinel for ene(2) (lanczos.f90:23-24)
     if (iters) *iters = ite;
