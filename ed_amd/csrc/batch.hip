@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "blas.h"
+#include "nt.h"
 #include "hv_common.h"
 #include "qs3x_internal.h"
 #include "rep.h"
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(kBThreads) void k_hv_block(BlockHvArgs a) {
         C out;
         out.x = mk * (dg * x.x + accx * scale);
         out.y = mk * (dg * x.y + accy * scale);
-        ((C *)a.y)[(base + r) * kQ + lane] = out;
+        st_nt(&((C *)a.y)[(base + r) * kQ + lane], out);  // written once per step
         dre += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
         dim_ += (double)x.x * (double)out.y - (double)x.y * (double)out.x;
       }
@@ -422,9 +423,9 @@ __global__ __launch_bounds__(kBThreads) void k_update3_block(typename CT<R>::T *
     for (int k = 0; k < U; ++k) {
       const int64_t r = r0 + k * step;
       if (r < dim) {
-        x[k] = w[r * kQ + q];
-        a[k] = u[r * kQ + q];
-        if (HASB) b[k] = up[r * kQ + q];
+        x[k] = ld_nt(&w[r * kQ + q]);
+        a[k] = ld_nt(&u[r * kQ + q]);
+        if (HASB) b[k] = ld_nt(&up[r * kQ + q]);
       }
     }
 #pragma unroll
@@ -437,7 +438,7 @@ __global__ __launch_bounds__(kBThreads) void k_update3_block(typename CT<R>::T *
           x[k].x -= cb * b[k].x;
           x[k].y -= cb * b[k].y;
         }
-        w[r * kQ + q] = x[k];
+        st_nt(&w[r * kQ + q], x[k]);
         acc += (double)x[k].x * (double)x[k].x + (double)x[k].y * (double)x[k].y;
       }
     }

@@ -32,6 +32,7 @@
 
 #include "blas.h"
 #include "hv_common.h"
+#include "nt.h"
 #include "qs3x_internal.h"
 
 namespace qs3x {
@@ -167,7 +168,7 @@ __device__ __forceinline__ void finish_rows(const HvArgs &a, const int64_t *row,
     C out;
     out.x = dg * x.x + accx * scale;
     out.y = dg * x.y + accy * scale;
-    ((C *)a.v0)[row[k]] = out;
+    st_nt(&((C *)a.v0)[row[k]], out);  // written once, read by the next pass
     if (a.dot_partials) {
       dre += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
       dim_ += (double)x.x * (double)out.y - (double)x.y * (double)out.x;
