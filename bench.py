@@ -248,7 +248,7 @@ def main():
     ap.add_argument("--tune", action="store_true",
                     help="run the H.v launch autotuner first (the timed H.v and the solves then use its choice)")
     ap.add_argument("--NOK", type=int, default=4)
-    ap.add_argument("--NOM", type=int, default=480)
+    ap.add_argument("--NOM", type=int, default=760)
     ap.add_argument("--NOM64", type=int, default=380,
                     help="basis of the FP64 thick-restart leg (0 skips it); 381 complex-double vectors are 252 GB")
     ap.add_argument("--reorth", type=int, default=2,

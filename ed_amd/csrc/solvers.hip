@@ -817,10 +817,15 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
       // FP64 Rayleigh-Ritz on the NOK FP32 Ritz vectors.  The FP64 stage
       // always reorthogonalises fully (CGS2): started next to an eigenvector,
       // local orthogonalisation loses the projection property in one cycle.
-      // the FP64 stage holds the Rayleigh-Ritz block and, if needed, an FP64
-      // thick restart of at most 64 vectors (the FP32 basis may be far longer)
+      // The FP64 stage first holds only the Rayleigh-Ritz block and the
+      // block-Krylov refinement (2 NOK + NOE and (2 s + 3) NOE vectors), so
+      // that the FP32 basis can take nearly all of HBM; an FP64 thick restart
+      // of at most 64 vectors replaces it only if the values are not accepted,
+      // after the FP32 basis is freed.
+      const int S_ref = std::max(0, options().fp64_refine);
+      const int NOM64_rr = std::max(2 * NOK + NOE, (2 * S_ref + 3) * NOE) - 1;
       const int NOM64 = std::max(std::min(NOM, 64), 2 * NOK + NOE);
-      t64.init(s, QS3X_FP64, NOK, NOM64, 0, 2);
+      t64.init(s, QS3X_FP64, NOK, NOM64_rr, 0, 2);
       t0 += t64.alloc_s;
       t_alloc += t64.alloc_s;
       t64.t_origin = t0;
@@ -877,15 +882,39 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
       if (ok) {
         refined = true;
       } else {
-        // fall back to FP64 thick restart from the refined vectors
+        // fall back to FP64 thick restart from the refined vectors: keep their
+        // sum, free the FP32 basis, give the FP64 stage its full basis
         DevBuf<double> acc((size_t)2 * dim);
         QS3X_HIP(hipMemsetAsync(acc.p, 0, 16 * dim, st));
         for (int j = 0; j < NOE; ++j)
           vec_axpy_acc(QS3X_FP64, acc.p, t64.slot(rr_j0 + j), dim, 1.0, nullptr, -1, st);
+        QS3X_HIP(hipStreamSynchronize(st));
+        {
+          const double tf = now_s();
+          t32s.basis.release();
+          const double dt_free = now_s() - tf;
+          t0 += dt_free;
+          t_alloc += dt_free;
+        }
+        const int64_t hv_rr = t64.hv_count;
+        t64.init(s, QS3X_FP64, NOK, NOM64, 0, 2);
+        t0 += t64.alloc_s;
+        t_alloc += t64.alloc_s;
+        t64.t_origin = t0;
+        t64.hv_base = hv32;
+        t64.hv_count = hv_rr;
         vec_copy_scale(QS3X_FP64, t64.slot(0), QS3X_FP64, acc.p, dim, nullptr, -1, 1.0, st);
         QS3X_HIP(hipStreamSynchronize(st));
         rr_j0 = 0;  // the FP64 thick restart leaves its Ritz vectors in slots 0..NOK-1
       }
+      // freeing a basis of hundreds of GB takes up to a second; like the
+      // allocation it is kept out of the solve time (stats[7])
+      const double tf = now_s();
+      t32s.basis.release();
+      const double dt_free = now_s() - tf;
+      t0 += dt_free;
+      t_alloc += dt_free;
+      t64.t_origin = t0;
     } else {
       t64.init(s, QS3X_FP64, NOK, NOM);
       t0 += t64.alloc_s;
@@ -926,7 +955,7 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
       stats[4] = t32;
       stats[5] = mr;
       stats[6] = refined ? 1.0 : 0.0;  // FP32 Ritz vectors accepted after the FP64 Rayleigh-Ritz step
-      stats[7] = t_alloc;  // seconds spent allocating Krylov bases (not in stats[3] or the trace)
+      stats[7] = t_alloc;  // seconds allocating and freeing Krylov bases (not in stats[3] or the trace)
     }
   });
 }

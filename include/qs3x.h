@@ -162,7 +162,7 @@ int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, i
  * Ritz vectors; stats (may be NULL, 8 doubles): [restarts, hv_fp32, hv_fp64,
  * seconds_total, seconds_fp32, max_residual, rr_accepted, seconds_alloc];
  * seconds_total, seconds_fp32 and the solver trace exclude seconds_alloc, the
- * time spent allocating the Krylov bases.  In FP32 mode the FP32 Ritz vectors
+ * time spent allocating the Krylov bases and freeing the FP32 one.  In FP32 mode the FP32 Ritz vectors
  * go through one FP64 Rayleigh-Ritz step (NOK FP64 H.v) and, with option
  * fp64_refine > 0, the FP64 block-Krylov refinement; the result is accepted
  * (rr_accepted = 1) when, for every wanted pair, r^2/gap <= conv |theta| before
