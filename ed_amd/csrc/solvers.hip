@@ -501,20 +501,44 @@ TRResult tr_cycles_checked(TR &t, int64_t maxitr, double tol, double abs_floor, 
   t.set_norm_scale(0);
   std::vector<double> wl, cb, a, b, lam, Z, prev;
   double beta_last = 0.0, mc_prev = -1.0;
+  // The checks run one interval behind the device, as in qs3x_lanczos: at a
+  // check point the coefficients so far are copied to pinned host memory and
+  // an event recorded; the previous check is evaluated once its event has
+  // completed, while the next steps run.  A cycle that met the test at step
+  // m ends at m (m_end), although up to itrint more steps were launched: the
+  // restart only uses slots 0..m and the coefficients before m.
+  struct PendCheck {
+    int m = 0;
+    int64_t hv = 0;
+    double *host = nullptr;
+    hipEvent_t ev = nullptr;
+  } pc[2];
+  for (auto &p : pc) {
+    QS3X_HIP(hipHostMalloc((void **)&p.host, sizeof(double) * 2 * (size_t)NOM, hipHostMallocDefault));
+    QS3X_HIP(hipEventCreateWithFlags(&p.ev, hipEventDisableTiming));
+  }
+  struct PendFree {
+    PendCheck *p;
+    ~PendFree() {
+      for (int k = 0; k < 2; ++k) {
+        if (p[k].host) (void)hipHostFree(p[k].host);
+        if (p[k].ev) (void)hipEventDestroy(p[k].ev);
+      }
+    }
+  } pend_free{pc};
   int k0 = 0;
   int64_t cycle = 0;
   for (cycle = 1; cycle <= maxitr; ++cycle) {
     const int i0 = k0;
     int m_end = NOM;
     bool conv = false;
-    for (int i = i0; i < NOM; ++i) {
-      // tr_reorth 1: three-term first cycle; 2: also every restart-cycle step
-      // after the first (the only one coupled to the locked vectors in exact arithmetic)
-      if ((k0 == 0 && t.reorth >= 1) || (t.reorth == 2 && i > k0)) t.step3(i);
-      else t.step(i, k0 == 0 ? 0 : 1);
-      const int m = i + 1;
-      if ((m - i0) % itrint != 0 || m < NOK + 1) continue;
-      t.read_ab(i0, m, a, b);
+    int npend = 0, slot = 0;
+    // evaluates a recorded check; true when the cycle may end at its step
+    auto evaluate = [&](PendCheck &p) -> bool {
+      QS3X_HIP(hipEventSynchronize(p.ev));
+      const int m = p.m;
+      a.assign(p.host, p.host + (m - i0));
+      b.assign(p.host + NOM, p.host + NOM + (m - i0));
       Arrow A{k0, m, k0 ? wl.data() : nullptr, k0 ? cb.data() : nullptr, a.data(), b.data()};
       std::vector<double> ev(nw);
       for (int j = 0; j < nw; ++j) ev[j] = arrow_eigenvalue(A, j);
@@ -527,7 +551,7 @@ TRResult tr_cycles_checked(TR &t, int64_t maxitr, double tol, double abs_floor, 
         mc = std::max(mc, ch);
       }
       prev = ev;
-      trace_point(t.t_origin, t.hv_base + t.hv_count, ev[0] + t.shift, t.prec);
+      trace_point(t.t_origin, p.hv, ev[0] + t.shift, t.prec);
       if (options().verbose > 1)
         std::fprintf(stderr, "[qs3x] cycle %lld step %d: theta0=%.15g change %.3e\n", (long long)cycle, m,
                      ev[0] + t.shift, mc);
@@ -546,12 +570,38 @@ TRResult tr_cycles_checked(TR &t, int64_t maxitr, double tol, double abs_floor, 
         }
       }
       mc_prev = first_check ? -1.0 : mc;
-      if (stop) {
-        conv = true;
-        m_end = m;
-        break;
+      return stop;
+    };
+    for (int i = i0; i < NOM; ++i) {
+      // tr_reorth 1: three-term first cycle; 2: also every restart-cycle step
+      // after the first (the only one coupled to the locked vectors in exact arithmetic)
+      if ((k0 == 0 && t.reorth >= 1) || (t.reorth == 2 && i > k0)) t.step3(i);
+      else t.step(i, k0 == 0 ? 0 : 1);
+      const int m = i + 1;
+      if ((m - i0) % itrint != 0 || m < NOK + 1) continue;
+      PendCheck &p = pc[slot];
+      p.m = m;
+      p.hv = t.hv_base + t.hv_count;
+      QS3X_HIP(hipMemcpyAsync(p.host, t.sc.p + t.A() + i0, sizeof(double) * (m - i0), hipMemcpyDeviceToHost, t.st));
+      QS3X_HIP(hipMemcpyAsync(p.host + NOM, t.sc.p + t.Bt() + i0, sizeof(double) * (m - i0), hipMemcpyDeviceToHost,
+                              t.st));
+      QS3X_HIP(hipEventRecord(p.ev, t.st));
+      slot ^= 1;
+      if (++npend == 2) {  // the previous check, while this one is in flight
+        --npend;
+        if (evaluate(pc[slot])) {
+          conv = true;
+          m_end = pc[slot].m;
+          npend = 0;
+          break;
+        }
       }
     }
+    if (!conv && npend == 1 && evaluate(pc[slot ^ 1])) {
+      conv = true;
+      m_end = pc[slot ^ 1].m;
+    }
+    QS3X_HIP(hipStreamSynchronize(t.st));
     t.read_ab(i0, m_end, a, b);
     beta_last = b[m_end - 1 - i0];
     projected_pairs(k0, m_end, wl, cb, a, b, NOK, lam, Z);
