@@ -361,9 +361,17 @@ def main():
         q3.set_option("tr_check_every", args.check_every)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        ene, _, st = q3.my_trlanczos_routines_complex(sec, NOE=1, NOK=args.NOK, NOM=args.NOM, maxitr=400,
-                                                       lnc_ene_conv=args.conv_mixed, prec=q3.FP32, seed=1,
-                                                       want_vectors=False)
+        nom_used = args.NOM
+        try:
+            ene, _, st = q3.my_trlanczos_routines_complex(sec, NOE=1, NOK=args.NOK, NOM=nom_used, maxitr=400,
+                                                           lnc_ene_conv=args.conv_mixed, prec=q3.FP32, seed=1,
+                                                           want_vectors=False)
+        except q3.QS3XError as e:  # the FP32 basis did not fit: the shorter basis that always does
+            log(f"FP32-mixed thick restart with NOM {nom_used} failed ({e}); retrying with NOM 480")
+            nom_used = 480
+            ene, _, st = q3.my_trlanczos_routines_complex(sec, NOE=1, NOK=args.NOK, NOM=nom_used, maxitr=400,
+                                                           lnc_ene_conv=args.conv_mixed, prec=q3.FP32, seed=1,
+                                                           want_vectors=False)
         torch.cuda.synchronize()
         t32 = time.perf_counter() - t0
         tr32 = q3.solver_trace()
@@ -377,7 +385,7 @@ def main():
                  "time_to_E0_fp32_mixed_s": st["seconds"], "wall_fp32_mixed_s": t32,
                  "basis_alloc_fp32_mixed_s": st.get("seconds_alloc"), "E0_fp32_mixed": E32,
                  "rel_err_fp32_mixed": abs(E32 / E0_EXACT - 1), "fp32_mixed_stats": st,
-                 "fp32_mixed_settings": {"NOK": args.NOK, "NOM": args.NOM, "tr_reorth": args.reorth,
+                 "fp32_mixed_settings": {"NOK": args.NOK, "NOM": nom_used, "tr_reorth": args.reorth,
                                          "cgs_passes": args.cgs, "fp32_floor": args.fp32_floor,
                                          "conv": args.conv_mixed, "check_every": args.check_every},
                  "fp64_plain_settings": {"conv": args.conv_plain, "itrint": 5, "minitr": 20},
