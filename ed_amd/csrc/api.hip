@@ -1,6 +1,7 @@
 // api.hip -- C ABI entry points of libqs3x (model, sectors, H.v, timing).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <utility>
@@ -107,10 +108,44 @@ int qs3x_set_option(const char *name, double value) {
   });
 }
 
+}  // extern "C"
+
+// Options from the environment, applied once before the first model is
+// created: QS3X_OPTIONS="name=value,name=value" with the names of
+// qs3x_set_option (so that a driver that cannot call it, such as QS3's own
+// Fortran main, can still choose e.g. the FP32 thick-restart settings).
+static void apply_env_options() {
+  static bool done = false;
+  if (done) return;
+  done = true;
+  const char *env = std::getenv("QS3X_OPTIONS");
+  if (!env) return;
+  std::string all(env);
+  size_t pos = 0;
+  while (pos < all.size()) {
+    size_t end = all.find(',', pos);
+    if (end == std::string::npos) end = all.size();
+    const std::string item = all.substr(pos, end - pos);
+    pos = end + 1;
+    if (item.empty()) continue;
+    const size_t eq = item.find('=');
+    QS3X_REQUIRE(eq != std::string::npos, "QS3X_OPTIONS: expected name=value, got '" + item + "'");
+    char *stop = nullptr;
+    const std::string val = item.substr(eq + 1);
+    const double v = std::strtod(val.c_str(), &stop);
+    QS3X_REQUIRE(stop && *stop == '\0' && !val.empty(), "QS3X_OPTIONS: bad value in '" + item + "'");
+    if (qs3x_set_option(item.substr(0, eq).c_str(), v) != 0)
+      throw Error{std::string("QS3X_OPTIONS: ") + qs3x_last_error()};
+  }
+}
+
+extern "C" {
+
 int qs3x_model_create(int32_t NOS, int32_t LX, int32_t LY, int32_t LZ, int32_t NOxxz,
                       const int32_t *p_xxz, const double *jxy, const double *jz,
                       qs3x_model **out) {
   return guarded([&] {
+    apply_env_options();
     QS3X_REQUIRE(out, "qs3x_model_create: out is NULL");
     QS3X_REQUIRE(NOS > 0 && LX > 0 && LY > 0 && LZ > 0 && (int64_t)LX * LY * LZ == NOS,
                  "qs3x_model_create: NOS must equal LX*LY*LZ");

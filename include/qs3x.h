@@ -76,7 +76,10 @@ int qs3x_set_device(int32_t device);
  *                   contiguous range of rows (8, the default: MI355X's XCDs;
  *                   1: plain round robin over workgroups).
  *   "verbose"       0 (default); 1: a stderr line per thick restart and per
- *                   100 Lanczos steps; 2: per convergence check. */
+ *                   100 Lanczos steps; 2: per convergence check.
+ * The environment variable QS3X_OPTIONS="name=value,name=value" applies the
+ * same settings once, before the first model is created (for drivers that
+ * cannot call this function); a bad entry makes qs3x_model_create fail. */
 int qs3x_set_option(const char *name, double value);
 
 /* ----------------------------------------------------------------- model */

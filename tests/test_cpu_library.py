@@ -131,3 +131,32 @@ def test_arrowhead_lowest_pairs(lib):
         res = A @ Z - Z * lam
         assert np.max(np.abs(res)) < 1e-9, (k, m, np.max(np.abs(res)))
         assert np.allclose(Z.T @ Z, np.eye(nv), atol=1e-9)
+
+
+def test_env_options_are_validated():
+    """QS3X_OPTIONS is parsed before the first model is created (no GPU needed
+    for that step): an unknown name or a bad value is reported with the
+    variable's name; valid settings pass on to the device set-up."""
+    import subprocess
+    import sys
+    code = (
+        "import ctypes, numpy as np\n"
+        "L = ctypes.CDLL('ed_amd/libqs3x.so')\n"
+        "L.qs3x_last_error.restype = ctypes.c_char_p\n"
+        "p = np.array([1, 2, 2, 3, 3, 1], dtype=np.int32); j = np.ones(3)\n"
+        "h = ctypes.c_void_p()\n"
+        "rc = L.qs3x_model_create(3, 3, 1, 1, 3, p.ctypes.data_as(ctypes.c_void_p), "
+        "j.ctypes.data_as(ctypes.c_void_p), j.ctypes.data_as(ctypes.c_void_p), ctypes.byref(h))\n"
+        "print(rc, L.qs3x_last_error().decode() if rc else '')\n")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for env, expect in (("hv_kernel=7", "QS3X_OPTIONS: hv_kernel must be 0..3"),
+                        ("nonsense=1", "QS3X_OPTIONS: qs3x_set_option: unknown option nonsense"),
+                        ("tr_reorth", "QS3X_OPTIONS: expected name=value"),
+                        ("tr_reorth=x", "QS3X_OPTIONS: bad value")):
+        out = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=120,
+                             env=dict(os.environ, QS3X_OPTIONS=env, HIP_VISIBLE_DEVICES=""))
+        assert expect in out.stdout, (env, out.stdout, out.stderr)
+    out = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=120,
+                         env=dict(os.environ, QS3X_OPTIONS="tr_check_every=5,tr_reorth=2,tr_cgs_passes=1",
+                                  HIP_VISIBLE_DEVICES=""))
+    assert "QS3X_OPTIONS" not in out.stdout, out.stdout
