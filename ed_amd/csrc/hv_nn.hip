@@ -282,15 +282,15 @@ __device__ __forceinline__ bool clear_slots(const NnCtx<N, R> &k, int bits, uint
   return true;
 }
 
-// RPT rows per lane per iteration.  One: two rows per lane (more loads in
-// flight per wavefront) measured the same at L = 1000 once the grid matches
-// the kernel's occupancy, and costs registers.
+// RPT rows per lane per iteration: two in FP32 up to N_down = 5 (more
+// independent loads in flight per wavefront; 2.5 % faster at L = 1000, N = 4),
+// one otherwise (the second row's registers cost occupancy: FP64 2.4 % slower).
 template <int N, class R, bool REALPH, bool UNI, class KeyT>
 __global__ __launch_bounds__(kThreads) void k_hv_chain_nn(HvArgs a) {
   static_assert(N >= 2, "N_down = 1 uses the general chain kernel");
   using C = typename CT<R>::T;
   constexpr int H = 2 * N;  // hop slots: 2m = spin m to the right, 2m + 1 = to the left
-  constexpr int RPT = 1;
+  constexpr int RPT = (sizeof(R) == 4 && N <= 5) ? 2 : 1;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int32_t s_queue[kThreads / 64][kQueue];
   const int L = a.L;
