@@ -451,7 +451,7 @@ def main():
 
     f32 = res["fp32"]
     # the committed counter profile names the launch configuration it covers
-    t32 = tune.get("fp32", {"kernel": 0, "xcd": 8, "grid_quarters": 4})
+    t32 = tune.get("fp32", {"kernel": 0, "xcd": 1, "grid_quarters": 4})  # the library's FP32 default
     traffic = load_traffic()
     if traffic and traffic.get("launch_config") != {"xcd": t32["xcd"], "grid_quarters": t32["grid_quarters"]}:
         traffic = None

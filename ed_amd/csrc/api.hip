@@ -94,7 +94,7 @@ int qs3x_set_option(const char *name, double value) {
       QS3X_REQUIRE(value >= 0 && value <= 1000, "tr_check_every must be 0..1000");
       options().tr_check_every = (int)value;
     } else if (n == "hv_xcd") {
-      QS3X_REQUIRE(value >= 1 && value <= 64, "hv_xcd must be 1..64");
+      QS3X_REQUIRE(value >= 0 && value <= 64, "hv_xcd must be 0..64");
       options().hv_xcd = (int)value;
     } else if (n == "hv_kernel") {
       QS3X_REQUIRE(value >= 0 && value <= 3, "hv_kernel must be 0..3");
@@ -406,8 +406,9 @@ int qs3x_hv_autotune(qs3x_sector *s, int32_t prec, int32_t n_rep, int32_t *best,
     // the resident grid: fewer resident blocks can keep more of a large
     // vector's gathers in cache).
     std::vector<qs3x_sector::HvTune> cand;
+    const int xd = hv_xcd_for(prec);  // the default schedule first
     for (int k = 0; k <= 3; ++k)
-      for (int x : {8, 1})
+      for (int x : {xd, xd == 1 ? 8 : 1})
         for (int q : {4, 3, 2, 1}) {
           qs3x_sector::HvTune c;
           c.set = true;
@@ -441,7 +442,7 @@ int qs3x_hv_autotune(qs3x_sector *s, int32_t prec, int32_t n_rep, int32_t *best,
         const double first = timed(1);
         if (first > 3.0 * best_t) {
           ms = -first;
-          if (c.xcd == 8 && c.grid_q == 4) skip_kernel = c.kernel;
+          if (c.xcd == xd && c.grid_q == 4) skip_kernel = c.kernel;  // the kernel's first configuration
         } else {
           ms = timed(n_rep);
           // a later candidate must win by 2 %, so that run-to-run noise does

@@ -783,7 +783,7 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
     a.rowmask = d_mask.p;
     a.eph = d_ephR.p;
     a.dot_partials = part.p;
-    a.nxcd = options().hv_xcd;
+    a.nxcd = options().hv_xcd > 0 ? options().hv_xcd : 8;  // per-XCD ranges unless set
     const double sztot_new = (double)L / 2.0 - (double)NT;
     const double c = (prec == QS3X_FP32) ? m->ezz : 0.0;
     hipEvent_t e0, e1;

@@ -47,10 +47,15 @@ struct Options {
   // MI355X has 8 XCDs: 8 halves the measured HBM traffic of the chain H.v at
   // L = 1000 but leaves its time unchanged (the kernel is latency-bound), so
   // the default stays 1.
-  int hv_xcd = 8;
+  int hv_xcd = 0;  // 0: per precision (see hv_xcd_for)
   int verbose = 0;            // 1: one stderr line per restart / per 100 Lanczos steps
 };
 Options &options();
+// Tile schedule of the H.v kernels: option hv_xcd, or by default one
+// contiguous range per XCD (8) in FP64 and plain round robin (1) in FP32,
+// the faster of each at L = 1000, N_down = 4 (0.684 vs 0.728 ms FP64;
+// 0.488 vs 0.514 ms FP32).
+inline int hv_xcd_for(int prec) { return options().hv_xcd > 0 ? options().hv_xcd : (prec == 1 /* QS3X_FP32 */ ? 1 : 8); }
 
 // host eigensolvers (eig.cpp)
 double tridiag_eigenvalue(const double *d, const double *e, int n, int k);

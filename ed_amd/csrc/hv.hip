@@ -716,7 +716,7 @@ int resident_grid(KFn f, size_t lds, int64_t ntiles) {
 }  // namespace
 
 static HvArgs make_args(qs3x_sector *s, void *v0, const void *v1, const double *scale,
-                        double *partials) {
+                        double *partials, int prec_of_args) {
   qs3x_model *m = s->model;
   HvArgs a{};
   a.cfg = s->d_cfg.p;
@@ -748,7 +748,7 @@ static HvArgs make_args(qs3x_sector *s, void *v0, const void *v1, const double *
   a.ezz = m->ezz;
   a.scale = scale;
   a.dot_partials = partials;
-  a.nxcd = options().hv_xcd;
+  a.nxcd = hv_xcd_for(prec_of_args);
   return a;
 }
 
@@ -777,7 +777,7 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
     if (nblocks_out) *nblocks_out = 0;
     return;
   }
-  HvArgs a = make_args(s, v0, v1, scale_dev, dot_partials);
+  HvArgs a = make_args(s, v0, v1, scale_dev, dot_partials, prec);
   const qs3x_sector::HvTune &tn = s->tune[prec == QS3X_FP64 ? 0 : 1];
   if (tn.set) a.nxcd = tn.xcd;
   // a tuned configuration divides the resident grid (never enlarges it, so the
@@ -846,7 +846,7 @@ int64_t count_nnz(qs3x_sector *s) {
   DevBuf<unsigned long long> d_n(1);
   unsigned long long zero = 0, nz = 0;
   d_n.upload(&zero, 1);
-  HvArgs a = make_args(s, nullptr, nullptr, nullptr, nullptr);
+  HvArgs a = make_args(s, nullptr, nullptr, nullptr, nullptr, QS3X_FP64);
   a.nnz_out = d_n.p;
   const int grid = persistent_grid(a.ntiles);
   hipStream_t st = stream();

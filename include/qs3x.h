@@ -73,8 +73,10 @@ int qs3x_set_device(int32_t device);
  *   "rot_vec"       Ritz rotation with 16-byte basis loads where the slots are
  *                   16-byte aligned (1, default) or element loads (0).
  *   "hv_xcd"        L2 domains (XCDs) for the H.v tile schedule: each gets one
- *                   contiguous range of rows (8, the default: MI355X's XCDs;
- *                   1: plain round robin over workgroups).
+ *                   contiguous range of rows (8: MI355X's XCDs; 1: plain
+ *                   round robin over workgroups).  0, the default: 1 for
+ *                   FP32 and 8 for FP64 single-vector H.v, 8 for the batched
+ *                   block H.v (the faster of each at L = 1000, N_down = 4).
  *   "verbose"       0 (default); 1: a stderr line per thick restart and per
  *                   100 Lanczos steps; 2: per convergence check.
  * The environment variable QS3X_OPTIONS="name=value,name=value" applies the
