@@ -112,7 +112,8 @@ struct NnCtx {
 // store[k] false are skipped (their slots must be -1).  All index reads of the
 // RPT rows are issued before any gather.  DIRECT1 (clear rows): the hops of
 // spin 1 (slots 2, 3) land on anchored rank crow +- 1, which is itself a
-// representative, so their rows are row +- 1 and need no index read.
+// representative, so their rows are row +- 1 and need no index read; the
+// other hops need no presence test.
 template <int N, class R, bool REALPH, int RPT, bool DIRECT1 = false>
 __device__ __forceinline__ void finish_rows(const HvArgs &a, const int64_t *row, const typename CT<R>::T *xin,
                                             const bool *store, const int (*hc)[2 * N], const R (*hcf)[2 * N],
@@ -127,31 +128,45 @@ __device__ __forceinline__ void finish_rows(const HvArgs &a, const int64_t *row,
 #pragma unroll
     for (int h = 0; h < H; ++h)
       if (!(DIRECT1 && (h == 2 || h == 3))) wd[k][h] = a.bitmap[(uint32_t)(hc[k][h] >= 0 ? hc[k][h] : 0) >> 5];
-  int hid[RPT][H];
+  // A hop that lands nowhere gathers the row's own element with coefficient 0.
+  // Rows fit in 32 bits (the index table is int32), so the gather index is
+  // zero-extended rather than selected in 64 bits.
+  uint32_t hid[RPT][H];
+  bool hin[RPT][H];
 #pragma unroll
-  for (int k = 0; k < RPT; ++k)
+  for (int k = 0; k < RPT; ++k) {
+    // rows that are not stored may lie past the end of the sector
+    const uint32_t self = store[k] ? (uint32_t)row[k] : 0u;
 #pragma unroll
     for (int h = 0; h < H; ++h) {
+      const bool ok = hc[k][h] >= 0;
+      uint32_t id;
+      bool in = ok;
       if (DIRECT1 && (h == 2 || h == 3)) {
-        hid[k][h] = hc[k][h] >= 0 ? (int)row[k] + (h == 2 ? 1 : -1) : -1;
-        continue;
+        id = (uint32_t)row[k] + (h == 2 ? 1u : ~0u);
+      } else {
+        const uint32_t b = (uint32_t)hc[k][h] & 31u;
+        id = wd[k][h].y + (uint32_t)__popc(wd[k][h].x & ((1u << b) - 1u));
+        // A clear row's images all have stabiliser order 1, so each one that
+        // is reached is a representative of the sector: no presence test.
+        if (!DIRECT1) in = ok & (((wd[k][h].x >> b) & 1u) != 0u);
       }
-      const uint32_t b = (uint32_t)hc[k][h] & 31u;
-      const bool in = hc[k][h] >= 0 && ((wd[k][h].x >> b) & 1u);
-      hid[k][h] = in ? (int)wd[k][h].y + __popc(wd[k][h].x & ((1u << b) - 1u)) : -1;
+      hin[k][h] = in;
+      hid[k][h] = in ? id : self;
     }
+  }
   C y[RPT][H];
 #pragma unroll
   for (int k = 0; k < RPT; ++k)
 #pragma unroll
-    for (int h = 0; h < H; ++h) y[k][h] = v1[hid[k][h] >= 0 ? (int64_t)hid[k][h] : row[k]];
+    for (int h = 0; h < H; ++h) y[k][h] = v1[(uint64_t)hid[k][h]];
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
     if (!store[k]) continue;
     R accx = 0, accy = 0;
 #pragma unroll
     for (int h = 0; h < H; ++h) {
-      const R cf = (hid[k][h] >= 0) ? hcf[k][h] : (R)0;
+      const R cf = hin[k][h] ? hcf[k][h] : (R)0;
       if constexpr (REALPH) {
         accx += cf * y[k][h].x;
         accy += cf * y[k][h].y;
