@@ -128,6 +128,28 @@ __device__ __forceinline__ void finish_rows(const HvArgs &a, const int64_t *row,
 #pragma unroll
     for (int h = 0; h < H; ++h)
       if (!(DIRECT1 && (h == 2 || h == 3))) wd[k][h] = a.bitmap[(uint32_t)(hc[k][h] >= 0 ? hc[k][h] : 0) >> 5];
+  // DIRECT1: a wavefront holds 64 consecutive rows, so rows row + 1 and row - 1
+  // are the neighbouring lanes' rows and their elements are those lanes'
+  // inputs.  Only lane 63 (row + 1) and lane 0 (row - 1) read memory, in one
+  // load.  The vector-memory data path is the kernel's bound, so two loads
+  // per row fewer is what counts.
+  C ynb[RPT][2];
+  if constexpr (DIRECT1) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+      C up, dn;
+      up.x = __shfl_down(xin[k].x, 1, 64);
+      up.y = __shfl_down(xin[k].y, 1, 64);
+      dn.x = __shfl_up(xin[k].x, 1, 64);
+      dn.y = __shfl_up(xin[k].y, 1, 64);
+      const bool e2 = lane == 63 && hc[k][2] >= 0, e3 = lane == 0 && hc[k][3] >= 0;
+      C ye = up;
+      if (e2 || e3) ye = v1[(uint64_t)(uint32_t)(row[k] + (e2 ? 1 : -1))];
+      ynb[k][0] = e2 ? ye : up;
+      ynb[k][1] = e3 ? ye : dn;
+    }
+  }
   // A hop that lands nowhere gathers the row's own element with coefficient 0.
   // Rows fit in 32 bits (the index table is int32), so the gather index is
   // zero-extended rather than selected in 64 bits.
@@ -159,7 +181,8 @@ __device__ __forceinline__ void finish_rows(const HvArgs &a, const int64_t *row,
 #pragma unroll
   for (int k = 0; k < RPT; ++k)
 #pragma unroll
-    for (int h = 0; h < H; ++h) y[k][h] = v1[(uint64_t)hid[k][h]];
+    for (int h = 0; h < H; ++h)
+      y[k][h] = (DIRECT1 && (h == 2 || h == 3)) ? ynb[k][h - 2] : v1[(uint64_t)hid[k][h]];
 #pragma unroll
   for (int k = 0; k < RPT; ++k) {
     if (!store[k]) continue;
