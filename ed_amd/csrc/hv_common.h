@@ -24,6 +24,7 @@ struct HvArgs {
   const double *list_r;
   const int32_t *index;
   const uint2 *bitmap;  // anchored index, 32 ranks per word: x = presence bits, y = row of the first
+  int64_t nbitmap;      // words in bitmap
   const int64_t *list_s;
   const void *v1;
   void *v0;

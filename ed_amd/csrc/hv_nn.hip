@@ -123,11 +123,65 @@ __device__ __forceinline__ void finish_rows(const HvArgs &a, const int64_t *row,
   constexpr int H = 2 * N;
   const C *__restrict__ v1 = (const C *)a.v1;
   uint2 wd[RPT][H];
+  // The cooperative read pays in FP32 with two rows per lane (-12 % at
+  // L = 1000, N = 4); in FP64 its cross-lane moves cost more than the loads
+  // it saves (+10 %).
+  constexpr bool COOP = DIRECT1 && sizeof(R) == 4 && N <= 5;
+  if constexpr (!COOP) {
 #pragma unroll
-  for (int k = 0; k < RPT; ++k)
+    for (int k = 0; k < RPT; ++k)
 #pragma unroll
-    for (int h = 0; h < H; ++h)
-      if (!(DIRECT1 && (h == 2 || h == 3))) wd[k][h] = a.bitmap[(uint32_t)(hc[k][h] >= 0 ? hc[k][h] : 0) >> 5];
+      for (int h = 0; h < H; ++h) wd[k][h] = a.bitmap[(uint32_t)(hc[k][h] >= 0 ? hc[k][h] : 0) >> 5];
+  } else {
+    // Cooperative index read.  The 64 rows of a wavefront are consecutive, so
+    // for each hop slot their anchored ranks are nearly consecutive and fall
+    // in a few words of the bitmap (2 or fewer for 90 % of the wavefronts at
+    // L = 1000, N = 4).  One load fetches kSpan words for every (row, slot)
+    // pair, starting at the word of the first lane that needs the slot; each
+    // lane takes its word from the lane that read it, and only lanes whose
+    // word lies outside that window read it themselves.
+    constexpr int NB = H - 2;  // slots that read the bitmap (all but 2 and 3)
+    constexpr int kSpan = 4;
+    static_assert(RPT * NB * kSpan <= 64, "one word per lane");
+    const int lane = threadIdx.x & 63;
+    uint32_t base[RPT][H];  // wavefront-uniform
+#pragma unroll
+    for (int k = 0; k < RPT; ++k)
+#pragma unroll
+      for (int h = 0; h < H; ++h) {
+        if (h == 2 || h == 3) continue;
+        const uint64_t m = __ballot(hc[k][h] >= 0);
+        const int first = m ? __builtin_ctzll(m) : 0;
+        base[k][h] = (uint32_t)__builtin_amdgcn_readlane((int)((uint32_t)max(hc[k][h], 0) >> 5), first);
+      }
+    uint32_t cb = 0;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k)
+#pragma unroll
+      for (int h = 0; h < H; ++h) {
+        if (h == 2 || h == 3) continue;
+        const int slot = k * NB + (h < 2 ? h : h - 2);
+        cb = ((lane >> 2) == slot) ? base[k][h] : cb;
+      }
+    const uint32_t cwi = min(cb + (uint32_t)(lane & (kSpan - 1)), (uint32_t)(a.nbitmap - 1));
+    const uint2 cw = a.bitmap[cwi];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k)
+#pragma unroll
+      for (int h = 0; h < H; ++h) {
+        if (h == 2 || h == 3) continue;
+        const int slot = k * NB + (h < 2 ? h : h - 2);
+        const uint32_t q = (uint32_t)max(hc[k][h], 0) >> 5;
+        const uint32_t j = q - base[k][h];
+        const bool inr = j < (uint32_t)kSpan;
+        const int src = slot * kSpan + (inr ? (int)j : 0);
+        uint2 w;
+        w.x = (uint32_t)__shfl((int)cw.x, src, 64);
+        w.y = (uint32_t)__shfl((int)cw.y, src, 64);
+        if (hc[k][h] >= 0 && !inr) w = a.bitmap[q];
+        wd[k][h] = w;
+      }
+  }
   // DIRECT1: a wavefront holds 64 consecutive rows, so rows row + 1 and row - 1
   // are the neighbouring lanes' rows and their elements are those lanes'
   // inputs.  Only lane 63 (row + 1) and lane 0 (row - 1) read memory, in one
@@ -324,7 +378,13 @@ __device__ __forceinline__ bool clear_slots(const NnCtx<N, R> &k, int bits, uint
 // independent loads in flight per wavefront; 2.5 % faster at L = 1000, N = 4),
 // one otherwise (the second row's registers cost occupancy: FP64 2.4 % slower).
 template <int N, class R, bool REALPH, bool UNI, class KeyT>
-__global__ __launch_bounds__(kThreads) void k_hv_chain_nn(HvArgs a) {
+// FP32 with two rows per lane is held to 5 wavefronts per SIMD (82 VGPRs
+// instead of 97, which would allow 4): the kernel is bound by its
+// vector-memory instruction stream, and the fifth wavefront is worth more
+// than the registers.
+__global__ __launch_bounds__(kThreads) void
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && N <= 5 ? 5 : 1)))
+k_hv_chain_nn(HvArgs a) {
   static_assert(N >= 2, "N_down = 1 uses the general chain kernel");
   using C = typename CT<R>::T;
   constexpr int H = 2 * N;  // hop slots: 2m = spin m to the right, 2m + 1 = to the left
