@@ -799,7 +799,8 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   if (pick <= 0 && use_chain(s, prec) && !normlist && s->model->nn_chain && s->d_bitmap.p) {
     const size_t lds = chain_nn_lds_bytes(s->nod, s->model->NOS, (int)s->model->bt_jxy2.size(), prec);
     const bool uniform = s->model->bt_jxy2.size() == 1;
-    KFn f = (lds <= 64 * 1024) ? chain_nn_kernel(s->nod, prec, s->real_phase, uniform, key32(s)) : nullptr;
+    const bool pairs = (((uintptr_t)a.v0 | (uintptr_t)a.v1) & 15u) == 0;
+    KFn f = (lds <= 64 * 1024) ? chain_nn_kernel(s->nod, prec, s->real_phase, uniform, key32(s), pairs) : nullptr;
     if (f) {
       grid = tuned(resident_grid(f, lds, a.ntiles));
       if (nblocks_out) *nblocks_out = grid;

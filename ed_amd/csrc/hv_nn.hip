@@ -374,23 +374,10 @@ __device__ __forceinline__ bool clear_slots(const NnCtx<N, R> &k, int bits, uint
   return true;
 }
 
-// RPT rows per lane per iteration: two in FP32 up to N_down = 5 (more
-// independent loads in flight per wavefront; 2.5 % faster at L = 1000, N = 4),
-// one otherwise (the second row's registers cost occupancy: FP64 2.4 % slower).
-template <int N, class R, bool REALPH, bool UNI, class KeyT>
-// FP32 with two rows per lane is held to 5 wavefronts per SIMD (82 VGPRs
-// instead of 97, which would allow 4): the kernel is bound by its
-// vector-memory instruction stream, and the fifth wavefront is worth more
-// than the registers.
-__global__ __launch_bounds__(kThreads) void
-__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && N <= 5 ? 5 : 1)))
-k_hv_chain_nn(HvArgs a) {
-  static_assert(N >= 2, "N_down = 1 uses the general chain kernel");
-  using C = typename CT<R>::T;
-  constexpr int H = 2 * N;  // hop slots: 2m = spin m to the right, 2m + 1 = to the left
-  constexpr int RPT = (sizeof(R) == 4 && N <= 5) ? 2 : 1;
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int32_t s_queue[kThreads / 64][kQueue];
+// LDS tables of the nearest-neighbour kernels (one block-wide pass, then a
+// barrier).
+template <int N, class R, bool UNI>
+__device__ __forceinline__ NnCtx<N, R> nn_setup(const HvArgs &a, unsigned char *smem) {
   const int L = a.L;
   // LDS: binom [N-1][L-1] int32 | rclass [N+1] R | r1/rclass [N+1] R | jxy2 [nbt] R | jz4 [nbt] R |
   //      bond type [L+1] u16
@@ -417,10 +404,31 @@ k_hv_chain_nn(HvArgs a) {
     for (int q = threadIdx.x; q <= L; q += kThreads) s_bt[q] = a.nn_bt[q];
   __syncthreads();
 
-  const NnCtx<N, R> k{L, a.kbits, s_binom, s_rcl, s_cfc, s_jxy2, s_jz4, s_bt};
+  return NnCtx<N, R>{L, a.kbits, s_binom, s_rcl, s_cfc, s_jxy2, s_jz4, s_bt};
+}
+
+// RPT rows per lane per iteration: two in FP32 up to N_down = 5 (more
+// independent loads in flight per wavefront; 2.5 % faster at L = 1000, N = 4),
+// one otherwise (the second row's registers cost occupancy: FP64 2.4 % slower).
+// FP32 with two rows per lane is held to 5 wavefronts per SIMD (82 VGPRs
+// instead of 97, which would allow 4): the kernel is bound by its
+// vector-memory instruction stream, and the fifth wavefront is worth more
+// than the registers.
+template <int N, class R, bool REALPH, bool UNI, class KeyT>
+__global__ __launch_bounds__(kThreads) void
+__attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 && N <= 5 ? 5 : 1)))
+k_hv_chain_nn(HvArgs a) {
+  static_assert(N >= 2, "N_down = 1 uses the general chain kernel");
+  using C = typename CT<R>::T;
+  constexpr int H = 2 * N;  // hop slots: 2m = spin m to the right, 2m + 1 = to the left
+  constexpr int RPT = (sizeof(R) == 4 && N <= 5) ? 2 : 1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int32_t s_queue[kThreads / 64][kQueue];
+  const int L = a.L;
+  const NnCtx<N, R> k = nn_setup<N, R, UNI>(a, smem);
   const C *__restrict__ v1 = (const C *)a.v1;
   const R scale = a.scale ? (R)(*a.scale) : (R)1;
-  const R jxy2u = s_jxy2[0], jz4u = s_jz4[0];
+  const R jxy2u = k.s_jxy2[0], jz4u = k.s_jz4[0];
   const int ei_ar = (L - 1) + 3 * (L + 1), ei_al = 1 + 3 * (L + 1);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int32_t *queue = s_queue[wid];
@@ -495,30 +503,254 @@ k_hv_chain_nn(HvArgs a) {
   if (a.dot_partials) block_reduce_store(dre, dim_, a.dot_partials);
 }
 
+typedef float nn_f4 __attribute__((ext_vector_type(4)));
+typedef float nn_f4a8 __attribute__((ext_vector_type(4))) __attribute__((aligned(8)));
+typedef unsigned long long nn_u64x2 __attribute__((ext_vector_type(2)));
+
+// FP32, N_down <= 5: each lane holds two adjacent rows, row0 = 2 * lane and
+// row0 + 1 of its wavefront's 128 consecutive rows.  A vector-memory
+// instruction costs the same whether a lane moves 8 or 16 bytes (measured on
+// L2-resident data: 16-19 cycles per wavefront load per CU for both), and
+// this kernel is bound by that instruction stream, so the rows' configuration
+// words, inputs and outputs move as 16-byte pairs, and a hop whose targets for
+// the two rows are adjacent rows (the common case: the targets of consecutive
+// rows are consecutive) gathers both with one 16-byte load.  The index read
+// is the cooperative one of finish_rows with one window of kSpan words per
+// hop slot covering both rows.  Requires 16-byte aligned vectors.
+template <int N, bool REALPH, bool UNI, class KeyT>
+__global__ __launch_bounds__(kThreads) void
+__attribute__((amdgpu_waves_per_eu(N <= 3 || (N == 4 && REALPH) ? 5 : 4)))  // 5 where that needs no scratch
+k_hv_chain_nn2(HvArgs a) {
+  static_assert(N >= 2 && N <= 5, "two-row kernel: 2 <= N_down <= 5");
+  using R = float;
+  using C = float2;
+  constexpr int H = 2 * N;
+  constexpr int NB = H - 2;  // hop slots that read the bitmap (all but 2 and 3)
+  constexpr int kSpan = 8;
+  static_assert(NB * kSpan <= 64, "one bitmap word per lane");
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int32_t s_queue[kThreads / 64][kQueue];
+  const int L = a.L;
+  const NnCtx<N, R> k = nn_setup<N, R, UNI>(a, smem);
+  const C *__restrict__ v1 = (const C *)a.v1;
+  C *__restrict__ v0 = (C *)a.v0;
+  const R scale = a.scale ? (R)(*a.scale) : (R)1;
+  const R jxy2u = k.s_jxy2[0], jz4u = k.s_jz4[0];
+  const int ei_ar = (L - 1) + 3 * (L + 1), ei_al = 1 + 3 * (L + 1);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int32_t *queue = s_queue[wid];
+  int qhead = 0, qcount = 0;  // wave-uniform
+  double dre = 0.0, dim_ = 0.0;
+  const int64_t dim = a.dim;
+  const int64_t ntile = (dim + 2 * kThreads - 1) / (2 * kThreads);
+
+  const TileRange tr = tile_range(ntile, a.nxcd);
+  for (int64_t t = tr.first; t < tr.end; t += tr.stride) {
+    const int64_t r0 = t * (2 * kThreads) + wid * 128 + 2 * lane;
+    const int64_t row[2] = {r0, r0 + 1};
+    const bool valid[2] = {r0 < dim, r0 + 1 < dim};
+    uint64_t w[2];
+    C xin[2];
+    if (valid[1]) {
+      const nn_u64x2 cw = *(const nn_u64x2 *)(a.cfg + r0);
+      const nn_f4 xv = *(const nn_f4 *)(v1 + r0);
+      w[0] = cw.x;
+      w[1] = cw.y;
+      xin[0] = make_float2(xv.x, xv.y);
+      xin[1] = make_float2(xv.z, xv.w);
+    } else {
+      const int64_t rr = min(r0, dim - 1);
+      w[0] = a.cfg[rr];
+      xin[0] = v1[rr];
+      w[1] = 0ull;
+      xin[1] = make_float2(0.f, 0.f);
+    }
+    int hc[2][H], hei[2][REALPH ? 1 : H];
+    R hcf[2][H], anti[2];
+    bool clear[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      clear[r] = valid[r] && clear_slots<N, R, REALPH, UNI>(k, a.bits, w[r], jxy2u, jz4u, ei_ar, ei_al, hc[r],
+                                                             hcf[r], REALPH ? nullptr : hei[r], anti[r]);
+      if (!clear[r]) {
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+          hc[r][h] = -1;
+          hcf[r][h] = (R)0;
+          if constexpr (!REALPH) hei[r][h] = 0;
+        }
+        anti[r] = (R)0;
+      }
+    }
+    // cooperative index read: window per hop slot starts at the word of the
+    // first lane whose row 0 (else row 1) has the hop
+    uint32_t base[H];
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      if (h == 2 || h == 3) continue;
+      const uint64_t m0 = __ballot(hc[0][h] >= 0), m1 = __ballot(hc[1][h] >= 0);
+      const uint64_t m = m0 ? m0 : m1;
+      const int first = m ? __builtin_ctzll(m) : 0;
+      const int qf = (int)((uint32_t)max(m0 ? hc[0][h] : hc[1][h], 0) >> 5);
+      base[h] = (uint32_t)__builtin_amdgcn_readlane(qf, first);
+    }
+    uint32_t cb = 0;
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      if (h == 2 || h == 3) continue;
+      cb = ((lane / kSpan) == (h < 2 ? h : h - 2)) ? base[h] : cb;
+    }
+    const uint2 cw = a.bitmap[min(cb + (uint32_t)(lane % kSpan), (uint32_t)(a.nbitmap - 1))];
+    uint32_t hid[2][H];
+    bool hin[2][H];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+      for (int h = 0; h < H; ++h) {
+        hin[r][h] = hc[r][h] >= 0;
+        hid[r][h] = 0u;
+        if (h == 2 || h == 3) continue;
+        const uint32_t q = (uint32_t)max(hc[r][h], 0) >> 5;
+        const uint32_t j = q - base[h];
+        const bool inr = j < (uint32_t)kSpan;
+        const int src = (h < 2 ? h : h - 2) * kSpan + (inr ? (int)j : 0);
+        uint2 wd;
+        wd.x = (uint32_t)__shfl((int)cw.x, src, 64);
+        wd.y = (uint32_t)__shfl((int)cw.y, src, 64);
+        if (hc[r][h] >= 0 && !inr) wd = a.bitmap[q];
+        // a clear row's reachable images are all representatives of the sector
+        const uint32_t b = (uint32_t)hc[r][h] & 31u;
+        hid[r][h] = wd.y + (uint32_t)__popc(wd.x & ((1u << b) - 1u));
+      }
+    // spin-1 hops (slots 2, 3) of a clear row land on rows row +- 1: the other
+    // row of the pair, or the neighbouring lane's; lanes 0 and 63 read the
+    // wavefront's edge rows in one load
+    C y[2][H];
+    {
+      C up0, dn1;
+      up0.x = __shfl_down(xin[0].x, 1, 64);
+      up0.y = __shfl_down(xin[0].y, 1, 64);
+      dn1.x = __shfl_up(xin[1].x, 1, 64);
+      dn1.y = __shfl_up(xin[1].y, 1, 64);
+      const bool e_hi = lane == 63 && hin[1][2], e_lo = lane == 0 && hin[0][3];
+      C ye = up0;
+      if (e_hi || e_lo) ye = v1[e_hi ? r0 + 2 : r0 - 1];
+      y[0][2] = xin[1];
+      y[0][3] = e_lo ? ye : dn1;
+      y[1][2] = e_hi ? ye : up0;
+      y[1][3] = xin[0];
+    }
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+      if (h == 2 || h == 3) continue;
+      const bool pair = hin[0][h] && hin[1][h] && hid[1][h] == hid[0][h] + 1u;
+      if (pair) {
+        const nn_f4a8 g = *(const nn_f4a8 *)(v1 + hid[0][h]);
+        y[0][h] = make_float2(g.x, g.y);
+        y[1][h] = make_float2(g.z, g.w);
+      } else {
+        y[0][h] = hin[0][h] ? v1[hid[0][h]] : make_float2(0.f, 0.f);
+        y[1][h] = hin[1][h] ? v1[hid[1][h]] : make_float2(0.f, 0.f);
+      }
+    }
+    C out[2];
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      R accx = 0, accy = 0;
+#pragma unroll
+      for (int h = 0; h < H; ++h) {
+        const R cf = hin[r][h] ? hcf[r][h] : (R)0;
+        if constexpr (REALPH) {
+          accx += cf * y[r][h].x;
+          accy += cf * y[r][h].y;
+        } else {
+          const R pr = (R)a.explist[2 * hei[r][h]] * cf, pi = (R)a.explist[2 * hei[r][h] + 1] * cf;
+          accx += pr * y[r][h].x - pi * y[r][h].y;
+          accy += pr * y[r][h].y + pi * y[r][h].x;
+        }
+      }
+      C x = xin[r];
+      x.x *= scale;
+      x.y *= scale;
+      const R dg = -(R)2 * anti[r];
+      out[r].x = dg * x.x + accx * scale;
+      out[r].y = dg * x.y + accy * scale;
+      if (a.dot_partials && clear[r]) {
+        dre += (double)x.x * (double)out[r].x + (double)x.y * (double)out[r].y;
+        dim_ += (double)x.x * (double)out[r].y - (double)x.y * (double)out[r].x;
+      }
+    }
+    if (clear[0] && clear[1]) {
+      nn_f4 o;
+      o.x = out[0].x;
+      o.y = out[0].y;
+      o.z = out[1].x;
+      o.w = out[1].y;
+      __builtin_nontemporal_store(o, (nn_f4 *)(v0 + r0));
+    } else {
+      if (clear[0]) st_nt(&v0[r0], out[0]);
+      if (clear[1]) st_nt(&v0[r0 + 1], out[1]);
+    }
+    // queue the rows that need the key search
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const bool needs = valid[r] && !clear[r];
+      const uint64_t mask = __ballot(needs);
+      if (mask != 0ull) {
+        const int before = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+        if (needs) queue[(qhead + qcount + before) & (kQueue - 1)] = (int32_t)row[r];
+        qcount += __popcll(mask);
+        __builtin_amdgcn_wave_barrier();
+        if (qcount >= 64) {
+          const int64_t rr = queue[(qhead + lane) & (kQueue - 1)];
+          __builtin_amdgcn_wave_barrier();
+          qhead = (qhead + 64) & (kQueue - 1);
+          qcount -= 64;
+          slow_row<N, R, REALPH, UNI, KeyT>(a, k, rr, scale, dre, dim_);
+        }
+      }
+    }
+  }
+  if (qcount > 0) {
+    __builtin_amdgcn_wave_barrier();
+    if (lane < qcount) {
+      const int64_t rr = queue[(qhead + lane) & (kQueue - 1)];
+      slow_row<N, R, REALPH, UNI, KeyT>(a, k, rr, scale, dre, dim_);
+    }
+  }
+  if (a.dot_partials) block_reduce_store(dre, dim_, a.dot_partials);
+}
+
 template <int N, bool UNI, class KeyT>
-KFn pick_nn3(int prec, bool realph) {
+KFn pick_nn3(int prec, bool realph, bool pairs) {
   if (prec == QS3X_FP64)
     return realph ? k_hv_chain_nn<N, double, true, UNI, KeyT> : k_hv_chain_nn<N, double, false, UNI, KeyT>;
+  if constexpr (N <= 5) {
+    if (pairs) return realph ? k_hv_chain_nn2<N, true, UNI, KeyT> : k_hv_chain_nn2<N, false, UNI, KeyT>;
+  }
   return realph ? k_hv_chain_nn<N, float, true, UNI, KeyT> : k_hv_chain_nn<N, float, false, UNI, KeyT>;
 }
 
 template <int N>
-KFn pick_nn(int prec, bool realph, bool uniform, bool key32) {
-  if (uniform) return key32 ? pick_nn3<N, true, uint32_t>(prec, realph) : pick_nn3<N, true, uint64_t>(prec, realph);
-  return key32 ? pick_nn3<N, false, uint32_t>(prec, realph) : pick_nn3<N, false, uint64_t>(prec, realph);
+KFn pick_nn(int prec, bool realph, bool uniform, bool key32, bool pairs) {
+  if (uniform)
+    return key32 ? pick_nn3<N, true, uint32_t>(prec, realph, pairs)
+                 : pick_nn3<N, true, uint64_t>(prec, realph, pairs);
+  return key32 ? pick_nn3<N, false, uint32_t>(prec, realph, pairs) : pick_nn3<N, false, uint64_t>(prec, realph, pairs);
 }
 
 }  // namespace
 
-KFn chain_nn_kernel(int N, int prec, bool realph, bool uniform, bool key32) {
+KFn chain_nn_kernel(int N, int prec, bool realph, bool uniform, bool key32, bool pairs) {
   switch (N) {
-    case 2: return pick_nn<2>(prec, realph, uniform, key32);
-    case 3: return pick_nn<3>(prec, realph, uniform, key32);
-    case 4: return pick_nn<4>(prec, realph, uniform, key32);
-    case 5: return pick_nn<5>(prec, realph, uniform, key32);
-    case 6: return pick_nn<6>(prec, realph, uniform, key32);
-    case 7: return pick_nn<7>(prec, realph, uniform, key32);
-    case 8: return pick_nn<8>(prec, realph, uniform, key32);
+    case 2: return pick_nn<2>(prec, realph, uniform, key32, pairs);
+    case 3: return pick_nn<3>(prec, realph, uniform, key32, pairs);
+    case 4: return pick_nn<4>(prec, realph, uniform, key32, pairs);
+    case 5: return pick_nn<5>(prec, realph, uniform, key32, pairs);
+    case 6: return pick_nn<6>(prec, realph, uniform, key32, pairs);
+    case 7: return pick_nn<7>(prec, realph, uniform, key32, pairs);
+    case 8: return pick_nn<8>(prec, realph, uniform, key32, pairs);
     default: return nullptr;
   }
 }

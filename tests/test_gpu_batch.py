@@ -144,11 +144,20 @@ def test_lanczos_group_matches_per_sector(prec_name, graph):
     E, its, st = q3.lanczos_group(secs, maxitr=500, minitr=10, itrint=5, lnc_ene_conv=conv, prec=prec, seed=5,
                                   use_graph=graph)
     assert st["groups"] == 6
-    tol = 1e-9 if prec == q3.FP64 else 2e-6
+    # FP32: the group kernel and the per-sector kernel round differently, and
+    # each FP32 recurrence ends within about 3e-6 of the FP64 value on these
+    # sectors (measured up to 2.8e-6), so both are held to 5e-6 of the FP64
+    # per-sector E0 and to each other.
+    tol = 1e-9 if prec == q3.FP64 else 5e-6
     for (N, K), s, e in zip(keys, secs, E):
         e1, _, _ = q3.lanczos_routines(s, maxitr=500, minitr=10, itrint=5, lnc_ene_conv=conv, prec=prec, seed=5,
                                        want_vector=False)
         assert abs(e - e1) <= tol * max(1.0, abs(e1)), (N, K, e, e1)
+        if prec == q3.FP32:
+            e64, _, _ = q3.lanczos_routines(s, maxitr=500, minitr=10, itrint=5, lnc_ene_conv=conv, prec=q3.FP64,
+                                            seed=5, want_vector=False)
+            assert abs(e - e64) <= tol * max(1.0, abs(e64)), (N, K, e, e64)
+            assert abs(e1 - e64) <= tol * max(1.0, abs(e64)), (N, K, e1, e64)
         if prec == q3.FP64:
             osec = om.build_sector(N, om.rk_from_K(K))
             H = om.full_hamiltonian(osec)
