@@ -462,18 +462,19 @@ def main():
 
     f32 = res["fp32"]
     # the committed counter profile names the launch configuration it covers
-    t32 = tune.get("fp32", {"kernel": 0, "xcd": 1, "grid_quarters": 4})  # the library's FP32 default
+    t32 = tune.get("fp32", {"kernel": 0, "xcd": 8, "grid_quarters": 4})  # the library's FP32 default
+    # interleaved chunks of 2^5 tiles per XCD (option hv_chunk's default) when xcd > 1
+    lc = {"xcd": t32["xcd"], "chunk": 5 if t32["xcd"] > 1 else 0, "grid_quarters": t32["grid_quarters"]}
     kname = ("k_hv_chain_nn2<4, real-phase, uniform, uint32> (FP32, two adjacent rows per lane)"
              if t32["kernel"] == 0 else f"H.v kernel family {t32['kernel']} (option hv_kernel)")
     traffic = load_traffic()
-    if traffic and (traffic.get("kernel") != kname or
-                    traffic.get("launch_config") != {"xcd": t32["xcd"], "grid_quarters": t32["grid_quarters"]}):
+    if traffic and (traffic.get("kernel") != kname or traffic.get("launch_config") != lc):
         traffic = None
     roof = {"bound": "hbm", "achieved": f32["alg_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": f32["alg_GBs"] / HBM_PEAK_GBS,
             "traffic": (traffic or {}).get("bytes_per_launch"),
             "alg_bytes_per_launch": f32["alg_bytes"], "kernel": kname,
-            "launch_config": {"xcd": t32["xcd"], "grid_quarters": t32["grid_quarters"]},
+            "launch_config": lc,
             "avg_launch_ms": f32["ms"]}
     out = {
         "metric": METRIC,

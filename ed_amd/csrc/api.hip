@@ -96,6 +96,9 @@ int qs3x_set_option(const char *name, double value) {
     } else if (n == "hv_xcd") {
       QS3X_REQUIRE(value >= 0 && value <= 64, "hv_xcd must be 0..64");
       options().hv_xcd = (int)value;
+    } else if (n == "hv_chunk") {
+      QS3X_REQUIRE(value >= -1 && value <= 24, "hv_chunk must be -1..24");
+      options().hv_chunk = (int)value;
     } else if (n == "hv_kernel") {
       QS3X_REQUIRE(value >= 0 && value <= 3, "hv_kernel must be 0..3");
       options().hv_kernel = (int)value;
@@ -401,8 +404,8 @@ int qs3x_hv_autotune(qs3x_sector *s, int32_t prec, int32_t n_rep, int32_t *best,
     QS3X_HIP(hipEventCreate(&e1));
     // Candidates: every H.v kernel that applies (the launcher falls through
     // to the next one when a kernel does not apply, so duplicates are timed
-    // and dropped by the order of the list), two tile schedules (round robin,
-    // one contiguous range per XCD), four grid sizes (4/4, 3/4, 2/4, 1/4 of
+    // and dropped by the order of the list), two tile schedules (the default,
+    // and round robin or 8 XCDs), four grid sizes (4/4, 3/4, 2/4, 1/4 of
     // the resident grid: fewer resident blocks can keep more of a large
     // vector's gathers in cache).
     std::vector<qs3x_sector::HvTune> cand;

@@ -44,18 +44,26 @@ struct Options {
   // 3 = general lattice kernel (translation scan).
   int hv_kernel = 0;
   // L2 domains (XCDs) for the H.v tile schedule; 1 = plain round-robin.
-  // MI355X has 8 XCDs: 8 halves the measured HBM traffic of the chain H.v at
-  // L = 1000 but leaves its time unchanged (the kernel is latency-bound), so
-  // the default stays 1.
   int hv_xcd = 0;  // 0: per precision (see hv_xcd_for)
+  // Nearest-neighbour chain kernels with hv_xcd > 1: 0 gives each domain one
+  // contiguous range of tiles; c > 0 deals chunks of 2^c tiles to the domains
+  // in turn, so every domain works through the whole row range (the rows that
+  // need the key search cluster at the end of the row order, and one range
+  // per domain leaves them to the last one).  -1: 5.
+  int hv_chunk = -1;
   int verbose = 0;            // 1: one stderr line per restart / per 100 Lanczos steps
 };
 Options &options();
-// Tile schedule of the H.v kernels: option hv_xcd, or by default one
-// contiguous range per XCD (8) in FP64 and plain round robin (1) in FP32,
-// the faster of each at L = 1000, N_down = 4 (0.684 vs 0.728 ms FP64;
-// 0.488 vs 0.514 ms FP32).
-inline int hv_xcd_for(int prec) { return options().hv_xcd > 0 ? options().hv_xcd : (prec == 1 /* QS3X_FP32 */ ? 1 : 8); }
+// Tile schedule of the H.v kernels: option hv_xcd, or by default the 8 XCDs
+// of MI355X, each working through interleaved chunks of 2^5 tiles (option
+// hv_chunk) in the nearest-neighbour kernels.  At L = 1000, N_down = 4 this
+// beats plain round robin and one range per XCD in both precisions (FP32
+// 0.392 / 0.398 / 0.445 ms, FP64 0.590 / 0.680 / 0.644 ms).
+inline int hv_xcd_for(int prec) {
+  (void)prec;
+  return options().hv_xcd > 0 ? options().hv_xcd : 8;
+}
+inline int hv_chunk_log2() { return options().hv_chunk >= 0 ? options().hv_chunk : 5; }
 
 // host eigensolvers (eig.cpp)
 double tridiag_eigenvalue(const double *d, const double *e, int n, int k);

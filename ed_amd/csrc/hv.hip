@@ -724,6 +724,7 @@ static HvArgs make_args(qs3x_sector *s, void *v0, const void *v1, const double *
   a.index = s->d_index.p;
   a.bitmap = s->d_bitmap.p;
   a.nbitmap = (int64_t)s->d_bitmap.n;
+  a.chunk_log2 = hv_chunk_log2();
   a.nn_bt = m->d_nn_bt.p;
   a.list_s = s->d_list_s.p;
   a.v1 = v1;

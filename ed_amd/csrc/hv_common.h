@@ -47,6 +47,7 @@ struct HvArgs {
   double *dot_partials; // 2 per block or null
   unsigned long long *nnz_out;
   int nxcd;             // tile ranges per L2 domain (1: plain round-robin over blocks)
+  int chunk_log2;       // nearest-neighbour kernels: interleaved chunks of 2^c tiles per domain (0: one range)
 };
 
 // Persistent-grid tile schedule.  Workgroups are dispatched round-robin over
@@ -64,6 +65,37 @@ __device__ __forceinline__ TileRange tile_range(int64_t ntiles, int nxcd) {
   const int64_t per = ((int64_t)gridDim.x - x + nx - 1) / nx;  // blocks on domain x
   const int64_t t0 = ntiles * x / nx, t1 = ntiles * (x + 1) / nx;
   return TileRange{t0 + (int64_t)(blockIdx.x / nx), t1, per};
+}
+
+// Interleaved variant for the nearest-neighbour kernels: chunks of 2^clog
+// tiles are dealt to the nxcd domains in turn, chunk g to domain g % nxcd.
+// A block walks domain-local tile numbers v = first, first + stride, ... <
+// end, and tile_at maps v to the global tile.  clog = 0 is tile_range.
+struct TileMap {
+  int64_t first, end, stride;
+  int nx, x, clog;
+};
+__device__ __forceinline__ TileMap tile_map(int64_t ntiles, int nxcd, int clog) {
+  if (clog <= 0 || nxcd <= 1) {
+    const TileRange r = tile_range(ntiles, nxcd);
+    return TileMap{r.first, r.end, r.stride, 1, 0, 0};
+  }
+  const int nx = (int)min<int64_t>(nxcd, (int64_t)gridDim.x);
+  const int x = (int)(blockIdx.x % nx);
+  const int64_t per = ((int64_t)gridDim.x - x + nx - 1) / nx;
+  const int64_t csz = (int64_t)1 << clog;
+  const int64_t nchunks = (ntiles + csz - 1) >> clog;
+  const int64_t K = x < nchunks ? (nchunks - x + nx - 1) / nx : 0;  // chunks of domain x
+  int64_t n = 0;
+  if (K > 0) {
+    const int64_t glast = (K - 1) * nx + x;
+    n = (K - 1) * csz + (glast == nchunks - 1 ? ntiles - glast * csz : csz);
+  }
+  return TileMap{(int64_t)(blockIdx.x / nx), n, per, nx, x, clog};
+}
+__device__ __forceinline__ int64_t tile_at(const TileMap &m, int64_t v) {
+  if (m.clog == 0) return v;
+  return ((((v >> m.clog) * m.nx) + m.x) << m.clog) + (v & (((int64_t)1 << m.clog) - 1));
 }
 
 constexpr int kThreads = 256;

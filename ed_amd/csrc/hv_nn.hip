@@ -436,8 +436,9 @@ k_hv_chain_nn(HvArgs a) {
   double dre = 0.0, dim_ = 0.0;
   const int64_t nsuper = (a.dim + (int64_t)kThreads * RPT - 1) / ((int64_t)kThreads * RPT);
 
-  const TileRange tr = tile_range(nsuper, a.nxcd);
-  for (int64_t t = tr.first; t < tr.end; t += tr.stride) {
+  const TileMap tm = tile_map(nsuper, a.nxcd, a.chunk_log2);
+  for (int64_t v = tm.first; v < tm.end; v += tm.stride) {
+    const int64_t t = tile_at(tm, v);
     int64_t row[RPT];
     bool valid[RPT], clear[RPT];
     uint64_t w[RPT];
@@ -544,8 +545,9 @@ k_hv_chain_nn2(HvArgs a) {
   const int64_t dim = a.dim;
   const int64_t ntile = (dim + 2 * kThreads - 1) / (2 * kThreads);
 
-  const TileRange tr = tile_range(ntile, a.nxcd);
-  for (int64_t t = tr.first; t < tr.end; t += tr.stride) {
+  const TileMap tm = tile_map(ntile, a.nxcd, a.chunk_log2);
+  for (int64_t v = tm.first; v < tm.end; v += tm.stride) {
+    const int64_t t = tile_at(tm, v);
     const int64_t r0 = t * (2 * kThreads) + wid * 128 + 2 * lane;
     const int64_t row[2] = {r0, r0 + 1};
     const bool valid[2] = {r0 < dim, r0 + 1 < dim};

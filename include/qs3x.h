@@ -72,11 +72,13 @@ int qs3x_set_device(int32_t device);
  *                   (s is lowered to fit).
  *   "rot_vec"       Ritz rotation with 16-byte basis loads where the slots are
  *                   16-byte aligned (1, default) or element loads (0).
- *   "hv_xcd"        L2 domains (XCDs) for the H.v tile schedule: each gets one
- *                   contiguous range of rows (8: MI355X's XCDs; 1: plain
- *                   round robin over workgroups).  0, the default: 1 for
- *                   FP32 and 8 for FP64 single-vector H.v, 8 for the batched
- *                   block H.v (the faster of each at L = 1000, N_down = 4).
+ *   "hv_xcd"        L2 domains (XCDs) for the H.v tile schedule (8: MI355X's
+ *                   XCDs; 1: plain round robin over workgroups).  0, the
+ *                   default: 8.
+ *   "hv_chunk"      Nearest-neighbour chain H.v with hv_xcd > 1: 0 gives each
+ *                   domain one contiguous range of rows; c > 0 deals chunks
+ *                   of 2^c tiles to the domains in turn.  -1, the default: 5
+ *                   (the fastest at L = 1000, N_down = 4 in both precisions).
  *   "verbose"       0 (default); 1: a stderr line per thick restart and per
  *                   100 Lanczos steps; 2: per convergence check.
  * The environment variable QS3X_OPTIONS="name=value,name=value" applies the

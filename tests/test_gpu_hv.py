@@ -174,3 +174,40 @@ def test_hv_autotuned_parity(case):
     gsec.hv_tune_clear()
     got = gsec.ham_to_vec_wave_vector(v)
     assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 1e-12
+
+
+@pytest.mark.parametrize("prec_name", ["fp32", "fp64"])
+def test_hv_tile_schedules_bit_identical(prec_name):
+    """Every tile schedule of the nearest-neighbour kernels (round robin, one
+    range per XCD, interleaved chunks of 2^c tiles) writes every row exactly
+    once with the same value: each row is computed by one lane in a fixed
+    order, so the outputs are bit-identical.  The output is filled with NaN
+    first, so a row that no block visits would show.  Chain L=120, N_down=4
+    (about 70,000 rows: many tiles, a partial last chunk, and chunks larger
+    than the sector), k=0 and k=7."""
+    print(f"[gpu] hv tile schedules {prec_name}", flush=True)
+    q3 = _q3()
+    prec = q3.FP32 if prec_name == "fp32" else q3.FP64
+    dt = torch.complex64 if prec_name == "fp32" else torch.complex128
+    L = 120
+    gm = q3.Model(L, L, 1, 1, orc.chain_bonds(L, [(1, 1.0, 0.9)]))
+    try:
+        for K in (0, 7):
+            s = q3.Sector.build(gm, 4, (q3.wavevector(K, L), 0.0, 0.0))
+            g = torch.Generator(device="cuda")
+            g.manual_seed(K + 1)
+            x = torch.randn(s.dim, dtype=dt, device="cuda", generator=g)
+            outs = []
+            for xcd, chunk in [(0, -1), (1, 0), (8, 0), (8, 1), (8, 3), (8, 9), (3, 2), (64, 0)]:
+                q3.set_option("hv_xcd", xcd)
+                q3.set_option("hv_chunk", chunk)
+                y = torch.full_like(x, float("nan"))
+                s.hv_dev(y.data_ptr(), x.data_ptr(), prec)
+                torch.cuda.synchronize()
+                outs.append(y.cpu().numpy())
+            assert not np.isnan(outs[0]).any()
+            for o in outs[1:]:
+                assert np.array_equal(o, outs[0])
+    finally:
+        q3.set_option("hv_xcd", 0)
+        q3.set_option("hv_chunk", -1)
