@@ -28,6 +28,13 @@
 // ranks: presence bits + row of the first present rank; 41 MB at L = 1000,
 // N_down = 4 instead of the 663 MB int32 table, so the lookups stay in L2 /
 // MALL).  All index reads of a row are issued together, then all gathers.
+//
+// The kernels are bound by their count of vector-memory instructions (a
+// wavefront load costs the texture path the same whether a lane moves 8 or
+// 16 bytes), so they save loads where they can: spin-1 neighbours come from
+// the adjacent lanes, and in FP32 one cooperative load reads the index words
+// of a whole wavefront, and k_hv_chain_nn2 gives each lane two adjacent rows
+// so that its streams and most gathers move 16 bytes per lane.
 #include <algorithm>
 
 #include "blas.h"
