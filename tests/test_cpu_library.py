@@ -152,11 +152,13 @@ def test_env_options_are_validated():
     for env, expect in (("hv_kernel=7", "QS3X_OPTIONS: hv_kernel must be 0..3"),
                         ("nonsense=1", "QS3X_OPTIONS: qs3x_set_option: unknown option nonsense"),
                         ("tr_reorth", "QS3X_OPTIONS: expected name=value"),
-                        ("tr_reorth=x", "QS3X_OPTIONS: bad value")):
+                        ("tr_reorth=x", "QS3X_OPTIONS: bad value"),
+                        ("hv_chunk=25", "QS3X_OPTIONS: hv_chunk must be -1..24"),
+                        ("hv_xcd=65", "QS3X_OPTIONS: hv_xcd must be 0..64")):
         out = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=120,
                              env=dict(os.environ, QS3X_OPTIONS=env, HIP_VISIBLE_DEVICES=""))
         assert expect in out.stdout, (env, out.stdout, out.stderr)
     out = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=120,
-                         env=dict(os.environ, QS3X_OPTIONS="tr_check_every=5,tr_reorth=2,tr_cgs_passes=1",
+                         env=dict(os.environ, QS3X_OPTIONS="tr_check_every=5,tr_reorth=2,tr_cgs_passes=1,hv_chunk=3",
                                   HIP_VISIBLE_DEVICES=""))
     assert "QS3X_OPTIONS" not in out.stdout, out.stdout
