@@ -67,34 +67,40 @@ __device__ __forceinline__ TileRange tile_range(int64_t ntiles, int nxcd) {
   return TileRange{t0 + (int64_t)(blockIdx.x / nx), t1, per};
 }
 
-// Interleaved variant for the nearest-neighbour kernels: chunks of 2^clog
+// Interleaved variant for the nearest-neighbour kernels: chunks of 2^c
 // tiles are dealt to the nxcd domains in turn, chunk g to domain g % nxcd.
-// A block walks domain-local tile numbers v = first, first + stride, ... <
-// end, and tile_at maps v to the global tile.  clog = 0 is tile_range.
+// c is clog, lowered until every domain owns at least 8 chunks (a domain's
+// blocks run on its XCD only, so a sector of few chunks would leave XCDs
+// idle); c = 0 deals single tiles.  A block walks domain-local tile numbers
+// v = first, first + stride, ... < end, and tile_at maps v to the global
+// tile.  clog <= 0 or nxcd <= 1 is tile_range.
 struct TileMap {
   int64_t first, end, stride;
   int nx, x, clog;
+  bool chunked;
 };
 __device__ __forceinline__ TileMap tile_map(int64_t ntiles, int nxcd, int clog) {
   if (clog <= 0 || nxcd <= 1) {
     const TileRange r = tile_range(ntiles, nxcd);
-    return TileMap{r.first, r.end, r.stride, 1, 0, 0};
+    return TileMap{r.first, r.end, r.stride, 1, 0, 0, false};
   }
   const int nx = (int)min<int64_t>(nxcd, (int64_t)gridDim.x);
   const int x = (int)(blockIdx.x % nx);
   const int64_t per = ((int64_t)gridDim.x - x + nx - 1) / nx;
-  const int64_t csz = (int64_t)1 << clog;
-  const int64_t nchunks = (ntiles + csz - 1) >> clog;
+  int c = clog;
+  while (c > 0 && (ntiles >> c) < (int64_t)nx * 8) --c;
+  const int64_t csz = (int64_t)1 << c;
+  const int64_t nchunks = (ntiles + csz - 1) >> c;
   const int64_t K = x < nchunks ? (nchunks - x + nx - 1) / nx : 0;  // chunks of domain x
   int64_t n = 0;
   if (K > 0) {
     const int64_t glast = (K - 1) * nx + x;
     n = (K - 1) * csz + (glast == nchunks - 1 ? ntiles - glast * csz : csz);
   }
-  return TileMap{(int64_t)(blockIdx.x / nx), n, per, nx, x, clog};
+  return TileMap{(int64_t)(blockIdx.x / nx), n, per, nx, x, c, true};
 }
 __device__ __forceinline__ int64_t tile_at(const TileMap &m, int64_t v) {
-  if (m.clog == 0) return v;
+  if (!m.chunked) return v;
   return ((((v >> m.clog) * m.nx) + m.x) << m.clog) + (v & (((int64_t)1 << m.clog) - 1));
 }
 

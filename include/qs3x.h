@@ -77,8 +77,10 @@ int qs3x_set_device(int32_t device);
  *                   default: 8.
  *   "hv_chunk"      Nearest-neighbour chain H.v with hv_xcd > 1: 0 gives each
  *                   domain one contiguous range of rows; c > 0 deals chunks
- *                   of 2^c tiles to the domains in turn.  -1, the default: 5
- *                   (the fastest at L = 1000, N_down = 4 in both precisions).
+ *                   of up to 2^c tiles to the domains in turn (smaller for
+ *                   small sectors, so that every domain owns at least 8).
+ *                   -1, the default: 5 (the fastest at L = 1000, N_down = 4
+ *                   in both precisions).
  *   "verbose"       0 (default); 1: a stderr line per thick restart and per
  *                   100 Lanczos steps; 2: per convergence check.
  * The environment variable QS3X_OPTIONS="name=value,name=value" applies the
