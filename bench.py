@@ -259,7 +259,9 @@ def main():
     ap.add_argument("--check-every", type=int, default=5,
                     help="thick restart: convergence check every n steps inside cycles (0: at restarts)")
     ap.add_argument("--conv-plain", type=float, default=1e-11)
-    ap.add_argument("--conv-mixed", type=float, default=1e-10)
+    # thick-restart stop tolerance: 3e-10 leaves the refined value 3x inside the
+    # 1e-9 target (measured 3.2e-10); 1e-9 itself ends at 8.8e-10
+    ap.add_argument("--conv-mixed", type=float, default=3e-10)
     ap.add_argument("--calibrate-copy", action="store_true", help="only run a 1 GiB device copy (counter scaling)")
     ap.add_argument("--dsf", action="store_true", help="run the batched 64-q DSF workload instead of the headline")
     ap.add_argument("--dsf-steps", type=int, default=200)
