@@ -238,3 +238,44 @@ def test_hv_fp32_unaligned_vectors(case):
         torch.cuda.synchronize()
         got = y.cpu().numpy().astype(np.complex128)
         assert np.max(np.abs(got - ref32)) <= 2e-6 * np.max(np.abs(ref32)), off
+
+
+LARGE_CHAINS = [
+    # name, L, N_down, K, bonds: sectors of 6,567 to 119,133 rows, where a
+    # wavefront's index words span several bitmap words, the per-XCD chunks
+    # are many, and runs of consecutive rows are long
+    ("chain128_n4_k7_dimer", 128, 4, 7, dimer(128, 1.0, 0.6, 0.9, 0.4)),
+    ("chain64_n5_k0", 64, 5, 0, chain(64, [(1, 1.0, 1.0)])),
+    ("chain100_n4_k50", 100, 4, 50, chain(100, [(1, 1.0, 0.7)])),
+    ("chain200_n3_k1", 200, 3, 1, chain(200, [(1, 1.0, 1.0)])),
+]
+
+
+@pytest.mark.parametrize("case", LARGE_CHAINS, ids=[c[0] for c in LARGE_CHAINS])
+def test_hv_large_chain_parity(case):
+    """The nearest-neighbour kernels on larger chains against the oracle:
+    FP64 within 1e-12 and FP32 (the two-rows-per-lane kernel) within 2e-6
+    relative, and the sector lists bit-identical."""
+    print(f"[gpu] hv large chain {case[0]}", flush=True)
+    q3 = _q3()
+    name, L, N, K, bonds = case
+    om = orc.Model(L, L, 1, 1, bonds)
+    rk = om.rk_from_K(K, 0, 0)
+    osec = om.build_sector(N, rk)
+    gm = q3.Model(L, L, 1, 1, bonds)
+    gsec = q3.Sector.build(gm, N, rk)
+    assert gsec.dim == osec.dim
+    ls, lr = gsec.lists()
+    assert np.array_equal(ls, osec.list_s) and np.array_equal(lr, osec.list_r)
+    rng = np.random.default_rng(23)
+    v = rng.uniform(-1, 1, osec.dim) + 1j * rng.uniform(-1, 1, osec.dim)
+    ref = om.hv(osec, v)
+    got = gsec.ham_to_vec_wave_vector(v)
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+    ref32 = ref - gm.calcu_FM_energy() * v
+    dv = torch.from_numpy(v.astype(np.complex64)).cuda()
+    out = torch.full_like(dv, float("nan"))
+    gsec.hv_dev(out.data_ptr(), dv.data_ptr(), q3.FP32)
+    torch.cuda.synchronize()
+    got32 = out.cpu().numpy().astype(np.complex128)
+    assert np.max(np.abs(got32 - ref32)) <= 2e-6 * np.max(np.abs(ref32))
