@@ -216,3 +216,32 @@ def test_dsf_recurrence(prec_name):
     s_o = orc.continued_fraction(a_o, b_o, n_o, 0.0, om_, 0.05)
     s_g = q3.dsf_analysis(a_o, b_o, n_o, 0.0, om_, 0.05)
     assert np.allclose(s_o, s_g, rtol=1e-12, atol=1e-14)
+
+
+def test_fp32_mixed_solvers_large_chain():
+    """FP32-mixed solvers on an 83,360-row sector (chain L=128, N_down=4,
+    k=0: the two-rows-per-lane H.v and the 16-byte Lanczos update) against
+    plain FP64 Lanczos: plain FP32-mixed Lanczos within 1e-8 relative, and the
+    FP32-mixed thick restart with its FP64 Rayleigh-Ritz step and refinement
+    within 1e-9."""
+    print("[gpu] fp32-mixed solvers L=128 N=4", flush=True)
+    q3 = _q3()
+    gm = q3.Model(128, 128, 1, 1, q3.chain_bonds(128, [(1, 1.0, 1.0)]))
+    s = q3.Sector.build(gm, 4, (0.0, 0.0, 0.0))
+    assert s.dim == 83360
+    E64, _, _ = q3.lanczos_routines(s, maxitr=3000, minitr=20, itrint=5, lnc_ene_conv=1e-14, prec=q3.FP64, seed=1,
+                                    want_vector=False)
+    E32, _, _ = q3.lanczos_routines(s, maxitr=3000, minitr=20, itrint=5, lnc_ene_conv=1e-12, prec=q3.FP32, seed=1,
+                                    want_vector=False)
+    assert abs(E32 - E64) <= 1e-8 * abs(E64), (E32, E64)
+    q3.set_option("tr_check_every", 5)
+    q3.set_option("tr_reorth", 2)
+    q3.set_option("tr_cgs_passes", 1)
+    try:
+        ene, _, st = q3.my_trlanczos_routines_complex(s, NOE=1, NOK=4, NOM=120, maxitr=400, lnc_ene_conv=1e-11,
+                                                       prec=q3.FP32, seed=1, want_vectors=False)
+    finally:
+        q3.set_option("tr_check_every", 0)
+        q3.set_option("tr_reorth", 0)
+        q3.set_option("tr_cgs_passes", 2)
+    assert abs(ene[0] - E64) <= 1e-9 * abs(E64), (ene[0], E64, st)
