@@ -769,6 +769,10 @@ static bool key32(qs3x_sector *s) {
   return (int64_t)(s->nod - 1) * kb <= 32;
 }
 
+// dynamic LDS the nearest-neighbour kernels may use: 160 KB per CU on gfx950
+// less their static queues (2 KB) and a margin
+constexpr size_t kNnMaxLds = 156 * 1024;
+
 static bool use_chain(qs3x_sector *s, int prec) {
   return s->chain && s->mode == kIndexAnchored && chain_lds_bytes(s, prec) <= 64 * 1024;
 }
@@ -797,12 +801,17 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   }
   const bool normlist = s->d_rclass.p == nullptr;  // sectors from caller lists carry list_r only
   const int pick = tn.set ? tn.kernel : options().hv_kernel;
-  if (pick <= 0 && use_chain(s, prec) && !normlist && s->model->nn_chain && s->d_bitmap.p) {
+  // The nearest-neighbour kernels need only their own LDS tables (binomials,
+  // bond types; not the adjacency of the general chain kernels), up to what
+  // a CU holds beside their 2 KB of static LDS.
+  if (pick <= 0 && s->chain && s->mode == kIndexAnchored && !normlist && s->model->nn_chain && s->d_bitmap.p) {
     const size_t lds = chain_nn_lds_bytes(s->nod, s->model->NOS, (int)s->model->bt_jxy2.size(), prec);
     const bool uniform = s->model->bt_jxy2.size() == 1;
     const bool pairs = (((uintptr_t)a.v0 | (uintptr_t)a.v1) & 15u) == 0;
-    KFn f = (lds <= 64 * 1024) ? chain_nn_kernel(s->nod, prec, s->real_phase, uniform, key32(s), pairs) : nullptr;
+    KFn f = (lds <= kNnMaxLds) ? chain_nn_kernel(s->nod, prec, s->real_phase, uniform, key32(s), pairs) : nullptr;
     if (f) {
+      if (lds > 64 * 1024)
+        QS3X_HIP(hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       grid = tuned(resident_grid(f, lds, a.ntiles));
       if (nblocks_out) *nblocks_out = grid;
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), lds, st, a);

@@ -139,7 +139,7 @@ using KFn = void (*)(HvArgs);
 // exists for this N / precision.
 // uniform: every bond of the chain has the same (Jxy, Jz).
 // pairs: FP32 vectors v0 and v1 are 16-byte aligned, so the two-rows-per-lane
-// kernel may be used (N_down <= 5).
+// kernel may be used (real phases, N_down <= 4).
 hvk::KFn chain_nn_kernel(int N, int prec, bool realph, bool uniform, bool key32, bool pairs);
 size_t chain_nn_lds_bytes(int N, int L, int nbt, int prec);
 

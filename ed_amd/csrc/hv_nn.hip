@@ -515,7 +515,7 @@ typedef float nn_f4 __attribute__((ext_vector_type(4)));
 typedef float nn_f4a8 __attribute__((ext_vector_type(4))) __attribute__((aligned(8)));
 typedef unsigned long long nn_u64x2 __attribute__((ext_vector_type(2)));
 
-// FP32, N_down <= 5: each lane holds two adjacent rows, row0 = 2 * lane and
+// FP32, real phases, N_down <= 4: each lane holds two adjacent rows, row0 = 2 * lane and
 // row0 + 1 of its wavefront's 128 consecutive rows.  A vector-memory
 // instruction costs the same whether a lane moves 8 or 16 bytes (measured on
 // L2-resident data: 16-19 cycles per wavefront load per CU for both), and
@@ -526,10 +526,8 @@ typedef unsigned long long nn_u64x2 __attribute__((ext_vector_type(2)));
 // is the cooperative one of finish_rows with one window of kSpan words per
 // hop slot covering both rows.  Requires 16-byte aligned vectors.
 template <int N, bool REALPH, bool UNI, class KeyT>
-__global__ __launch_bounds__(kThreads) void
-__attribute__((amdgpu_waves_per_eu(N <= 3 || (N == 4 && REALPH) ? 5 : 4)))  // 5 where that needs no scratch
-k_hv_chain_nn2(HvArgs a) {
-  static_assert(N >= 2 && N <= 5, "two-row kernel: 2 <= N_down <= 5");
+__global__ __launch_bounds__(kThreads) void __attribute__((amdgpu_waves_per_eu(5))) k_hv_chain_nn2(HvArgs a) {
+  static_assert(N >= 2 && N <= 4, "two-row kernel: 2 <= N_down <= 4");
   using R = float;
   using C = float2;
   constexpr int H = 2 * N;
@@ -735,8 +733,12 @@ template <int N, bool UNI, class KeyT>
 KFn pick_nn3(int prec, bool realph, bool pairs) {
   if (prec == QS3X_FP64)
     return realph ? k_hv_chain_nn<N, double, true, UNI, KeyT> : k_hv_chain_nn<N, double, false, UNI, KeyT>;
-  if constexpr (N <= 5) {
-    if (pairs) return realph ? k_hv_chain_nn2<N, true, UNI, KeyT> : k_hv_chain_nn2<N, false, UNI, KeyT>;
+  // the two-row kernel pays with real phases up to N_down = 4; with complex
+  // phases or N_down = 5 its registers cost a wavefront per SIMD (FP32 at
+  // L = 1000, N = 4, k = 1: 0.551 against 0.513 ms; L = 300, N = 5: 0.917
+  // against 0.874 ms)
+  if constexpr (N <= 4) {
+    if (pairs && realph) return k_hv_chain_nn2<N, true, UNI, KeyT>;
   }
   return realph ? k_hv_chain_nn<N, float, true, UNI, KeyT> : k_hv_chain_nn<N, float, false, UNI, KeyT>;
 }

@@ -279,3 +279,37 @@ def test_hv_large_chain_parity(case):
     torch.cuda.synchronize()
     got32 = out.cpu().numpy().astype(np.complex128)
     assert np.max(np.abs(got32 - ref32)) <= 2e-6 * np.max(np.abs(ref32))
+
+
+@pytest.mark.parametrize("L,K", [(8000, 0), (8000, 3), (20000, 1)])
+def test_hv_nn_long_chain_matches_general(L, K):
+    """Long chains (N_down = 2): the nearest-neighbour kernel runs on its own
+    LDS tables (48 KB at L = 8000; about 120 KB at L = 20000, beyond the
+    default 64 KB) where the general chain kernels' tables would not fit.
+    Its H.v equals the general translation-scan kernel's (option hv_kernel
+    3, pinned against the oracle on the smaller lattices): FP64 within 1e-12,
+    FP32 within 2e-6 relative.  The oracle's own sector build is too slow at
+    these lengths."""
+    print(f"[gpu] hv long chain L={L} k={K}", flush=True)
+    q3 = _q3()
+    gm = q3.Model(L, L, 1, 1, orc.chain_bonds(L, [(1, 1.0, 0.8)]))
+    s = q3.Sector.build(gm, 2, (q3.wavevector(K, L), 0.0, 0.0))
+    rng = np.random.default_rng(29)
+    v = rng.uniform(-1, 1, s.dim) + 1j * rng.uniform(-1, 1, s.dim)
+    out = {}
+    try:
+        for kern in (0, 3):
+            q3.set_option("hv_kernel", kern)
+            out[(kern, 64)] = s.ham_to_vec_wave_vector(v)
+            dv = torch.from_numpy(v.astype(np.complex64)).cuda()
+            y = torch.full_like(dv, float("nan"))
+            s.hv_dev(y.data_ptr(), dv.data_ptr(), q3.FP32)
+            torch.cuda.synchronize()
+            out[(kern, 32)] = y.cpu().numpy().astype(np.complex128)
+    finally:
+        q3.set_option("hv_kernel", 0)
+    ref = out[(3, 64)]
+    assert np.max(np.abs(out[(0, 64)] - ref)) <= 1e-12 * np.max(np.abs(ref))
+    ref32 = ref - gm.calcu_FM_energy() * v
+    for kern in (0, 3):
+        assert np.max(np.abs(out[(kern, 32)] - ref32)) <= 2e-6 * np.max(np.abs(ref32)), kern
