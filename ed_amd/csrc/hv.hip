@@ -774,7 +774,14 @@ static bool key32(qs3x_sector *s) {
 constexpr size_t kNnMaxLds = 156 * 1024;
 
 static bool use_chain(qs3x_sector *s, int prec) {
-  return s->chain && s->mode == kIndexAnchored && chain_lds_bytes(s, prec) <= 64 * 1024;
+  return s->chain && s->mode == kIndexAnchored && chain_lds_bytes(s, prec) <= kNnMaxLds;
+}
+
+// Kernels whose tables exceed the default 64 KB of dynamic LDS (long chains)
+// must say so before their launch.
+static void allow_lds(KFn f, size_t lds) {
+  if (lds > 64 * 1024)
+    QS3X_HIP(hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 }
 
 void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double *scale_dev,
@@ -810,8 +817,7 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
     const bool pairs = (((uintptr_t)a.v0 | (uintptr_t)a.v1) & 15u) == 0;
     KFn f = (lds <= kNnMaxLds) ? chain_nn_kernel(s->nod, prec, s->real_phase, uniform, key32(s), pairs) : nullptr;
     if (f) {
-      if (lds > 64 * 1024)
-        QS3X_HIP(hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      allow_lds(f, lds);
       grid = tuned(resident_grid(f, lds, a.ntiles));
       if (nblocks_out) *nblocks_out = grid;
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), lds, st, a);
@@ -822,6 +828,7 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   if (pick <= 1 && use_chain(s, prec) && !normlist) {
     KFn f = chain_mlp_kernel(s->nod, prec, s->real_phase, s->model->max_deg, key32(s));
     if (f) {
+      allow_lds(f, chain_lds_bytes(s, prec));
       grid = tuned(resident_grid(f, chain_lds_bytes(s, prec), a.ntiles));
       if (nblocks_out) *nblocks_out = grid;
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, prec), st, a);
@@ -832,6 +839,7 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   if (pick <= 2 && use_chain(s, prec) && !(normlist && prec != QS3X_FP64)) {
     KFn f = chain_kernel(s->nod, prec, s->real_phase, normlist, false, key32(s));
     if (f) {
+      allow_lds(f, chain_lds_bytes(s, prec));
       grid = tuned(resident_grid(f, chain_lds_bytes(s, prec), a.ntiles));
       if (nblocks_out) *nblocks_out = grid;
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, prec), st, a);
@@ -867,8 +875,10 @@ int64_t count_nnz(qs3x_sector *s) {
               ? chain_kernel(s->nod, QS3X_FP64, true, normlist, true, false) : nullptr;
   if (s->mode == kIndexU1)
     hipLaunchKernelGGL((k_hv_u1<double, true>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
-  else if (f)
+  else if (f) {
+    allow_lds(f, chain_lds_bytes(s, QS3X_FP64));
     hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, QS3X_FP64), st, a);
+  }
   else
     hipLaunchKernelGGL((k_hv_general<double, true, true>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
   QS3X_HIP(hipGetLastError());

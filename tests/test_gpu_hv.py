@@ -313,3 +313,35 @@ def test_hv_nn_long_chain_matches_general(L, K):
     ref32 = ref - gm.calcu_FM_energy() * v
     for kern in (0, 3):
         assert np.max(np.abs(out[(kern, 32)] - ref32)) <= 2e-6 * np.max(np.abs(ref32)), kern
+
+
+@pytest.mark.parametrize("K", [0, 2])
+def test_hv_long_j1j2_chain_matches_general(K):
+    """A long J1-J2 chain (L = 4000, N_down = 2; about 96 KB of LDS tables):
+    the batched-load and per-hop chain kernels (options hv_kernel 1 and 2)
+    equal the general translation-scan kernel (hv_kernel 3) within 1e-12 in
+    FP64 and 2e-6 in FP32."""
+    print(f"[gpu] hv long j1j2 chain k={K}", flush=True)
+    q3 = _q3()
+    L = 4000
+    gm = q3.Model(L, L, 1, 1, orc.chain_bonds(L, [(1, 1.0, 0.8), (2, 0.4, 0.3)]))
+    s = q3.Sector.build(gm, 2, (q3.wavevector(K, L), 0.0, 0.0))
+    rng = np.random.default_rng(31)
+    v = rng.uniform(-1, 1, s.dim) + 1j * rng.uniform(-1, 1, s.dim)
+    out = {}
+    try:
+        for kern in (1, 2, 3):
+            q3.set_option("hv_kernel", kern)
+            out[(kern, 64)] = s.ham_to_vec_wave_vector(v)
+            dv = torch.from_numpy(v.astype(np.complex64)).cuda()
+            y = torch.full_like(dv, float("nan"))
+            s.hv_dev(y.data_ptr(), dv.data_ptr(), q3.FP32)
+            torch.cuda.synchronize()
+            out[(kern, 32)] = y.cpu().numpy().astype(np.complex128)
+    finally:
+        q3.set_option("hv_kernel", 0)
+    ref = out[(3, 64)]
+    ref32 = ref - gm.calcu_FM_energy() * v
+    for kern in (1, 2):
+        assert np.max(np.abs(out[(kern, 64)] - ref)) <= 1e-12 * np.max(np.abs(ref)), kern
+        assert np.max(np.abs(out[(kern, 32)] - ref32)) <= 2e-6 * np.max(np.abs(ref32)), kern
