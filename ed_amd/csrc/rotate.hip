@@ -171,9 +171,15 @@ void ritz_rotate(int prec, void *base, int64_t dim, const double *scales_dev, in
     const int64_t vt = (M + 16 * (16 / (int64_t)rsz) - 1) / (16 * (16 / (int64_t)rsz));
     grid = (int)std::max<int64_t>(1, std::min<int64_t>((vt + waves - 1) / waves, (int64_t)vec_grid(dim) * 2));
   }
-  QS3X_REQUIRE(lds <= 64 * 1024, "ritz_rotate: NOM x NOK too large for LDS staging");
+  // up to what a CU holds (160 KB on gfx950, less a margin); beyond 64 KB the
+  // kernel must be told before its launch
+  QS3X_REQUIRE(lds <= 156 * 1024, "ritz_rotate: NOM x NOK too large for LDS staging");
 #define ROT(R, NTV)                                                                                           \
   do {                                                                                                        \
+    if (lds > 64 * 1024)                                                                                      \
+      QS3X_HIP(hipFuncSetAttribute(vec ? (const void *)&k_rotate_mfma_v<R, NTV, 4>                            \
+                                       : (const void *)&k_rotate_mfma<R, NTV>,                                \
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));                    \
     if (vec)                                                                                                  \
       hipLaunchKernelGGL((k_rotate_mfma_v<R, NTV, 4>), dim3(grid), dim3(kRotThreads), lds, st, (R *)base, M, \
                          scales_dev, nom, nok, C_dev);                                                        \

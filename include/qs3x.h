@@ -61,8 +61,9 @@ int qs3x_set_device(int32_t device);
  *                   ends the cycle there, once the change is below conv and
  *                   so is the remaining change extrapolated from the last two
  *                   (with tr_reorth >= 1 its first cycle is the plain
- *                   three-term recurrence).  NOM is at most 1023 (and
- *                   NOM x NOK at most 8192 for the rotation's staging).
+ *                   three-term recurrence).  NOM is at most 1023, and the
+ *                   rotation stages NOM x NOK FP64 coefficients (NOM x 16
+ *                   ceil(NOK / 16) with element loads) in at most 156 KB.
  *   "fp64_refine"   FP32-mixed thick restart: after the FP64 Rayleigh-Ritz
  *                   values are accepted, Rayleigh-Ritz on the block Krylov
  *                   space {Y, HY, ..., H^s Y} of the NOE accepted vectors, in

@@ -132,7 +132,9 @@ def test_ritz_rotation_mfma(rot_vec):
                                         ctypes.c_int32, ctypes.c_void_p]
     rng = np.random.default_rng(9)
     for dt, tol in ((np.complex128, 1e-13), (np.complex64, 2e-6)):
-        for dim, nom, nok in ((1000, 12, 5), (777, 30, 17), (5, 8, 4), (4099, 41, 33), (4098, 64, 64), (130, 96, 48)):
+        # the last two stage 94-113 KB of coefficients in LDS (beyond the default 64 KB)
+        for dim, nom, nok in ((1000, 12, 5), (777, 30, 17), (5, 8, 4), (4099, 41, 33), (4098, 64, 64), (130, 96, 48),
+                              (3000, 200, 64), (2001, 300, 40)):
             V = (rng.normal(size=(nom + 1, dim)) + 1j * rng.normal(size=(nom + 1, dim))).astype(dt)
             C = rng.normal(size=(nom, nok))
             ref = (C.T @ V[:nom].astype(np.complex128))
