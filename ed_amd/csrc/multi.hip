@@ -527,7 +527,11 @@ int qs3x_lanczos_group(qs3x_sector *const *secs, int32_t nsec, int32_t prec, int
       const size_t rs = prec == QS3X_FP64 ? 8 : 4;
       const size_t nb = N > 1 ? (size_t)(N - 1) * (m->NOS - 1) : 0;
       g.lds = ((nb * 4 + 15) & ~(size_t)15) + 2 * rs * m->bt_jxy2.size() + 4 * (size_t)(m->NOS + 2) + 4 * m->adj.size();
-      QS3X_REQUIRE(g.lds <= 64 * 1024, "qs3x_lanczos_group: LDS tables too large");
+      // up to what a CU holds (160 KB on gfx950, less a margin); beyond 64 KB
+      // the kernel must be told before the occupancy query and the launches
+      QS3X_REQUIRE(g.lds <= 156 * 1024, "qs3x_lanczos_group: LDS tables too large");
+      if (g.lds > 64 * 1024)
+        QS3X_HIP(hipFuncSetAttribute((const void *)g.f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.lds));
       int nbpc = 0;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbpc, g.f, kMT, g.lds) != hipSuccess || nbpc < 1) nbpc = 1;
       g.grid = (int)std::max<int64_t>(1, std::min<int64_t>(tile0[k] - tile0[j], (int64_t)cus() * nbpc));

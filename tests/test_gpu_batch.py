@@ -163,3 +163,20 @@ def test_lanczos_group_matches_per_sector(prec_name, graph):
             H = om.full_hamiltonian(osec)
             ed = np.linalg.eigvalsh(H)[0]
             assert abs(e - ed) <= 1e-9 * max(1.0, abs(ed)), (N, K, e, ed)
+
+
+def test_lanczos_group_long_chain():
+    """A group of long-chain sectors (L = 6000, N_down = 2, k = 0..3; about
+    96 KB of LDS tables, beyond the default 64 KB): each E0 equals the
+    per-sector plain Lanczos within 1e-9 (FP64)."""
+    print("[gpu] lanczos group long chain", flush=True)
+    q3 = _q3()
+    L = 6000
+    gm = q3.Model(L, L, 1, 1, orc.chain_bonds(L, [(1, 1.0, 0.8)]))
+    secs = [q3.Sector.build(gm, 2, (q3.wavevector(K, L), 0.0, 0.0)) for K in range(4)]
+    E, its, st = q3.lanczos_group(secs, maxitr=400, minitr=10, itrint=5, lnc_ene_conv=1e-12, prec=q3.FP64, seed=5,
+                                  use_graph=True)
+    for s, e in zip(secs, E):
+        e1, _, _ = q3.lanczos_routines(s, maxitr=400, minitr=10, itrint=5, lnc_ene_conv=1e-12, prec=q3.FP64, seed=5,
+                                       want_vector=False)
+        assert abs(e - e1) <= 1e-9 * max(1.0, abs(e1)), (e, e1)
