@@ -67,3 +67,15 @@ if [ -d "$SRC1" ]; then
     "$OUT1/ref_harness_u1.o"
   echo "built $OUT/libqs3ref_u1.so"
 fi
+
+# DSFLan (source_DSFLan/main.f90): plain Fortran, no BLAS.  Built unmodified as
+# an executable with the reference's default integers; it reads
+# alpha_beta_values_E0-tmp.d from its working directory and appends
+# DSF_itrnum*_eta004.dat there (tests/test_dsflan_reference.py runs it in a
+# scratch directory).
+SRCD="$REF/source_DSFLan/main.f90"
+if [ -f "$SRCD" ]; then
+  mkdir -p "$OUT/dsflan"
+  $FC -O2 -module-dir "$OUT/dsflan" "$SRCD" -o "$OUT/dsflan.exe"
+  echo "built $OUT/dsflan.exe"
+fi

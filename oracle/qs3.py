@@ -18,11 +18,16 @@ Parity status of each oracle function (how it is pinned, tests/ file):
          be built here; pinned by exact spectra (ferromagnet, one magnon,
          XX free fermions, Bethe ansatz) and dense numpy.linalg.eigh
          (tests/test_oracle_math.py).
-  dsf_recurrence, continued_fraction
+  dsf_recurrence
       -- pinned by the sum rule norm1 = <psi|S+_q S-_q|psi> (through the
          correlation functions), by the spectral (Lehmann) representation of
          the resolvent on small dense sectors, and by the first moment alpha_1
          (tests/test_oracle_math.py).
+  continued_fraction
+      -- reference_quirk=True: pinned to 1e-13 against the output of the
+         reference's own DSFLan program compiled here (oracle/_ref/dsflan.exe;
+         fixtures tests/golden/dsflan_*.npz, tests/test_dsflan_reference.py);
+         reference_quirk=False: pinned by the Lehmann sum of the Jacobi matrix.
 """
 import ctypes
 import math

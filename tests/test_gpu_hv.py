@@ -108,7 +108,7 @@ def test_hv_fp32_parity(case, kernel):
     om, osec, gm, gsec = _build(case)
     rng = np.random.default_rng(11)
     v = rng.uniform(-1, 1, osec.dim) + 1j * rng.uniform(-1, 1, osec.dim)
-    ref = om.hv(osec, v) - gm.calcu_FM_energy() * v
+    ref = om.hv(osec, v) - om.fm_energy() * v
     dv = torch.from_numpy(v.astype(np.complex64)).cuda()
     out = torch.empty_like(dv)
     q3.set_option("hv_kernel", kernel)
@@ -138,14 +138,6 @@ def test_hv_from_reference_lists():
         assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 1e-12
 
 
-def test_start_vector_matches_host_formula():
-    """The device start-vector generator equals the documented splitmix64 formula."""
-    q3 = _q3()
-    # exercised through lanczos with a tiny sector: covered in test_gpu_solvers
-    v = q3.start_vector(1000, 5)
-    assert np.all(np.abs(v.real) <= 1) and np.all(np.abs(v.imag) <= 1)
-
-
 @pytest.mark.parametrize("case", [c for c in CASES if c[0] in ("chain12_n4_k3_j1j2", "chain16_n8_k4", "chain10_n5_k5",
                                                              "torus6x6_n3_k12")], ids=lambda c: c[0])
 def test_hv_autotuned_parity(case):
@@ -164,7 +156,7 @@ def test_hv_autotuned_parity(case):
     got = gsec.ham_to_vec_wave_vector(v)
     assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 1e-12
     best32, ms32, _ = gsec.hv_autotune(q3.FP32, n_rep=2)
-    ref32 = ref - gm.calcu_FM_energy() * v
+    ref32 = ref - om.fm_energy() * v
     dv = torch.from_numpy(v.astype(np.complex64)).cuda()
     out = torch.empty_like(dv)
     gsec.hv_dev(out.data_ptr(), dv.data_ptr(), q3.FP32)
@@ -226,7 +218,7 @@ def test_hv_fp32_unaligned_vectors(case):
     om, osec, gm, gsec = _build(case)
     rng = np.random.default_rng(17)
     v = rng.uniform(-1, 1, osec.dim) + 1j * rng.uniform(-1, 1, osec.dim)
-    ref32 = om.hv(osec, v) - gm.calcu_FM_energy() * v
+    ref32 = om.hv(osec, v) - om.fm_energy() * v
     D = gsec.dim
     for off in (0, 1):
         xb = torch.zeros(D + 1, dtype=torch.complex64, device="cuda")
@@ -272,7 +264,7 @@ def test_hv_large_chain_parity(case):
     ref = om.hv(osec, v)
     got = gsec.ham_to_vec_wave_vector(v)
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
-    ref32 = ref - gm.calcu_FM_energy() * v
+    ref32 = ref - om.fm_energy() * v
     dv = torch.from_numpy(v.astype(np.complex64)).cuda()
     out = torch.full_like(dv, float("nan"))
     gsec.hv_dev(out.data_ptr(), dv.data_ptr(), q3.FP32)
@@ -310,7 +302,7 @@ def test_hv_nn_long_chain_matches_general(L, K):
         q3.set_option("hv_kernel", 0)
     ref = out[(3, 64)]
     assert np.max(np.abs(out[(0, 64)] - ref)) <= 1e-12 * np.max(np.abs(ref))
-    ref32 = ref - gm.calcu_FM_energy() * v
+    ref32 = ref - (gm.bonds[:, 3].sum() / 4.0) * v
     for kern in (0, 3):
         assert np.max(np.abs(out[(kern, 32)] - ref32)) <= 2e-6 * np.max(np.abs(ref32)), kern
 
@@ -341,7 +333,7 @@ def test_hv_long_j1j2_chain_matches_general(K):
     finally:
         q3.set_option("hv_kernel", 0)
     ref = out[(3, 64)]
-    ref32 = ref - gm.calcu_FM_energy() * v
+    ref32 = ref - (gm.bonds[:, 3].sum() / 4.0) * v
     for kern in (1, 2):
         assert np.max(np.abs(out[(kern, 64)] - ref)) <= 1e-12 * np.max(np.abs(ref)), kern
         assert np.max(np.abs(out[(kern, 32)] - ref32)) <= 2e-6 * np.max(np.abs(ref32)), kern

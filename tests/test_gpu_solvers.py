@@ -44,8 +44,15 @@ def test_start_vector_generator_matches_formula():
     v = q3.start_vector(gsec.dim, 77)
     v = v / np.linalg.norm(v)
     H = om.full_hamiltonian(osec)
-    E, _, it = q3.lanczos_routines(gsec, maxitr=1, minitr=1, itrint=1, seed=77, want_vector=False)
+    E, psi, it = q3.lanczos_routines(gsec, maxitr=1, minitr=1, itrint=1, seed=77, want_vector=True)
     assert it == 1 and E == pytest.approx(np.vdot(v, H @ v).real, abs=1e-12)
+    # after one step the Ritz vector is the normalised start vector itself:
+    # the device generator equals the host formula element by element
+    assert np.max(np.abs(psi - v)) <= 1e-14, np.max(np.abs(psi - v))
+    # and a host start vector equal to it gives the same energy bit for bit
+    E2, _, _ = q3.lanczos_routines(gsec, maxitr=1, minitr=1, itrint=1, start=q3.start_vector(gsec.dim, 77),
+                                   want_vector=False)
+    assert E2 == E
 
 
 @pytest.mark.parametrize("prec_name", ["fp64", "fp32"])
@@ -70,7 +77,7 @@ def test_lanczos_bethe_c1_size():
     s = q3.Sector.build(gm, 6, (0.0, 0.0, 0.0))
     assert s.dim == 54132
     E, _, it = q3.lanczos_routines(s, maxitr=3000, lnc_ene_conv=1e-14, seed=1, want_vector=False)
-    assert E == pytest.approx(-2.687328296769200, abs=1e-10)
+    assert E == pytest.approx(-2.687328296769200, abs=1e-12)
 
 
 @pytest.mark.parametrize("prec_name", ["fp64", "fp32"])

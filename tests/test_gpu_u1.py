@@ -56,7 +56,7 @@ def test_u1_hv_parity(case):
     ref = om.hv_u1(N, v)
     got = gs.ham_to_vec_wave_vector(v)
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
-    ref32 = ref - gm.calcu_FM_energy() * v
+    ref32 = ref - om.fm_energy() * v
     dv = torch.from_numpy(v.astype(np.complex64)).cuda()
     out = torch.empty_like(dv)
     gs.hv_dev(out.data_ptr(), dv.data_ptr(), q3.FP32)
