@@ -362,6 +362,14 @@ int qs3x_hv_dev(qs3x_sector *s, int32_t prec, void *v0_dev, const void *v1_dev, 
   });
 }
 
+int qs3x_hv_last_launch(const qs3x_sector *s, int32_t *family, int64_t *lds_bytes) {
+  return guarded([&] {
+    QS3X_REQUIRE(s && family && lds_bytes, "qs3x_hv_last_launch: NULL argument");
+    *family = s->last_family;
+    *lds_bytes = s->last_lds;
+  });
+}
+
 int qs3x_time_hv(qs3x_sector *s, int32_t prec, int32_t n_rep, double *ms) {
   return guarded([&] {
     QS3X_REQUIRE(s && ms && n_rep > 0, "qs3x_time_hv: bad argument");

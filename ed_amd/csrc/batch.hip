@@ -758,7 +758,9 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
     size_t lds = 0;
     BFn hv = block_kernel(NT, prec, key32, &lds, L, (int)m->bt_jxy2.size());
     QS3X_REQUIRE(hv, "qs3x_dsf_sminus_batch: no block kernel for this N_down");
-    QS3X_REQUIRE(lds <= 160 * 1024, "qs3x_dsf_sminus_batch: L too large for the block kernel's LDS tables");
+    // 160 KB per CU less the kernel's static LDS (its reduction arrays) and a margin
+    QS3X_REQUIRE(lds <= 156 * 1024, "qs3x_dsf_sminus_batch: L too large for the block kernel's LDS tables "
+                                    "(over 156 KB)");
     if (lds > 64 * 1024)
       QS3X_HIP(hipFuncSetAttribute((const void *)hv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int nbpc = 0;

@@ -24,6 +24,8 @@
 // contain site 1, and looked up in a dense rank-addressed index (one 4-byte
 // read) -- replacing findstate's binary search over list_s.
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 #include "blas.h"
 #include "hv_common.h"
@@ -778,10 +780,17 @@ static bool use_chain(qs3x_sector *s, int prec) {
 }
 
 // Kernels whose tables exceed the default 64 KB of dynamic LDS (long chains)
-// must say so before their launch.
+// must say so before their launch.  The attribute is set once per kernel (to
+// the largest size asked so far), not on every launch of a Lanczos loop.
 static void allow_lds(KFn f, size_t lds) {
-  if (lds > 64 * 1024)
-    QS3X_HIP(hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  if (lds <= 64 * 1024) return;
+  static std::mutex mu;
+  static std::map<KFn, size_t> done;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = done.find(f);
+  if (it != done.end() && it->second >= lds) return;
+  QS3X_HIP(hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  done[f] = lds;
 }
 
 void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double *scale_dev,
@@ -804,6 +813,8 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
     else
       hipLaunchKernelGGL((k_hv_u1<float, false>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
     QS3X_HIP(hipGetLastError());
+    s->last_family = 4;
+    s->last_lds = 0;
     return;
   }
   const bool normlist = s->d_rclass.p == nullptr;  // sectors from caller lists carry list_r only
@@ -822,6 +833,8 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
       if (nblocks_out) *nblocks_out = grid;
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), lds, st, a);
       QS3X_HIP(hipGetLastError());
+      s->last_family = 0;
+      s->last_lds = (int64_t)lds;
       return;
     }
   }
@@ -833,6 +846,8 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
       if (nblocks_out) *nblocks_out = grid;
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, prec), st, a);
       QS3X_HIP(hipGetLastError());
+      s->last_family = 1;
+      s->last_lds = (int64_t)chain_lds_bytes(s, prec);
       return;
     }
   }
@@ -844,6 +859,8 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
       if (nblocks_out) *nblocks_out = grid;
       hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, prec), st, a);
       QS3X_HIP(hipGetLastError());
+      s->last_family = 2;
+      s->last_lds = (int64_t)chain_lds_bytes(s, prec);
       return;
     }
   }
@@ -859,6 +876,8 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
       hipLaunchKernelGGL((k_hv_general<float, false, false>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
   }
   QS3X_HIP(hipGetLastError());
+  s->last_family = 3;
+  s->last_lds = 0;
 }
 
 int64_t count_nnz(qs3x_sector *s) {

@@ -138,6 +138,11 @@ struct qs3x_sector {
     int xcd = 1;     // L2 domains of the tile schedule, as option hv_xcd
     int grid_q = 4;  // quarters of the resident grid launched (4: all of it)
   } tune[2];
+  // the last H.v launch: kernel family (0 nearest-neighbour chain, 1
+  // batched-load chain, 2 per-hop chain, 3 general lattice, 4 U(1)-only) and
+  // its dynamic LDS bytes (qs3x_hv_last_launch)
+  int last_family = -1;
+  int64_t last_lds = 0;
 };
 
 namespace qs3x {

@@ -154,6 +154,13 @@ __global__ __launch_bounds__(kRotThreads) void k_rotate_mfma_v(R *base, int64_t 
 
 }  // namespace
 
+size_t ritz_rotate_lds(int prec, int64_t dim, int nom, int nok, bool base16) {
+  const size_t rsz = prec == QS3X_FP64 ? sizeof(double) : sizeof(float);
+  const bool vec = base16 && ((2 * dim * (int64_t)rsz) % 16 == 0) && options().rot_vec;
+  if (vec) return sizeof(double) * (size_t)nom * nok;
+  return sizeof(double) * (size_t)((nom + 3) & ~3) * (size_t)((nok + 15) / 16) * 16;
+}
+
 void ritz_rotate(int prec, void *base, int64_t dim, const double *scales_dev, int nom, int nok,
                  const double *C_dev, hipStream_t st) {
   QS3X_REQUIRE(nok >= 1 && nok <= 16 * kMaxNT, "ritz_rotate: NOK must be in 1..64");

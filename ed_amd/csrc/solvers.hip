@@ -24,6 +24,7 @@
 using namespace qs3x;
 
 namespace qs3x {
+size_t ritz_rotate_lds(int prec, int64_t dim, int nom, int nok, bool base16);
 void ritz_rotate(int prec, void *base, int64_t dim, const double *scales_dev, int nom, int nok,
                  const double *C_dev, hipStream_t st);
 }
@@ -832,6 +833,14 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
     const int64_t dim = s->dim;
     QS3X_REQUIRE(!(dim < NOE || dim < NOK || dim < NOM), "qs3x_trlanczos: dim < NOM (stop 999)");
     QS3X_REQUIRE(NOM + 1 <= kMaxBasis, "qs3x_trlanczos: NOM too large");
+    // the Ritz rotation stages NOM x NOK coefficients in LDS: refuse here,
+    // before the basis is allocated, rather than at the first restart
+    if (NOK < NOM) {
+      const size_t lds_rot = ritz_rotate_lds(prec, dim, NOM, NOK, true);
+      QS3X_REQUIRE(lds_rot <= 156 * 1024,
+                   "qs3x_trlanczos: NOM x NOK too large for the Ritz rotation's LDS staging (" +
+                       std::to_string(lds_rot) + " bytes of coefficients, at most 156 KB)");
+    }
     hipStream_t st = stream();
     // Solve time and the trace exclude the Krylov basis allocations (their
     // cost depends on what the process freed before: 0 to about 4.5 s for

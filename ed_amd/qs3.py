@@ -56,6 +56,7 @@ def _load():
         "qs3x_ham_to_vec_wave_vector": [_P, _P, _P],
         "qs3x_hv_dev": [_P, _i32, _P, _P, _P],
         "qs3x_time_hv": [_P, _i32, _i32, _P],
+        "qs3x_hv_last_launch": [_P, _P, _P],
         "qs3x_hv_autotune": [_P, _i32, _i32, _P, _P, _i32, _P, _P],
         "qs3x_hv_tune_clear": [_P],
         "qs3x_lanczos": [_P, _i32, _i64, _i64, _i64, _dbl, _P, _u64, _P, _P, _P],
@@ -219,6 +220,15 @@ class Sector:
         """Device-pointer H.v (FP32: v0 = (H - e_zz) v1)."""
         _check(_lib.qs3x_hv_dev(self._h, int(prec), ctypes.c_void_p(v0_ptr), ctypes.c_void_p(v1_ptr),
                                 ctypes.c_void_p(stream or 0)))
+
+    def hv_last_launch(self):
+        """(kernel family, dynamic LDS bytes) of the last H.v launch: 0
+        nearest-neighbour chain, 1 batched-load chain, 2 per-hop chain, 3
+        general lattice, 4 U(1)-only."""
+        f = np.zeros(1, dtype=np.int32)
+        lds = np.zeros(1, dtype=np.int64)
+        _check(_lib.qs3x_hv_last_launch(self._h, _p(f), _p(lds)))
+        return int(f[0]), int(lds[0])
 
     def time_hv(self, prec=FP64, n_rep=10):
         ms = np.zeros(1)

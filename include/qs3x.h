@@ -273,6 +273,11 @@ int qs3x_continued_fraction(int32_t nlev, const double *alpha, const double *bet
                             int32_t reference_quirk, double *dsf);
 
 /* ---------------------------------------------------------- measurement */
+/* The kernel family of the sector's last H . v launch (no counterpart in the
+ * reference): 0 nearest-neighbour chain, 1 batched-load chain, 2 per-hop
+ * chain, 3 general lattice, 4 U(1)-only; -1 before the first launch.
+ * lds_bytes: its dynamic LDS. */
+int qs3x_hv_last_launch(const qs3x_sector *s, int32_t *family, int64_t *lds_bytes);
 /* Times n_rep device H . v applications (events on the library stream) and
  * returns the mean milliseconds per application. */
 int qs3x_time_hv(qs3x_sector *s, int32_t prec, int32_t n_rep, double *ms);

@@ -293,11 +293,16 @@ def test_hv_nn_long_chain_matches_general(L, K):
         for kern in (0, 3):
             q3.set_option("hv_kernel", kern)
             out[(kern, 64)] = s.ham_to_vec_wave_vector(v)
+            fam64, lds64 = s.hv_last_launch()
             dv = torch.from_numpy(v.astype(np.complex64)).cuda()
             y = torch.full_like(dv, float("nan"))
             s.hv_dev(y.data_ptr(), dv.data_ptr(), q3.FP32)
             torch.cuda.synchronize()
             out[(kern, 32)] = y.cpu().numpy().astype(np.complex128)
+            # the kernel asked for is the one that ran (no silent fallback)
+            assert s.hv_last_launch()[0] == fam64 == kern, (kern, fam64, s.hv_last_launch())
+            if kern == 0 and L == 20000:
+                assert lds64 > 64 * 1024
     finally:
         q3.set_option("hv_kernel", 0)
     ref = out[(3, 64)]
@@ -325,11 +330,17 @@ def test_hv_long_j1j2_chain_matches_general(K):
         for kern in (1, 2, 3):
             q3.set_option("hv_kernel", kern)
             out[(kern, 64)] = s.ham_to_vec_wave_vector(v)
+            fam64, lds64 = s.hv_last_launch()
             dv = torch.from_numpy(v.astype(np.complex64)).cuda()
             y = torch.full_like(dv, float("nan"))
             s.hv_dev(y.data_ptr(), dv.data_ptr(), q3.FP32)
             torch.cuda.synchronize()
             out[(kern, 32)] = y.cpu().numpy().astype(np.complex128)
+            # the kernel asked for ran, with its tables beyond 64 KB of LDS
+            fam32, lds32 = s.hv_last_launch()
+            assert fam64 == fam32 == kern, (kern, fam64, fam32)
+            if kern in (1, 2):
+                assert lds64 > 64 * 1024 and lds32 > 64 * 1024, (kern, lds64, lds32)
     finally:
         q3.set_option("hv_kernel", 0)
     ref = out[(3, 64)]
