@@ -54,6 +54,7 @@ def _load():
         "qs3x_sector_lists": [_P, _P, _P],
         "qs3x_sector_nnz": [_P, _P],
         "qs3x_ham_to_vec_wave_vector": [_P, _P, _P],
+        "qs3x_ham_to_vec_wave_vector_noalloc": [_P, _P, _P],
         "qs3x_hv_dev": [_P, _i32, _P, _P, _P],
         "qs3x_time_hv": [_P, _i32, _i32, _P],
         "qs3x_hv_last_launch": [_P, _P, _P],
@@ -214,6 +215,15 @@ class Sector:
         assert a.shape == (self.dim,)
         out = np.zeros_like(a)
         _check(_lib.qs3x_ham_to_vec_wave_vector(self._h, _p(out), _p(a)))
+        return out
+
+    def ham_to_vec_wave_vector_noalloc(self, v1):
+        """v0 = H v1 (host vectors) staged through the library's static device
+        array: no device allocation; dim <= 2,097,152."""
+        a = _c128(v1)
+        assert a.shape == (self.dim,)
+        out = np.zeros_like(a)
+        _check(_lib.qs3x_ham_to_vec_wave_vector_noalloc(self._h, _p(out), _p(a)))
         return out
 
     def hv_dev(self, v0_ptr, v1_ptr, prec=FP64, stream=None):

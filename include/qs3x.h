@@ -142,6 +142,13 @@ int qs3x_sector_nnz(qs3x_sector *s, int64_t *nnz);
  * Replaces ham_to_vec_wave_vector (ham2vec_v3.f90:37-72).  Stages both
  * vectors over PCIe; solvers below keep everything on the device. */
 int qs3x_ham_to_vec_wave_vector(qs3x_sector *s, double *v0, const double *v1);
+/* The same H . v as qs3x_ham_to_vec_wave_vector (host vectors, FP64), with no
+ * device allocation: no counterpart in the reference; for callers that cannot
+ * allocate device memory (loops that forbid hipMalloc, harnesses that
+ * pre-size all device memory).  Both vectors are staged through one static
+ * per-device array of 64 MiB, so dim <= 2,097,152 rows (larger sectors return
+ * an error).  Calls that overlap on one GPU share that array and race. */
+int qs3x_ham_to_vec_wave_vector_noalloc(qs3x_sector *s, double *v0, const double *v1);
 /* Device-pointer H . v on the given stream (hipStream_t passed as void*;
  * NULL is the HIP null stream).  prec: QS3X_FP64 -> v0, v1 complex double;
  * QS3X_FP32 -> complex float with the e_zz shift (v0 = (H - e_zz) v1). */
