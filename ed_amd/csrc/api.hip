@@ -102,6 +102,15 @@ int qs3x_set_option(const char *name, double value) {
     } else if (n == "hv_kernel") {
       QS3X_REQUIRE(value >= 0 && value <= 3, "hv_kernel must be 0..3");
       options().hv_kernel = (int)value;
+    } else if (n == "hv_stencil") {
+      QS3X_REQUIRE(value == 0 || value == 1, "hv_stencil must be 0 or 1");
+      options().hv_stencil = (int)value;
+    } else if (n == "hv_stencil_lines") {
+      QS3X_REQUIRE(value >= 0 && value <= 8, "hv_stencil_lines must be 0..8");
+      options().hv_stencil_lines = (int)value;
+    } else if (n == "hv_stencil_slabs") {
+      QS3X_REQUIRE(value >= 0 && value <= 1024, "hv_stencil_slabs must be 0..1024");
+      options().hv_stencil_slabs = (int)value;
     } else if (n == "fp32_floor") {
       QS3X_REQUIRE(value > 0, "fp32_floor must be > 0");
       options().fp32_floor = value;

@@ -51,6 +51,11 @@ struct Options {
   // need the key search cluster at the end of the row order, and one range
   // per domain leaves them to the last one).  -1: 5.
   int hv_chunk = -1;
+  // Gap-lattice stencil H.v (nearest-neighbour chains, N_down = 4, uniform
+  // bonds): 1 = use it where it applies (default), 0 = never.
+  int hv_stencil = 1;
+  int hv_stencil_lines = 0;   // n3-lines per tile of the stencil kernel (0: per precision, 1..8)
+  int hv_stencil_slabs = 0;   // n4-slabs per stencil task (0: per precision)
   int verbose = 0;            // 1: one stderr line per restart / per 100 Lanczos steps
 };
 Options &options();

@@ -823,6 +823,14 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   // bond types; not the adjacency of the general chain kernels), up to what
   // a CU holds beside their 2 KB of static LDS.
   if (pick <= 0 && s->chain && s->mode == kIndexAnchored && !normlist && s->model->nn_chain && s->d_bitmap.p) {
+    int sg = 0;
+    size_t slds = 0;
+    if (launch_hv_stencil(s, prec, a, st, &sg, &slds)) {
+      if (nblocks_out) *nblocks_out = sg;
+      s->last_family = 5;
+      s->last_lds = (int64_t)slds;
+      return;
+    }
     const size_t lds = chain_nn_lds_bytes(s->nod, s->model->NOS, (int)s->model->bt_jxy2.size(), prec);
     const bool uniform = s->model->bt_jxy2.size() == 1;
     const bool pairs = (((uintptr_t)a.v0 | (uintptr_t)a.v1) & 15u) == 0;

@@ -142,5 +142,10 @@ using KFn = void (*)(HvArgs);
 // kernel may be used (real phases, N_down <= 4).
 hvk::KFn chain_nn_kernel(int N, int prec, bool realph, bool uniform, bool key32, bool pairs);
 size_t chain_nn_lds_bytes(int N, int L, int nbt, int prec);
+// Gap-lattice stencil H.v (hv_nn.hip): launches it and returns true where it
+// applies (nearest-neighbour chain, uniform bonds, N_down = 4, option
+// hv_stencil), with its grid and dynamic LDS; false leaves nothing launched.
+bool launch_hv_stencil(qs3x_sector *s, int prec, const hvk::HvArgs &a, hipStream_t st, int *grid_out,
+                       size_t *lds_out);
 
 }  // namespace qs3x

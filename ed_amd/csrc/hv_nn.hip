@@ -36,6 +36,9 @@
 // of a whole wavefront, and k_hv_chain_nn2 gives each lane two adjacent rows
 // so that its streams and most gathers move 16 bytes per lane.
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <vector>
 
 #include "blas.h"
 #include "hv_common.h"
@@ -729,6 +732,300 @@ __global__ __launch_bounds__(kThreads) void __attribute__((amdgpu_waves_per_eu(5
   if (a.dot_partials) block_reduce_store(dre, dim_, a.dot_partials);
 }
 
+// ---------------------------------------------------------------------------
+// Gap-lattice stencil H.v: nearest-neighbour chains with uniform bonds and
+// N_down = 4.  A row is the anchored representative (1, a, b, c) (a = n_2,
+// b = n_3, c = n_4); rows ascend in colex order, i.e. by (c, b, a), so the rows
+// of one "line" (b, c) are consecutive in a.  For a clear row (see the file
+// comment) every hop lands on a fixed neighbour of (a, b, c), a
+// representative of stabiliser order 1:
+//   spin a: (a +- 1, b, c)   spin b: (a, b +- 1, c)   spin c: (a, b, c +- 1)
+//   anchor to site 2: (a - 1, b - 1, c - 1), phase explist(L - 1)
+//   anchor to site L: (a + 1, b + 1, c + 1), phase explist(1)
+// so H.v is an 8-point stencil on the (a, b, c) lattice.  A task is a tile of
+// kSegA values of a (a0 ..) by B lines (b0 ..) swept over the slabs c0 .. c1 - 1.
+// A slab's window -- the tile plus a halo of one in a and b: B + 2 line
+// segments of kSegA + 2 = 64 positions, each one wavefront load of consecutive
+// rows -- goes into one slot of a four-slot LDS ring: slab c is computed from
+// slots c - 1, c, c + 1 while slab c + 2 is in flight in registers, with one
+// barrier per slab.  Every element of v1 is read about (B + 2) 64 / (B kSegA)
+// times instead of once per hop that reaches it, and no configuration word or
+// index is read.  Rows that are not clear go through the per-wavefront queue
+// and the key search of slow_row, as in the kernels above.
+constexpr int kSegA = 62;   // a-values per tile
+constexpr int kSegW = 64;   // window positions per line (halo of one each side)
+constexpr int kRing = 4;    // LDS slots
+constexpr int kMaxLines = 8;
+
+struct StencilArgs {
+  const int4 *lines;    // [(L + 1) * c + b]: first row, lowest a, highest a + 1 (lowest = highest + 1: empty)
+  const int4 *tasks;    // a0, b0, c0, c1 (c1 <= c0: nothing to do)
+  int ntasks;
+  int B;                // lines per tile
+};
+
+template <class R, bool REALPH>
+__global__ __launch_bounds__(kThreads) void k_hv_stencil4(HvArgs a, StencilArgs sa) {
+  using C = typename CT<R>::T;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int L = a.L;
+  const int B = sa.B, NL = B + 2;
+  // LDS: ring [kRing][NL][kSegW] elements | line entries of the task's slabs
+  // c0 - 1 .. c1: [slabs + 2][NL] int4
+  C *s_ring = (C *)smem;
+  int4 *s_tl = (int4 *)(smem + (size_t)kRing * NL * kSegW * sizeof(C));
+  const C *__restrict__ v1 = (const C *)a.v1;
+  C *__restrict__ v0 = (C *)a.v0;
+  const R scale = a.scale ? (R)(*a.scale) : (R)1;
+  const R jxy2 = (R)a.bt[0], jz4 = (R)a.bt[1];
+  const int ei_ar = (L - 1) + 3 * (L + 1), ei_al = 1 + 3 * (L + 1);
+  R ph_ar_r = (R)1, ph_ar_i = (R)0, ph_al_r = (R)1, ph_al_i = (R)0;
+  if constexpr (!REALPH) {
+    ph_ar_r = (R)a.explist[2 * ei_ar];
+    ph_ar_i = (R)a.explist[2 * ei_ar + 1];
+    ph_al_r = (R)a.explist[2 * ei_al];
+    ph_al_i = (R)a.explist[2 * ei_al + 1];
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double dre = 0.0, dim_ = 0.0;
+  const int W1 = L + 1;
+  constexpr int kPer = (kMaxLines + 2 + 3) / 4;  // window lines per wavefront
+
+  // Registers of slab windows: wavefront w holds lines w, w + 4, ...; lane =
+  // position a0 - 1 + lane.  Line entries come from LDS, so the loads issue at
+  // once.  Two windows are in flight (sets X and Y, alternating) while a slab
+  // is computed, so a block keeps about two slabs of loads outstanding.
+  C preX[kPer], preY[kPer];
+  auto fetch = [&](C (&pre)[kPer], int a0, const int4 *te) __attribute__((always_inline)) {
+#pragma unroll
+    for (int m = 0; m < kPer; ++m) {
+      const int j = wid + 4 * m;
+      C v;
+      v.x = v.y = (R)0;
+      if (j < NL) {
+        const int4 e = te[j];
+        const int av = a0 - 1 + lane;
+        if (av >= e.y && av < e.z) v = v1[e.x + (av - e.y)];
+      }
+      pre[m] = v;
+    }
+  };
+  auto store = [&](const C (&pre)[kPer], int sl) __attribute__((always_inline)) {
+#pragma unroll
+    for (int m = 0; m < kPer; ++m) {
+      const int j = wid + 4 * m;
+      if (j < NL) s_ring[((size_t)sl * NL + j) * kSegW + lane] = pre[m];
+    }
+  };
+  // slab c from slots c - 1, c, c + 1 of the ring
+  auto compute = [&](int a0, int b0, int c, const int4 *te0) __attribute__((always_inline)) {
+    const int s0 = c % kRing, sm = (c + kRing - 1) % kRing, sp = (c + 1) % kRing;
+    const C *r0 = s_ring + (size_t)s0 * NL * kSegW, *rm = s_ring + (size_t)sm * NL * kSegW,
+            *rp = s_ring + (size_t)sp * NL * kSegW;
+    const int W = L + 1 - c;
+    const int npos = B * kSegA;
+    for (int pb = 0; pb < npos; pb += kThreads) {
+      const int q = pb + (int)threadIdx.x;
+      const int j = 1 + q / kSegA, ai = q % kSegA;  // window line, a - a0
+      if (q < npos) {
+        const int4 e = te0[j];
+        const int av = a0 + ai;
+        const int b = b0 - 1 + j;
+        const int g1 = av - 1, g2 = b - av, g3 = c - b;
+        // clear rows only: the others are in the sector's slow-row list
+        if (av >= e.y && av < e.z && max(g1, max(g2, g3)) <= W - 3) {
+          const int r = e.x + (av - e.y);
+          const int x0 = j * kSegW + ai + 1;  // window index of (a, b) in a slot
+          C x = r0[x0];
+          R sx = 0, sy = 0, arx = 0, ary = 0, alx, aly;
+          int nanti = 2;  // spin c to the right and the anchor to site L: W > 1
+          C y = rp[x0];   // (a, b, c + 1)
+          sx += y.x;
+          sy += y.y;
+          y = rp[x0 + kSegW + 1];  // anchor to site L: (a + 1, b + 1, c + 1)
+          alx = y.x;
+          aly = y.y;
+          if (g2 > 1) {
+            y = r0[x0 + 1];  // (a + 1, b, c)
+            sx += y.x;
+            sy += y.y;
+            y = r0[x0 - kSegW];  // (a, b - 1, c)
+            sx += y.x;
+            sy += y.y;
+            nanti += 2;
+          }
+          if (g1 > 1) {
+            y = r0[x0 - 1];  // (a - 1, b, c)
+            sx += y.x;
+            sy += y.y;
+            y = rm[x0 - kSegW - 1];  // anchor to site 2: (a - 1, b - 1, c - 1)
+            arx = y.x;
+            ary = y.y;
+            nanti += 2;
+          }
+          if (g3 > 1) {
+            y = r0[x0 + kSegW];  // (a, b + 1, c)
+            sx += y.x;
+            sy += y.y;
+            y = rm[x0];  // (a, b, c - 1)
+            sx += y.x;
+            sy += y.y;
+            nanti += 2;
+          }
+          if constexpr (REALPH) {
+            sx += arx + alx;
+            sy += ary + aly;
+          } else {
+            sx += ph_ar_r * arx - ph_ar_i * ary + ph_al_r * alx - ph_al_i * aly;
+            sy += ph_ar_r * ary + ph_ar_i * arx + ph_al_r * aly + ph_al_i * alx;
+          }
+          x.x *= scale;
+          x.y *= scale;
+          const R dg = (sizeof(R) == 8 ? (R)a.ezz : (R)0) - (R)2 * jz4 * (R)nanti;
+          C out;
+          out.x = dg * x.x + jxy2 * scale * sx;
+          out.y = dg * x.y + jxy2 * scale * sy;
+          st_nt(&v0[r], out);
+          if (a.dot_partials) {
+            dre += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
+            dim_ += (double)x.x * (double)out.y - (double)x.y * (double)out.x;
+          }
+        }
+      }
+    }
+  };
+
+  __syncthreads();
+  for (int t = blockIdx.x; t < sa.ntasks; t += gridDim.x) {
+    const int4 task = sa.tasks[t];
+    const int a0 = task.x, b0 = task.y, c0 = task.z, c1 = task.w;
+    if (c1 <= c0) continue;
+    // line entries of slabs c0 - 1 .. c1 (entry (cs, j) at s_tl[(cs - c0 + 1) NL + j])
+    const int nent = (c1 - c0 + 2) * NL;
+    for (int q = threadIdx.x; q < nent; q += kThreads) {
+      const int cs = c0 - 1 + q / NL, b = b0 - 1 + q % NL;
+      int4 e = make_int4(0, 0, 0, 0);
+      if (cs >= 0 && cs <= L && b >= 0 && b <= L) e = sa.lines[(size_t)cs * W1 + b];
+      s_tl[q] = e;
+    }
+    __syncthreads();
+    auto te = [&](int cs) { return s_tl + (cs - c0 + 1) * NL; };
+    // prologue: slabs c0 - 1, c0, c0 + 1 loaded together into their slots,
+    // then c0 + 2 in flight (X)
+    {
+      C preZ[kPer];
+      fetch(preX, a0, te(c0 - 1));
+      fetch(preY, a0, te(c0));
+      fetch(preZ, a0, te(c0 + 1));
+      store(preX, (c0 + kRing - 1) % kRing);
+      store(preY, c0 % kRing);
+      store(preZ, (c0 + 1) % kRing);
+    }
+    if (c0 + 2 <= c1) fetch(preX, a0, te(c0 + 2));
+    __syncthreads();
+    // slab c: slab c + 3 goes in flight in one set while slab c + 2 (the other
+    // set) is stored after the slab is computed
+    for (int c = c0; c < c1; c += 2) {
+      if (c + 3 <= c1) fetch(preY, a0, te(c + 3));
+      compute(a0, b0, c, te(c));
+      if (c + 2 <= c1) store(preX, (c + 2) % kRing);
+      __syncthreads();
+      if (c + 1 < c1) {
+        if (c + 4 <= c1) fetch(preX, a0, te(c + 4));
+        compute(a0, b0, c + 1, te(c + 1));
+        if (c + 3 <= c1) store(preY, (c + 3) % kRing);
+        __syncthreads();
+      }
+    }
+  }
+  if (a.dot_partials) block_reduce_store(dre, dim_, a.dot_partials);
+}
+
+// Rows of an N_down = 4 stencil sector that are not clear (the stencil kernel
+// skips them): a count per chunk of kSlowChunk rows, then the rows in order.
+constexpr int kSlowChunk = 4096;
+__device__ __forceinline__ bool stencil_slow(uint64_t w, int bits, int L) {
+  const uint64_t m = (1ull << bits) - 1ull;
+  const int av = (int)((w >> bits) & m), b = (int)((w >> (2 * bits)) & m), c = (int)((w >> (3 * bits)) & m);
+  return max(av - 1, max(b - av, c - b)) > L + 1 - c - 3;
+}
+__global__ void k_slow_count(const uint64_t *__restrict__ cfg, int64_t dim, int bits, int L, int *counts) {
+  __shared__ int s_n;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  int n = 0;
+  const int64_t r0 = (int64_t)blockIdx.x * kSlowChunk;
+  for (int q = threadIdx.x; q < kSlowChunk; q += blockDim.x) {
+    const int64_t r = r0 + q;
+    if (r < dim && stencil_slow(cfg[r], bits, L)) ++n;
+  }
+  atomicAdd(&s_n, n);
+  __syncthreads();
+  if (threadIdx.x == 0) counts[blockIdx.x] = s_n;
+}
+// blockDim.x = 256; offsets: exclusive prefix of the counts
+__global__ void k_slow_write(const uint64_t *__restrict__ cfg, int64_t dim, int bits, int L, const int *offsets,
+                             int32_t *out) {
+  __shared__ int s_wave[4];
+  int base = offsets[blockIdx.x];
+  const int64_t r0 = (int64_t)blockIdx.x * kSlowChunk;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int q0 = 0; q0 < kSlowChunk; q0 += 256) {
+    const int64_t r = r0 + q0 + threadIdx.x;
+    const bool sl = r < dim && stencil_slow(cfg[r], bits, L);
+    const uint64_t mask = __ballot(sl);
+    const int before = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+    if (lane == 0) s_wave[wid] = __popcll(mask);
+    __syncthreads();
+    int wbase = base;
+    for (int w = 0; w < wid; ++w) wbase += s_wave[w];
+    if (sl) out[wbase + before] = (int32_t)r;
+    base += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+    __syncthreads();
+  }
+}
+// H.v of the listed rows through the key search of slow_row (one row per thread).
+template <class R, bool REALPH, class KeyT>
+__global__ __launch_bounds__(kThreads) void k_hv_slow_rows(HvArgs a, const int32_t *__restrict__ rows, int nrows,
+                                                           const int32_t *__restrict__ binom32) {
+  constexpr int N = 4;
+  __shared__ R s_tab[2 * (N + 1) + 2];
+  for (int q = threadIdx.x; q <= N; q += kThreads) {
+    s_tab[q] = (R)a.rclass[q];
+    s_tab[N + 1 + q] = (R)(a.rclass[1] / a.rclass[q]);
+  }
+  if (threadIdx.x == 0) {
+    s_tab[2 * (N + 1)] = (R)a.bt[0];
+    s_tab[2 * (N + 1) + 1] = (R)a.bt[1];
+  }
+  __syncthreads();
+  const NnCtx<N, R> k{a.L, a.kbits, binom32, s_tab, s_tab + (N + 1), s_tab + 2 * (N + 1), s_tab + 2 * (N + 1) + 1,
+                      nullptr};
+  const R scale = a.scale ? (R)(*a.scale) : (R)1;
+  double dre = 0.0, dim_ = 0.0;
+  for (int i = blockIdx.x * kThreads + threadIdx.x; i - (int)threadIdx.x < nrows; i += gridDim.x * kThreads) {
+    if (i < nrows) slow_row<N, R, REALPH, true, KeyT>(a, k, rows[i], scale, dre, dim_);
+  }
+  if (a.dot_partials) block_reduce_store(dre, dim_, a.dot_partials);
+}
+
+// Line table of an anchored N_down = 4 chain sector: for every row, the first
+// row of its line (b, c) writes (row, a) and the last row writes a + 1.
+__global__ void k_stencil_lines(const uint64_t *__restrict__ cfg, int64_t dim, int bits, int L, int4 *lines) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim) return;
+  const uint64_t m = (1ull << bits) - 1ull;
+  auto key = [&](uint64_t w) { return (int64_t)((w >> (3 * bits)) & m) * (L + 1) + (int64_t)((w >> (2 * bits)) & m); };
+  const uint64_t w = cfg[i];
+  const int64_t ln = key(w);
+  const int av = (int)((w >> bits) & m);
+  if (i == 0 || key(cfg[i - 1]) != ln) {
+    lines[ln].x = (int)i;
+    lines[ln].y = av;
+  }
+  if (i == dim - 1 || key(cfg[i + 1]) != ln) lines[ln].z = av + 1;
+}
+
 template <int N, bool UNI, class KeyT>
 KFn pick_nn3(int prec, bool realph, bool pairs) {
   if (prec == QS3X_FP64)
@@ -770,6 +1067,217 @@ size_t chain_nn_lds_bytes(int N, int L, int nbt, int prec) {
   const size_t rs = (prec == QS3X_FP64) ? 8 : 4;
   const size_t nb = (size_t)(N - 1) * (L - 1);
   return ((nb * 4 + 15) & ~(size_t)15) + 2 * rs * (N + 1) + 2 * rs * nbt + 2 * (size_t)(L + 1);
+}
+
+// ---------------------------------------------------------------- stencil
+namespace {
+
+using SFn = void (*)(HvArgs, StencilArgs);
+using SlowFn = void (*)(HvArgs, const int32_t *, int, const int32_t *);
+
+SFn stencil_kernel(int prec, bool realph) {
+  if (prec == QS3X_FP64) return realph ? k_hv_stencil4<double, true> : k_hv_stencil4<double, false>;
+  return realph ? k_hv_stencil4<float, true> : k_hv_stencil4<float, false>;
+}
+SlowFn slow_kernel(int prec, bool realph, bool key32) {
+  if (prec == QS3X_FP64) {
+    if (realph) return key32 ? k_hv_slow_rows<double, true, uint32_t> : k_hv_slow_rows<double, true, uint64_t>;
+    return key32 ? k_hv_slow_rows<double, false, uint32_t> : k_hv_slow_rows<double, false, uint64_t>;
+  }
+  if (realph) return key32 ? k_hv_slow_rows<float, true, uint32_t> : k_hv_slow_rows<float, true, uint64_t>;
+  return key32 ? k_hv_slow_rows<float, false, uint32_t> : k_hv_slow_rows<float, false, uint64_t>;
+}
+
+// Band width and task length: options hv_stencil_lines / hv_stencil_slabs,
+// or by default the fastest measured at L = 1000 (FP32: 8 lines, 8 slabs,
+// 0.223 ms; FP64: 4 lines, 16 slabs, 0.383 ms -- FP64 is limited by its
+// registers, 5 wavefronts per SIMD, so narrower bands keep more blocks
+// resident).
+int stencil_lines(int prec) {
+  if (options().hv_stencil_lines > 0) return std::min(options().hv_stencil_lines, kMaxLines);
+  return prec == QS3X_FP64 ? 4 : 8;
+}
+int stencil_slabs(int prec) {
+  if (options().hv_stencil_slabs > 0) return options().hv_stencil_slabs;
+  return prec == QS3X_FP64 ? 16 : 8;
+}
+
+// Plan for one precision: tiles of kSegA values of a by B lines, each swept
+// over the slabs where it holds rows, in tasks of up to `slabs` slabs.  Tasks
+// are dealt to the 8 XCDs by tile: task 8 m + x belongs to XCD x, whose tiles
+// are one contiguous range in (b, a) order holding about an eighth of the
+// tasks, so the halo lines and columns that neighbouring tiles share stay in
+// one L2.
+void stencil_plan(int prec, qs3x_sector::StencilPlan &pl, const std::vector<int4> &lines, int L) {
+  const int W1 = L + 1;
+  const int slabs = stencil_slabs(prec);
+  const int B = stencil_lines(prec);
+  const int na = (L + kSegA - 1) / kSegA + 1, nb = (L + B - 1) / B + 1;
+  std::vector<int> cmin((size_t)na * nb, 0x7fffffff), cmax((size_t)na * nb, -1);
+  for (int c = 0; c <= L; ++c)
+    for (int b = 3; b < c; ++b) {
+      const int4 e = lines[(size_t)c * W1 + b];
+      if (e.z <= e.y) continue;
+      const int tb = (b - 3) / B;
+      for (int ta = (e.y - 2) / kSegA; ta <= (e.z - 1 - 2) / kSegA; ++ta) {
+        const size_t t = (size_t)tb * na + ta;
+        cmin[t] = std::min(cmin[t], c);
+        cmax[t] = std::max(cmax[t], c);
+      }
+    }
+  std::vector<std::vector<int4>> per_tile;
+  size_t total = 0;
+  for (int tb = 0; tb < nb; ++tb)
+    for (int ta = 0; ta < na; ++ta) {
+      const size_t t = (size_t)tb * na + ta;
+      if (cmax[t] < cmin[t]) continue;
+      std::vector<int4> tl;
+      for (int c0 = cmin[t]; c0 <= cmax[t]; c0 += slabs)
+        tl.push_back(make_int4(2 + ta * kSegA, 3 + tb * B, c0, std::min(c0 + slabs, cmax[t] + 1)));
+      total += tl.size();
+      per_tile.push_back(tl);
+    }
+  const int nx = 8;
+  std::vector<std::vector<int4>> grp(nx);
+  {
+    // contiguous runs of tiles holding about total / nx tasks each
+    size_t acc = 0;
+    int x = 0;
+    std::vector<std::vector<const std::vector<int4> *>> gt(nx);
+    for (auto &tl : per_tile) {
+      if (x < nx - 1 && acc >= total * (size_t)(x + 1) / nx) ++x;
+      gt[x].push_back(&tl);
+      acc += tl.size();
+    }
+    for (int g = 0; g < nx; ++g) {
+      size_t depth = 0;
+      for (auto *tl : gt[g]) depth = std::max(depth, tl->size());
+      for (size_t d = 0; d < depth; ++d)
+        for (auto *tl : gt[g])
+          if (d < tl->size()) grp[g].push_back((*tl)[d]);
+    }
+  }
+  size_t mx = 0;
+  for (auto &g : grp) mx = std::max(mx, g.size());
+  std::vector<int4> tasks(mx * nx, make_int4(0, 0, 0, 0));
+  for (int x = 0; x < nx; ++x)
+    for (size_t m = 0; m < grp[x].size(); ++m) tasks[m * nx + x] = grp[x][m];
+  const size_t es = prec == QS3X_FP64 ? 16 : 8;
+  pl.B = B;
+  pl.slabs = slabs;
+  pl.lds = (size_t)kRing * (B + 2) * kSegW * es + (size_t)(slabs + 2) * (B + 2) * sizeof(int4);
+  pl.ntasks = (int)tasks.size();
+  pl.tasks.alloc(std::max<size_t>(tasks.size(), 1));
+  if (!tasks.empty()) pl.tasks.upload(tasks.data(), tasks.size());
+  pl.grid = 0;  // at the first launch
+}
+
+// The rows the stencil kernel skips, in row order (deterministic).
+void stencil_slow_list(qs3x_sector *s) {
+  const int L = s->model->NOS;
+  const int nch = (int)((s->dim + kSlowChunk - 1) / kSlowChunk);
+  DevBuf<int> counts(std::max(nch, 1)), offs(std::max(nch, 1));
+  hipStream_t st = stream();
+  hipLaunchKernelGGL(k_slow_count, dim3(nch), dim3(256), 0, st, s->d_cfg.p, s->dim, s->bits, L, counts.p);
+  QS3X_HIP(hipGetLastError());
+  std::vector<int> hc(nch), ho(nch);
+  counts.download(hc.data(), nch);
+  int64_t tot = 0;
+  for (int i = 0; i < nch; ++i) {
+    ho[i] = (int)tot;
+    tot += hc[i];
+  }
+  s->nslow = tot;
+  s->d_slow.alloc(std::max<int64_t>(tot, 1));
+  offs.upload(ho.data(), nch);
+  hipLaunchKernelGGL(k_slow_write, dim3(nch), dim3(256), 0, st, s->d_cfg.p, s->dim, s->bits, L, offs.p, s->d_slow.p);
+  QS3X_HIP(hipGetLastError());
+  QS3X_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+bool launch_hv_stencil(qs3x_sector *s, int prec, const HvArgs &a, hipStream_t st, int *grid_out, size_t *lds_out) {
+  if (!options().hv_stencil || s->nod != 4 || !s->chain || s->mode != kIndexAnchored || !s->model->nn_chain ||
+      s->model->bt_jxy2.size() != 1 || s->d_rclass.p == nullptr || s->dim >= ((int64_t)1 << 31) ||
+      binom_host(s->model->NOS - 2, 3) >= ((int64_t)1 << 31))
+    return false;
+  const int L = s->model->NOS;
+  qs3x_sector::StencilPlan &pl = s->stencil[prec == QS3X_FP64 ? 0 : 1];
+  if (pl.tried && (pl.slabs != stencil_slabs(prec) || pl.B != stencil_lines(prec))) pl.tried = false;  // options changed
+  if (!pl.tried) {
+    pl.tried = true;
+    pl.ok = false;
+    if (!s->d_lines.p) {
+      const size_t n = (size_t)(L + 1) * (L + 1);
+      s->d_lines.alloc(n);
+      QS3X_HIP(hipMemsetAsync(s->d_lines.p, 0, n * sizeof(int4), stream()));
+      hipLaunchKernelGGL(k_stencil_lines, dim3((unsigned)((s->dim + 255) / 256)), dim3(256), 0, stream(),
+                         s->d_cfg.p, s->dim, s->bits, L, s->d_lines.p);
+      QS3X_HIP(hipGetLastError());
+      std::vector<int32_t> b32((size_t)3 * (L - 1));
+      for (int q = 1; q <= 3; ++q)
+        for (int x = 0; x < L - 1; ++x) b32[(size_t)(q - 1) * (L - 1) + x] = (int32_t)binom_host(x, q);
+      s->d_binom32.alloc(b32.size());
+      s->d_binom32.upload(b32.data(), b32.size());
+      stencil_slow_list(s);
+    }
+    std::vector<int4> lines((size_t)(L + 1) * (L + 1));
+    s->d_lines.download(lines.data(), lines.size());
+    stencil_plan(prec, pl, lines, L);
+    // the plan's tables went up on the library stream; the launch below may
+    // be on the caller's stream, which nothing orders after it
+    QS3X_HIP(hipStreamSynchronize(stream()));
+    pl.ok = true;
+  }
+  if (!pl.ok) return false;
+  int kb = 1;
+  while ((1 << kb) <= L) ++kb;
+  SFn f = stencil_kernel(prec, s->real_phase);
+  SlowFn fs = slow_kernel(prec, s->real_phase, 3 * kb <= 32);
+  if (pl.lds > 64 * 1024) {
+    static std::mutex mu;
+    static std::map<SFn, size_t> done;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = done.find(f);
+    if (it == done.end() || it->second < pl.lds) {
+      QS3X_HIP(hipFuncSetAttribute((const void *)f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)pl.lds));
+      done[f] = pl.lds;
+    }
+  }
+  const int slow_grid = s->nslow > 0 ? (int)std::min<int64_t>((s->nslow + kThreads - 1) / kThreads, 512) : 0;
+  if (pl.grid == 0) {
+    // resident blocks per CU, from the kernel's registers and LDS (160 KB per
+    // CU, 512 VGPRs per lane slot on each of 4 SIMDs, at most 8 wavefronts
+    // per SIMD), a multiple of the 8 XCDs so that block g (on XCD g % 8) takes
+    // only tasks 8 m + g % 8, its XCD's tiles; with the slow-row grid within
+    // the callers' partial-sum buffers
+    int dev = 0, cus = 0;
+    QS3X_HIP(hipGetDevice(&dev));
+    QS3X_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    hipFuncAttributes at{};
+    QS3X_HIP(hipFuncGetAttributes(&at, (const void *)f));
+    const int vg = std::max(at.numRegs, 1);
+    const int waves_simd = std::min(8, 512 / ((vg + 7) & ~7));
+    const int by_regs = std::max(1, waves_simd * 4 / (kThreads / 64));
+    const int by_lds = std::max(1, (int)((160 * 1024) / (pl.lds + at.sharedSizeBytes)));
+    const int per_cu = std::min(by_regs, by_lds);
+    int grid = std::min<int64_t>(pl.ntasks, (int64_t)cus * per_cu);
+    grid = std::min(grid, kMaxBlocks - 512);
+    pl.grid = std::max(8, (grid / 8) * 8);
+  }
+  StencilArgs sa{s->d_lines.p, pl.tasks.p, pl.ntasks, pl.B};
+  hipLaunchKernelGGL(f, dim3(pl.grid), dim3(kThreads), pl.lds, st, a, sa);
+  QS3X_HIP(hipGetLastError());
+  if (slow_grid > 0) {
+    HvArgs b = a;
+    if (b.dot_partials) b.dot_partials += 2 * (size_t)pl.grid;
+    hipLaunchKernelGGL(fs, dim3(slow_grid), dim3(kThreads), 0, st, b, s->d_slow.p, (int)s->nslow, s->d_binom32.p);
+    QS3X_HIP(hipGetLastError());
+  }
+  if (grid_out) *grid_out = pl.grid + slow_grid;
+  if (lds_out) *lds_out = pl.lds;
+  return true;
 }
 
 }  // namespace qs3x

@@ -143,6 +143,19 @@ struct qs3x_sector {
   // its dynamic LDS bytes (qs3x_hv_last_launch)
   int last_family = -1;
   int64_t last_lds = 0;
+  // Gap-lattice stencil H.v (hv_nn.hip): line table over (n3, n4) of the
+  // anchored rows (int4: first row, lowest n2, highest n2 + 1, 0) and, per
+  // precision, the task list of its launch plan.
+  struct StencilPlan {
+    bool tried = false, ok = false;
+    int B = 0, slabs = 0, ntasks = 0, grid = 0;
+    size_t lds = 0;
+    qs3x::DevBuf<int4> tasks;
+  } stencil[2];
+  qs3x::DevBuf<int4> d_lines;
+  qs3x::DevBuf<int32_t> d_binom32;  // C(x, q) at [(q - 1)(L - 1) + x], q = 1..3
+  qs3x::DevBuf<int32_t> d_slow;     // rows the stencil kernel skips (not clear), ascending
+  int64_t nslow = 0;
 };
 
 namespace qs3x {

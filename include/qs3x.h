@@ -82,6 +82,15 @@ int qs3x_set_device(int32_t device);
  *                   small sectors, so that every domain owns at least 8).
  *                   -1, the default: 5 (the fastest at L = 1000, N_down = 4
  *                   in both precisions).
+ *   "hv_stencil"    1 (default): nearest-neighbour chains with uniform bonds
+ *                   and N_down = 4 use the gap-lattice stencil H.v (tiles of
+ *                   (n2, n3) swept over n4 through LDS, every hop of a clear
+ *                   row read from there, the other rows in a second launch
+ *                   through the key search); 0: the nearest-neighbour chain
+ *                   kernel.  "hv_stencil_lines" (1..8; 0, the default: 8 in
+ *                   FP32, 4 in FP64) and "hv_stencil_slabs" (0, the default:
+ *                   8 in FP32, 16 in FP64) set its tile height and the
+ *                   n4-slabs per task.
  *   "verbose"       0 (default); 1: a stderr line per thick restart and per
  *                   100 Lanczos steps; 2: per convergence check.
  * The environment variable QS3X_OPTIONS="name=value,name=value" applies the
@@ -282,7 +291,8 @@ int qs3x_continued_fraction(int32_t nlev, const double *alpha, const double *bet
 /* ---------------------------------------------------------- measurement */
 /* The kernel family of the sector's last H . v launch (no counterpart in the
  * reference): 0 nearest-neighbour chain, 1 batched-load chain, 2 per-hop
- * chain, 3 general lattice, 4 U(1)-only; -1 before the first launch.
+ * chain, 3 general lattice, 4 U(1)-only, 5 gap-lattice stencil; -1 before
+ * the first launch.
  * lds_bytes: its dynamic LDS. */
 int qs3x_hv_last_launch(const qs3x_sector *s, int32_t *family, int64_t *lds_bytes);
 /* Times n_rep device H . v applications (events on the library stream) and

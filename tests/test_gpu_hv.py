@@ -348,3 +348,86 @@ def test_hv_long_j1j2_chain_matches_general(K):
     for kern in (1, 2):
         assert np.max(np.abs(out[(kern, 64)] - ref)) <= 1e-12 * np.max(np.abs(ref)), kern
         assert np.max(np.abs(out[(kern, 32)] - ref32)) <= 2e-6 * np.max(np.abs(ref32)), kern
+
+
+STENCIL_CASES = [
+    # L, K, Jz, hv_stencil_lines, hv_stencil_slabs
+    (40, 0, 0.8, 0, 16),
+    (40, 11, 0.8, 3, 5),
+    (64, 3, 1.0, 1, 1),
+    (64, 0, 1.0, 8, 16),
+    (128, 0, 1.0, 0, 0),
+    (128, 17, 0.6, 4, 7),
+]
+
+
+@pytest.mark.parametrize("L,K,jz,B,slabs", STENCIL_CASES)
+def test_hv_stencil_parity(L, K, jz, B, slabs):
+    """Gap-lattice stencil H.v (nearest-neighbour chains, uniform bonds,
+    N_down = 4): FP64 within 1e-12 of the oracle's ham_to_vec_wave_vector
+    (ham2vec_v3.f90:37-72), FP32 (H - e_zz) v within 2e-6, for several band
+    widths and task lengths, with 16-byte aligned and unaligned FP32 vectors;
+    the stencil kernel is the one that ran."""
+    print(f"[gpu] hv stencil L={L} K={K} B={B} slabs={slabs}", flush=True)
+    q3 = _q3()
+    bonds = orc.chain_bonds(L, [(1, 1.0, jz)])
+    om = orc.Model(L, L, 1, 1, bonds)
+    rk = om.rk_from_K(K)
+    osec = om.build_sector(4, rk)
+    gm = q3.Model(L, L, 1, 1, bonds)
+    q3.set_option("hv_stencil_lines", B)
+    q3.set_option("hv_stencil_slabs", slabs)
+    try:
+        s = q3.Sector.build(gm, 4, rk)
+        rng = np.random.default_rng(L + K)
+        v = rng.uniform(-1, 1, s.dim) + 1j * rng.uniform(-1, 1, s.dim)
+        ref = om.hv(osec, v)
+        got = s.ham_to_vec_wave_vector(v)
+        assert s.hv_last_launch()[0] == 5
+        assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+        ref32 = ref - om.fm_energy() * v
+        for shift in (0, 1):  # element offset 1: v1 and v0 only 8-byte aligned
+            buf = torch.from_numpy(np.concatenate([np.zeros(shift), v]).astype(np.complex64)).cuda()
+            dv = buf[shift:]
+            ybuf = torch.full((s.dim + shift,), float("nan"), dtype=torch.complex64, device="cuda")
+            y = ybuf[shift:]
+            s.hv_dev(y.data_ptr(), dv.data_ptr(), q3.FP32)
+            torch.cuda.synchronize()
+            assert s.hv_last_launch()[0] == 5
+            got32 = y.cpu().numpy().astype(np.complex128)
+            assert np.max(np.abs(got32 - ref32)) <= 2e-6 * np.max(np.abs(ref32)), shift
+    finally:
+        q3.set_option("hv_stencil_lines", 0)
+        q3.set_option("hv_stencil_slabs", 0)
+
+
+def test_hv_stencil_matches_nn_kernel_c3():
+    """At L = 1000, N_down = 4 (41,417,250 rows) the stencil H.v and the
+    nearest-neighbour chain kernel give the same result (FP64 1e-13 of the
+    largest element, FP32 2e-6), k = 0 and k = 2 pi 7 / 1000."""
+    q3 = _q3()
+    L = 1000
+    gm = q3.Model(L, L, 1, 1, orc.chain_bonds(L, [(1, 1.0, 1.0)]))
+    for K in (0, 7):
+        print(f"[gpu] hv stencil vs nn kernel L=1000 K={K}", flush=True)
+        s = q3.Sector.build(gm, 4, (q3.wavevector(K, L), 0.0, 0.0))
+        g = torch.Generator(device="cuda")
+        g.manual_seed(5 + K)
+        for dt, tol in ((torch.complex128, 1e-13), (torch.complex64, 2e-6)):
+            prec = q3.FP64 if dt == torch.complex128 else q3.FP32
+            x = torch.randn(s.dim, dtype=dt, device="cuda", generator=g)
+            y1 = torch.empty_like(x)
+            y0 = torch.empty_like(x)
+            s.hv_dev(y1.data_ptr(), x.data_ptr(), prec)
+            assert s.hv_last_launch()[0] == 5
+            q3.set_option("hv_stencil", 0)
+            try:
+                s.hv_dev(y0.data_ptr(), x.data_ptr(), prec)
+                assert s.hv_last_launch()[0] == 0
+            finally:
+                q3.set_option("hv_stencil", 1)
+            torch.cuda.synchronize()
+            err = ((y1 - y0).abs().max() / y0.abs().max()).item()
+            assert err <= tol, (K, dt, err)
+            del x, y0, y1
+        del s
