@@ -111,6 +111,9 @@ int qs3x_set_option(const char *name, double value) {
     } else if (n == "hv_stencil_slabs") {
       QS3X_REQUIRE(value >= 0 && value <= 1024, "hv_stencil_slabs must be 0..1024");
       options().hv_stencil_slabs = (int)value;
+    } else if (n == "hv_stencil_side") {
+      QS3X_REQUIRE(value == 0 || value == 1, "hv_stencil_side must be 0 or 1");
+      options().hv_stencil_side = (int)value;
     } else if (n == "fp32_floor") {
       QS3X_REQUIRE(value > 0, "fp32_floor must be > 0");
       options().fp32_floor = value;
@@ -376,6 +379,21 @@ int qs3x_hv_last_launch(const qs3x_sector *s, int32_t *family, int64_t *lds_byte
     QS3X_REQUIRE(s && family && lds_bytes, "qs3x_hv_last_launch: NULL argument");
     *family = s->last_family;
     *lds_bytes = s->last_lds;
+  });
+}
+
+int qs3x_hv_stencil_info(const qs3x_sector *s, int32_t prec, int64_t *info) {
+  return guarded([&] {
+    QS3X_REQUIRE(s && info, "qs3x_hv_stencil_info: NULL argument");
+    QS3X_REQUIRE(prec == QS3X_FP64 || prec == QS3X_FP32, "qs3x_hv_stencil_info: bad precision");
+    const qs3x_sector::StencilPlan &pl = s->stencil[prec == QS3X_FP64 ? 0 : 1];
+    QS3X_REQUIRE(pl.tried && pl.ok, "qs3x_hv_stencil_info: no stencil launch on this sector and precision yet");
+    info[0] = s->nslow;
+    info[1] = pl.nlines;
+    info[2] = pl.ntasks;
+    info[3] = pl.B;
+    info[4] = pl.slabs;
+    info[5] = pl.grid;
   });
 }
 

@@ -32,7 +32,7 @@ int vec_maxpy(int prec, const void *const *V, const double *scales, const int *s
 // Tunables (qs3x_set_option): thick-restart reorthogonalisation and the FP32 phase.
 struct Options {
   int tr_reorth = 0;          // 0: every previous basis vector; 1: locked Ritz vectors + last two;
-                              // 2: as 1 for the first step of a restart cycle, three-term after it
+                              // 2: the three-term recurrence in the first cycle (with tr_check_every), then as 1
   int tr_cgs_passes = 2;      // classical Gram-Schmidt passes per step
   double fp32_floor = 1e-6;   // |delta theta| that ends the FP32 phase of the mixed thick restart
   int fp64_refine = 4;         // FP32-mixed thick restart: block Krylov steps of the final FP64 refinement (0: none)
@@ -56,6 +56,7 @@ struct Options {
   int hv_stencil = 1;
   int hv_stencil_lines = 0;   // n3-lines per tile of the stencil kernel (0: per precision, 1..8)
   int hv_stencil_slabs = 0;   // n4-slabs per stencil task (0: per precision)
+  int hv_stencil_side = 1;    // stencil: key-search rows on a side stream (1) or the caller's stream (0)
   int verbose = 0;            // 1: one stderr line per restart / per 100 Lanczos steps
 };
 Options &options();

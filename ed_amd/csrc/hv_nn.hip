@@ -1163,6 +1163,8 @@ void stencil_plan(int prec, qs3x_sector::StencilPlan &pl, const std::vector<int4
   for (int x = 0; x < nx; ++x)
     for (size_t m = 0; m < grp[x].size(); ++m) tasks[m * nx + x] = grp[x][m];
   const size_t es = prec == QS3X_FP64 ? 16 : 8;
+  pl.nlines = 0;
+  for (const int4 &e : lines) pl.nlines += e.z > e.y ? 1 : 0;
   pl.B = B;
   pl.slabs = slabs;
   pl.lds = (size_t)kRing * (B + 2) * kSegW * es + (size_t)(slabs + 2) * (B + 2) * sizeof(int4);
@@ -1267,12 +1269,39 @@ bool launch_hv_stencil(qs3x_sector *s, int prec, const HvArgs &a, hipStream_t st
     pl.grid = std::max(8, (grid / 8) * 8);
   }
   StencilArgs sa{s->d_lines.p, pl.tasks.p, pl.ntasks, pl.B};
-  hipLaunchKernelGGL(f, dim3(pl.grid), dim3(kThreads), pl.lds, st, a, sa);
-  QS3X_HIP(hipGetLastError());
-  if (slow_grid > 0) {
+  // The slow rows (a latency-bound tail of about 1 % of the rows) run on a
+  // side stream, forked from and joined back into st, so they overlap the
+  // stencil launch; the two write disjoint rows and partial sums.
+  if (slow_grid > 0 && !options().hv_stencil_side) {
     HvArgs b = a;
     if (b.dot_partials) b.dot_partials += 2 * (size_t)pl.grid;
+    hipLaunchKernelGGL(f, dim3(pl.grid), dim3(kThreads), pl.lds, st, a, sa);
+    QS3X_HIP(hipGetLastError());
     hipLaunchKernelGGL(fs, dim3(slow_grid), dim3(kThreads), 0, st, b, s->d_slow.p, (int)s->nslow, s->d_binom32.p);
+    QS3X_HIP(hipGetLastError());
+  } else if (slow_grid > 0) {
+    static hipStream_t side[64] = {};
+    static hipEvent_t fork[64] = {}, join[64] = {};
+    const int dev = device();
+    QS3X_REQUIRE(dev >= 0 && dev < 64, "launch_hv_stencil: device ordinal out of range");
+    if (!side[dev]) {
+      QS3X_HIP(hipStreamCreateWithFlags(&side[dev], hipStreamNonBlocking));
+      QS3X_HIP(hipEventCreateWithFlags(&fork[dev], hipEventDisableTiming));
+      QS3X_HIP(hipEventCreateWithFlags(&join[dev], hipEventDisableTiming));
+    }
+    QS3X_HIP(hipEventRecord(fork[dev], st));
+    QS3X_HIP(hipStreamWaitEvent(side[dev], fork[dev], 0));
+    HvArgs b = a;
+    if (b.dot_partials) b.dot_partials += 2 * (size_t)pl.grid;
+    hipLaunchKernelGGL(fs, dim3(slow_grid), dim3(kThreads), 0, side[dev], b, s->d_slow.p, (int)s->nslow,
+                       s->d_binom32.p);
+    QS3X_HIP(hipGetLastError());
+    QS3X_HIP(hipEventRecord(join[dev], side[dev]));
+    hipLaunchKernelGGL(f, dim3(pl.grid), dim3(kThreads), pl.lds, st, a, sa);
+    QS3X_HIP(hipGetLastError());
+    QS3X_HIP(hipStreamWaitEvent(st, join[dev], 0));
+  } else {
+    hipLaunchKernelGGL(f, dim3(pl.grid), dim3(kThreads), pl.lds, st, a, sa);
     QS3X_HIP(hipGetLastError());
   }
   if (grid_out) *grid_out = pl.grid + slow_grid;

@@ -149,6 +149,7 @@ struct qs3x_sector {
   struct StencilPlan {
     bool tried = false, ok = false;
     int B = 0, slabs = 0, ntasks = 0, grid = 0;
+    int64_t nlines = 0;  // non-empty (n3, n4) lines
     size_t lds = 0;
     qs3x::DevBuf<int4> tasks;
   } stencil[2];

@@ -335,7 +335,11 @@ struct TR {
   }
 
   // One Lanczos step from slot i into slot i+1, then classical Gram-Schmidt
-  // (cgs_passes times, FP64 coefficients) against the step's slot set.
+  // (cgs_passes times, FP64 coefficients) against the step's slot set.  In
+  // restart cycles (phase 1) at least two passes: the new vector must stay
+  // orthogonal to the converged locked Ritz vectors, which one pass does not
+  // keep over many restarts (measured: FP64 at L = 1000, NOM 128 ended far
+  // outside the spectrum with one pass, at the ground state with two).
   void step(int i, int phase) {
     int nb = 0;
     launch_hv(s, prec, slot(i + 1), slot(i), sc.p + S() + i, part.p, &nb, st);
@@ -345,7 +349,8 @@ struct TR {
     const int m = cnt[row];
     const void *const *P = (const void *const *)(ptab.p + row * (NOM + 1));
     const int *X = stab.p + row * (NOM + 1);
-    for (int pass = 0; pass < cgs_passes; ++pass) {
+    const int passes = phase == 1 ? std::max(2, cgs_passes) : cgs_passes;
+    for (int pass = 0; pass < passes; ++pass) {
       vec_mdot(prec, P, sc.p + S(), X, m, slot(i + 1), dim, part.p, h.p, st);
       nb = vec_maxpy(prec, P, sc.p + S(), X, m, h.p, slot(i + 1), dim, part.p, st);
     }
@@ -493,7 +498,7 @@ TRResult tr_cycles(TR &t, int64_t maxitr, double tol, double abs_floor, std::vec
 // the NOK lowest Ritz pairs are rotated into slots 0..NOK-1 as in tr_cycles.
 // In the first cycle, with tr_reorth >= 1, each step is the three-term
 // recurrence (no Gram-Schmidt); restart cycles keep the locked + local CGS of
-// step() (tr_reorth 1) or use it only for their first step (tr_reorth 2).
+// step().
 TRResult tr_cycles_checked(TR &t, int64_t maxitr, double tol, double abs_floor, int itrint,
                            std::vector<double> &theta, std::vector<double> &resid, int nwant) {
   const int NOK = t.NOK, NOM = t.NOM;
@@ -574,9 +579,13 @@ TRResult tr_cycles_checked(TR &t, int64_t maxitr, double tol, double abs_floor, 
       return stop;
     };
     for (int i = i0; i < NOM; ++i) {
-      // tr_reorth 1: three-term first cycle; 2: also every restart-cycle step
-      // after the first (the only one coupled to the locked vectors in exact arithmetic)
-      if ((k0 == 0 && t.reorth >= 1) || (t.reorth == 2 && i > k0)) t.step3(i);
+      // tr_reorth >= 1: the first cycle is the three-term recurrence; restart
+      // cycles orthogonalise every step against the locked Ritz vectors and
+      // the last two Lanczos vectors.  (Three-term steps in restart cycles
+      // lose orthogonality to the converged locked vectors over many
+      // restarts, and the arrowhead projection, which assumes it, then gives
+      // Ritz values below the spectrum: measured at L = 1000, NOM 128.)
+      if (k0 == 0 && t.reorth >= 1) t.step3(i);
       else t.step(i, k0 == 0 ? 0 : 1);
       const int m = i + 1;
       if ((m - i0) % itrint != 0 || m < NOK + 1) continue;

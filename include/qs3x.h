@@ -45,9 +45,10 @@ int qs3x_set_device(int32_t device);
  *                   Lanczos vector against all previous basis vectors, as
  *                   eigen_solver.f90:233-239 does; 1: against the locked Ritz
  *                   vectors and the last two Lanczos vectors only; 2 (with
- *                   tr_check_every > 0): as 1 for the first step of a restart
- *                   cycle, the three-term recurrence for the others.
- *   "tr_cgs_passes" classical Gram-Schmidt passes per step (default 2).
+ *                   tr_check_every > 0): the three-term recurrence in the
+ *                   first cycle, then as 1.
+ *   "tr_cgs_passes" classical Gram-Schmidt passes per step (default 2; steps
+ *                   of restart cycles take at least 2).
  *   "fp32_floor"    Ritz-value change (absolute) that ends the FP32 phase of
  *                   the mixed thick restart (default 1e-6).
  *   "hv_kernel"     0 (default): the fastest applicable H.v kernel (nearest-
@@ -295,6 +296,11 @@ int qs3x_continued_fraction(int32_t nlev, const double *alpha, const double *bet
  * the first launch.
  * lds_bytes: its dynamic LDS. */
 int qs3x_hv_last_launch(const qs3x_sector *s, int32_t *family, int64_t *lds_bytes);
+/* The stencil H . v's plan for this sector and precision, after its first
+ * launch (no counterpart in the reference): info (6 int64): rows left to the
+ * key search, non-empty (n3, n4) lines, tasks, tile height in lines, n4-slabs
+ * per task, workgroups. */
+int qs3x_hv_stencil_info(const qs3x_sector *s, int32_t prec, int64_t *info);
 /* Times n_rep device H . v applications (events on the library stream) and
  * returns the mean milliseconds per application. */
 int qs3x_time_hv(qs3x_sector *s, int32_t prec, int32_t n_rep, double *ms);

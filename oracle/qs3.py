@@ -649,6 +649,21 @@ class RefLib:
         self.L.ref_hv(_ptr(v0), _ptr(v1))
         return v0
 
+    def set_lists(self, list_s, list_r):
+        """Install lists built elsewhere as the module's list_s / list_r."""
+        self._lists = (np.ascontiguousarray(list_s, dtype=np.int64), np.ascontiguousarray(list_r, dtype=np.float64))
+        self.L.ref_set_lists(i64(len(self._lists[0])), _ptr(self._lists[0]), _ptr(self._lists[1]))
+        self.THS = len(self._lists[0])
+
+    def hv_rows(self, v, r0, r1, out=None):
+        """Rows r0..r1-1 (0-based) of ham_to_vec_wave_vector's loop
+        (ham2vec_v3.f90:52-70) with the reference's own routines; the other
+        rows of the returned vector are left as they were in out."""
+        v1 = np.ascontiguousarray(v, dtype=np.complex128)
+        v0 = np.zeros_like(v1) if out is None else out
+        self.L.ref_hv_rows(i64(r0 + 1), i64(r1), _ptr(v0), _ptr(v1))
+        return v0
+
     def representative(self, n):
         a = np.ascontiguousarray(np.asarray(n, dtype=np.int64)).copy()
         t = np.zeros(1, dtype=np.int64)

@@ -77,6 +77,54 @@ contains
     call ham_to_vec_wave_vector(v0, v1, THS, NOD, list_s, list_r, explist)
   end subroutine ref_hv
 
+  ! Lists built elsewhere (the reference's own allocate_lists_omp_SQ takes hours
+  ! at L = 1000): THS rows of list_s / list_r into the module globals.
+  subroutine ref_set_lists(ths_, ls, lr) bind(C, name="ref_set_lists")
+    integer(c_int64_t), value :: ths_
+    integer(c_int64_t), intent(in) :: ls(ths_)
+    real(c_double), intent(in) :: lr(ths_)
+    if (allocated(list_s)) deallocate(list_s)
+    if (allocated(list_r)) deallocate(list_r)
+    THS = ths_
+    allocate(list_s(THS), list_r(THS))
+    list_s = ls
+    list_r = lr
+  end subroutine ref_set_lists
+
+  ! Rows i0..i1 (1-based) of ham_to_vec_wave_vector (ham2vec_v3.f90:52-70): the
+  ! same loop body over a window of rows, calling the reference's list_fly,
+  ! j_flip_ni, representative_SQ and findstate against the full list, with the
+  ! rows spread over OpenMP threads as the reference spreads them.  For timing
+  ! the reference on a bounded sample of a large sector.
+  subroutine ref_hv_rows(i0, i1, v0, v1) bind(C, name="ref_hv_rows")
+    integer(c_int64_t), value :: i0, i1
+    complex(c_double_complex), intent(inout) :: v0(THS)
+    complex(c_double_complex), intent(in) :: v1(THS)
+    integer :: i, j, ell(3), s, id
+    integer, allocatable :: ni(:), st_list(:)
+    logical :: f1, f2
+    allocate(ni(NOD), st_list(NOD))
+    !$omp parallel do private(f1,f2,s,ni,id,ell,i,j,st_list)
+    do i = i0, i1
+      v0(i) = (0.0d0, 0.0d0)
+      st_list = list_fly(list_s(i), NOD, NOS)
+      do j = 1, NOxxz
+        f1 = any(st_list == p_xxz(1,j))
+        f2 = any(st_list == p_xxz(2,j))
+        if (f1 .neqv. f2) then
+          v0(i) = v0(i) - sJint(j,3) * v1(i)
+          ni = j_flip_ni(p_xxz(1,j), p_xxz(2,j), st_list, NOD)
+          call representative_SQ(NOD, s, ell, ni)
+          call findstate(s, list_s, id, THS)
+          if (id > 0) v0(i) = v0(i) + sJint(j,2) * list_r(id) / list_r(i) &
+               * explist(ell(1),ell(2),ell(3)) * v1(id)
+        else
+          v0(i) = v0(i) + sJint(j,3) * v1(i)
+        end if
+      end do
+    end do
+  end subroutine ref_hv_rows
+
   subroutine ref_explist(ex) bind(C, name="ref_explist")
     complex(c_double_complex), intent(out) :: ex((LX+1)*(LY+1)*(LZ+1))
     ex = reshape(explist, [(LX+1)*(LY+1)*(LZ+1)])

@@ -254,3 +254,32 @@ def test_fp32_mixed_solvers_large_chain():
         q3.set_option("tr_reorth", 0)
         q3.set_option("tr_cgs_passes", 2)
     assert abs(ene[0] - E64) <= 1e-9 * abs(E64), (ene[0], E64, st)
+
+
+@pytest.mark.parametrize("prec_name", ["fp64", "fp32"])
+def test_thick_restart_many_restarts_small_basis(prec_name):
+    """Thick restart with the three-term first cycle (tr_reorth 2), one
+    Gram-Schmidt pass requested and checks inside cycles, on a basis small
+    enough to force many restarts (chain L = 200, N_down = 4, NOM 24): the
+    restart cycles must keep the new vectors orthogonal to the locked Ritz
+    vectors (eigen_solver.f90:233-239 reorthogonalises fully), so E0 equals
+    plain FP64 Lanczos (lanczos.f90:50-152) to 1e-10 (FP64) / 1e-8 (FP32-mixed)."""
+    print(f"[gpu] thick restart many restarts {prec_name}", flush=True)
+    q3 = _q3()
+    prec = q3.FP64 if prec_name == "fp64" else q3.FP32
+    gm = q3.Model(200, 200, 1, 1, q3.chain_bonds(200, [(1, 1.0, 1.0)]))
+    s = q3.Sector.build(gm, 4, (0.0, 0.0, 0.0))
+    E64, _, _ = q3.lanczos_routines(s, maxitr=4000, lnc_ene_conv=1e-14, prec=q3.FP64, seed=1, want_vector=False)
+    q3.set_option("tr_check_every", 5)
+    q3.set_option("tr_reorth", 2)
+    q3.set_option("tr_cgs_passes", 1)
+    try:
+        ene, _, st = q3.my_trlanczos_routines_complex(s, NOE=1, NOK=4, NOM=24, maxitr=500, lnc_ene_conv=1e-12,
+                                                       prec=prec, seed=1, want_vectors=False)
+    finally:
+        q3.set_option("tr_check_every", 0)
+        q3.set_option("tr_reorth", 0)
+        q3.set_option("tr_cgs_passes", 2)
+    assert st["restarts"] >= 5
+    tol = 1e-10 if prec == q3.FP64 else 1e-8
+    assert abs(ene[0] - E64) <= tol * abs(E64), (ene[0], E64, st)

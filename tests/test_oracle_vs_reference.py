@@ -181,3 +181,24 @@ def test_u1_hv_against_reference(L, N, fam):
         r = ref.hv(v)
         o = om.hv_u1(N, v)
         assert np.max(np.abs(o - r)) <= 1e-13 * max(1.0, np.max(np.abs(r)))
+
+
+def test_reference_row_window_hv_equals_reference_hv():
+    """The harness's row-window H.v (ref_hv_rows, the loop body of
+    ham2vec_v3.f90:52-70 over rows i0..i1 with the reference's own routines,
+    used to time the reference on a sample of a large sector) equals the
+    reference's ham_to_vec_wave_vector row for row, also after the lists are
+    installed from elsewhere (ref_set_lists)."""
+    L, N, K = 18, 4, 5
+    bonds = orc.chain_bonds(L, [(1, 1.0, 0.7)])
+    ref = _ref_setup(L, L, 1, 1, N, (K, 0, 0), bonds)
+    D = ref.build_sector()
+    ls, lr = ref.lists()
+    rng = np.random.default_rng(4)
+    v = rng.uniform(-1, 1, D) + 1j * rng.uniform(-1, 1, D)
+    full = ref.hv(v)
+    ref.set_lists(ls, lr)
+    out = np.full(D, np.nan + 0j)
+    for r0, r1 in ((0, 7), (7, D // 2), (D // 2, D)):
+        ref.hv_rows(v, r0, r1, out=out)
+    assert np.array_equal(out, full)
