@@ -65,6 +65,14 @@ void trace_point(double t0, int64_t hv, double e, int prec) {
   trace().push_back({now_s() - t0, (double)hv, e, prec == QS3X_FP64 ? 64.0 : 32.0});
 }
 
+// Acceptance of the last FP32-mixed thick restart's lowest pair: spectral
+// gap estimate, FP64 residual, bound r^2 / gap, acceptance (1 certified, 2
+// agreement only, 0 not accepted; -1: no mixed solve yet).
+double *certificate() {
+  static double c[4] = {0.0, 0.0, 0.0, -1.0};
+  return c;
+}
+
 // Loads a start vector (host complex double) or generates the counter-based one.
 void load_start(int prec, void *dst, int64_t dim, const double *start, uint64_t seed, hipStream_t st) {
   if (start) {
@@ -80,6 +88,13 @@ void load_start(int prec, void *dst, int64_t dim, const double *start, uint64_t 
 }  // namespace
 
 extern "C" {
+
+int qs3x_solver_certificate(double *out) {
+  return guarded2([&] {
+    QS3X_REQUIRE(out, "qs3x_solver_certificate: NULL argument");
+    for (int k = 0; k < 4; ++k) out[k] = certificate()[k];
+  });
+}
 
 int qs3x_solver_trace(double *out, int64_t cap, int64_t *n) {
   return guarded2([&] {
@@ -862,6 +877,7 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
     std::vector<double> startv;  // FP64 start for the refinement (host not needed: device copy)
     TR t64;
     bool refined = false;
+    double accept_code = 0.0;
     int rr_j0 = 0;
     trace().clear();
     if (prec == QS3X_FP32) {
@@ -903,25 +919,40 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
       QS3X_HIP(hipStreamSynchronize(st));
       RRResult rr = rayleigh_ritz(t64, NOK, NOE);
       trace_point(t0, hv32 + t64.hv_count, rr.theta[0], QS3X_FP64);
-      // Accept the FP64 Rayleigh-Ritz values when, for every wanted pair,
-      // either the residual bound r^2 / gap (gap to the nearest other
-      // Rayleigh-Ritz value) is within conv |theta|, or the FP32 phase met its
-      // convergence test and the FP64 Rayleigh quotient agrees with the FP32
-      // Ritz value to conv |theta| (FP32 rounding has not moved the answer).
+      // Spectral gaps of the wanted pairs, from the FP32 projection's NOK
+      // lowest Ritz values: distance from theta_j to the nearest other Ritz
+      // value outside its cluster (which also holds any ghost copies of a
+      // converged value in a recurrence without reorthogonalisation).  A pair is certified when r^2 / gap <= conv |theta| with r
+      // its FP64 residual (the Kato-Temple bound, gap estimated from above by
+      // the next Ritz value).  Values the bound does not certify are still
+      // accepted when the FP32 phase met its convergence test and the FP64
+      // Rayleigh quotient agrees with the FP32 Ritz value to conv |theta|
+      // (stats[6] = 2: accepted, not certified).
+      // Values closer than 1e-8 max(1, |theta_j|) form one cluster (degenerate
+      // levels, split only by FP32 rounding); the gap of a cluster member is
+      // to the nearest value outside it, as in a block (cluster) bound.
+      std::vector<double> gap32(NOE, 0.0);
+      for (int j = 0; j < NOE; ++j) {
+        const double tol_c = 1e-8 * std::max(1.0, std::fabs(theta[j]));
+        double g = 1e300;
+        for (int i = 0; i < (int)theta.size(); ++i) {
+          const double d = std::fabs(theta[i] - theta[j]);
+          if (i != j && d > tol_c) g = std::min(g, d);
+        }
+        gap32[j] = g < 1e300 ? g : 0.0;
+      }
       auto bound_ok = [&](const RRResult &q, int j) {
-        double gap = 1e300;
-        for (int i = 0; i < (int)q.theta.size(); ++i)
-          if (i != j) gap = std::min(gap, std::fabs(q.theta[i] - q.theta[j]));
-        const double bound = q.resid[j] * q.resid[j] / std::max(gap, 1e-300);
-        return std::make_pair(bound <= conv * std::fabs(q.theta[j]), bound);
+        const double bound = q.resid[j] * q.resid[j] / std::max(gap32[j], 1e-300);
+        return std::make_pair(gap32[j] > 0.0 && bound <= conv * std::fabs(q.theta[j]), bound);
       };
-      bool ok = true;
+      bool ok = true, certified = true;
       for (int j = 0; j < NOE; ++j) {
         const auto b = bound_ok(rr, j);
         const bool agree = r.converged && std::fabs(rr.theta[j] - theta[j]) <= conv * std::fabs(rr.theta[j]);
         if (options().verbose)
-          std::fprintf(stderr, "[qs3x] pair %d: fp32 %.15g fp64 RR %.15g residual %.3e bound %.3e\n", j, theta[j],
-                       rr.theta[j], rr.resid[j], b.second);
+          std::fprintf(stderr, "[qs3x] pair %d: fp32 %.15g fp64 RR %.15g residual %.3e gap %.3e bound %.3e\n", j,
+                       theta[j], rr.theta[j], rr.resid[j], gap32[j], b.second);
+        if (!b.first) certified = false;
         if (!(b.first || agree)) ok = false;
       }
       resid = rr.resid;
@@ -940,15 +971,20 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
           const auto b = bound_ok(rf.rr, j);
           if (!b.first) ok2 = false;
           if (options().verbose)
-            std::fprintf(stderr, "[qs3x] refined pair %d: %.15g residual %.3e bound %.3e (block Krylov, %d vectors)\n",
-                         j, theta[j], resid[j], b.second, rf.k);
+            std::fprintf(stderr, "[qs3x] refined pair %d: %.15g residual %.3e gap %.3e bound %.3e (block Krylov, %d vectors)\n",
+                         j, theta[j], resid[j], gap32[j], b.second, rf.k);
         }
+        certified = certified || ok2;
         ok = ok || ok2;
         rr_j0 = 2 * rf.k;
         trace_point(t0, hv32 + t64.hv_count, theta[0], QS3X_FP64);
       }
+      certificate()[0] = gap32[0];
+      certificate()[1] = resid[0];
+      certificate()[2] = resid[0] * resid[0] / std::max(gap32[0], 1e-300);
       if (ok) {
         refined = true;
+        accept_code = certified ? 1.0 : 2.0;
       } else {
         // fall back to FP64 thick restart from the refined vectors: keep their
         // sum, free the FP32 basis, give the FP64 stage its full basis
@@ -1022,7 +1058,10 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
       stats[3] = now_s() - t0;
       stats[4] = t32;
       stats[5] = mr;
-      stats[6] = refined ? 1.0 : 0.0;  // FP32 Ritz vectors accepted after the FP64 Rayleigh-Ritz step
+      // FP32 Ritz vectors accepted after the FP64 Rayleigh-Ritz step: 1
+      // certified by the residual bound, 2 by agreement only, 0 not accepted
+      stats[6] = refined ? accept_code : 0.0;
+      if (prec == QS3X_FP32) certificate()[3] = stats[6];
       stats[7] = t_alloc;  // seconds allocating and freeing Krylov bases (not in stats[3] or the trace)
     }
   });

@@ -58,11 +58,13 @@ def _load():
         "qs3x_hv_dev": [_P, _i32, _P, _P, _P],
         "qs3x_time_hv": [_P, _i32, _i32, _P],
         "qs3x_hv_last_launch": [_P, _P, _P],
+        "qs3x_hv_stencil_info": [_P, _i32, _P],
         "qs3x_hv_autotune": [_P, _i32, _i32, _P, _P, _i32, _P, _P],
         "qs3x_hv_tune_clear": [_P],
         "qs3x_lanczos": [_P, _i32, _i64, _i64, _i64, _dbl, _P, _u64, _P, _P, _P],
         "qs3x_trlanczos": [_P, _i32, _i32, _i32, _i32, _i64, _dbl, _P, _u64, _P, _P, _P],
         "qs3x_solver_trace": [_P, _i64, _P],
+        "qs3x_solver_certificate": [_P],
         "qs3x_sector_build_all_k": [_P, _i32, _P],
         "qs3x_lanczos_group": [_P, _i32, _i32, _i64, _i64, _i64, _dbl, _u64, _i32, _P, _P, _P],
         "qs3x_dsf_sminus_batch": [_P, _i32, _P, _P, _P, _i32, _dbl, _i32, _P, _P, _P, _P],
@@ -240,6 +242,13 @@ class Sector:
         _check(_lib.qs3x_hv_last_launch(self._h, _p(f), _p(lds)))
         return int(f[0]), int(lds[0])
 
+    def hv_stencil_info(self, prec=FP32):
+        """Plan of the stencil H.v (after its first launch): dict of slow_rows,
+        lines, tasks, tile_lines, slabs, grid."""
+        info = np.zeros(6, dtype=np.int64)
+        _check(_lib.qs3x_hv_stencil_info(self._h, int(prec), _p(info)))
+        return dict(zip(["slow_rows", "lines", "tasks", "tile_lines", "slabs", "grid"], info.tolist()))
+
     def time_hv(self, prec=FP64, n_rep=10):
         ms = np.zeros(1)
         _check(_lib.qs3x_time_hv(self._h, int(prec), int(n_rep), _p(ms)))
@@ -304,6 +313,15 @@ def my_trlanczos_routines_complex(sector, NOE, NOK, NOM, maxitr=1000, lnc_ene_co
     keys = ["restarts", "hv_fp32", "hv_fp64", "seconds", "seconds_fp32", "max_residual", "rr_accepted",
             "seconds_alloc"]
     return ene, (psi.T.copy() if psi is not None else None), dict(zip(keys, stats.tolist()))
+
+
+def solver_certificate():
+    """Certificate of the last FP32-mixed thick restart: dict of gap (spectral
+    gap estimate), residual (FP64), bound (r^2 / gap), accepted (1 certified,
+    2 agreement only, 0 not accepted, -1 none)."""
+    out = np.zeros(4)
+    _check(_lib.qs3x_solver_certificate(_p(out)))
+    return dict(zip(["gap", "residual", "bound", "accepted"], out.tolist()))
 
 
 def solver_trace():

@@ -187,7 +187,8 @@ int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, i
  * in FP64 until the FP64 criterion holds.  ene (length NOE) receives the
  * lowest Ritz values; psi (host, dim x NOE column-major, may be NULL) the
  * Ritz vectors; stats (may be NULL, 8 doubles): [restarts, hv_fp32, hv_fp64,
- * seconds_total, seconds_fp32, max_residual, rr_accepted, seconds_alloc];
+ * seconds_total, seconds_fp32, max_residual, rr_accepted (1 certified by the
+ * residual bound, 2 accepted by agreement, 0 continued in FP64), seconds_alloc];
  * seconds_total, seconds_fp32 and the solver trace exclude seconds_alloc, the
  * time spent allocating the Krylov bases and freeing the FP32 one.  In FP32 mode the FP32 Ritz vectors
  * go through one FP64 Rayleigh-Ritz step (NOK FP64 H.v) and, with option
@@ -238,6 +239,15 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
 int qs3x_lanczos_group(qs3x_sector *const *secs, int32_t nsec, int32_t prec, int64_t maxitr,
                        int64_t minitr, int64_t itrint, double conv, uint64_t seed, int32_t use_graph,
                        double *ene, int64_t *iters, double *stats);
+
+/* Certificate of the last FP32-mixed qs3x_trlanczos (no counterpart in the
+ * reference): out (4 doubles) = [spectral gap of the lowest pair, estimated
+ * from the FP32 projection's next distinct Ritz value; that pair's FP64
+ * residual |H y - theta y|; the bound r^2 / gap on |theta - E0|; acceptance:
+ * 1 certified by the bound (<= conv |theta|), 2 accepted because the FP32
+ * and FP64 values agree (not certified), 0 not accepted (an FP64 thick
+ * restart continued), -1 no mixed solve yet]. */
+int qs3x_solver_certificate(double *out);
 
 /* Convergence trace of the last qs3x_lanczos / qs3x_trlanczos call (no
  * counterpart in the reference, whose solvers print progress instead):
