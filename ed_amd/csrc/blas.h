@@ -56,7 +56,11 @@ struct Options {
   int hv_stencil = 1;
   int hv_stencil_lines = 0;   // n3-lines per tile of the stencil kernel (0: per precision, 1..8)
   int hv_stencil_slabs = 0;   // n4-slabs per stencil task (0: per precision)
-  int hv_stencil_side = 1;    // stencil: key-search rows on a side stream (1) or the caller's stream (0)
+  // stencil: key-search rows on a side stream, overlapping the stencil launch
+  // (1), or after it on the caller's stream (0, the default: the overlap
+  // measured no gain at L = 1000 -- the stencil fills the CUs -- and slowed
+  // the stencil by 4 %)
+  int hv_stencil_side = 0;
   int verbose = 0;            // 1: one stderr line per restart / per 100 Lanczos steps
 };
 Options &options();

@@ -1269,9 +1269,9 @@ bool launch_hv_stencil(qs3x_sector *s, int prec, const HvArgs &a, hipStream_t st
     pl.grid = std::max(8, (grid / 8) * 8);
   }
   StencilArgs sa{s->d_lines.p, pl.tasks.p, pl.ntasks, pl.B};
-  // The slow rows (a latency-bound tail of about 1 % of the rows) run on a
-  // side stream, forked from and joined back into st, so they overlap the
-  // stencil launch; the two write disjoint rows and partial sums.
+  // The slow rows (a latency-bound tail of about 1 % of the rows) run after
+  // the stencil on st, or with option hv_stencil_side on a side stream forked
+  // from and joined back into st; the two write disjoint rows and partial sums.
   if (slow_grid > 0 && !options().hv_stencil_side) {
     HvArgs b = a;
     if (b.dot_partials) b.dot_partials += 2 * (size_t)pl.grid;

@@ -67,9 +67,10 @@ void trace_point(double t0, int64_t hv, double e, int prec) {
 
 // Acceptance of the last FP32-mixed thick restart's lowest pair: spectral
 // gap estimate, FP64 residual, bound r^2 / gap, acceptance (1 certified, 2
-// agreement only, 0 not accepted; -1: no mixed solve yet).
+// agreement only, 0 not accepted; -1: no mixed solve yet); then the last
+// thick restart's seconds allocating and freeing Krylov bases.
 double *certificate() {
-  static double c[4] = {0.0, 0.0, 0.0, -1.0};
+  static double c[6] = {0.0, 0.0, 0.0, -1.0, 0.0, 0.0};
   return c;
 }
 
@@ -92,7 +93,7 @@ extern "C" {
 int qs3x_solver_certificate(double *out) {
   return guarded2([&] {
     QS3X_REQUIRE(out, "qs3x_solver_certificate: NULL argument");
-    for (int k = 0; k < 4; ++k) out[k] = certificate()[k];
+    for (int k = 0; k < 6; ++k) out[k] = certificate()[k];
   });
 }
 
@@ -869,7 +870,7 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
     // Solve time and the trace exclude the Krylov basis allocations (their
     // cost depends on what the process freed before: 0 to about 4.5 s for
     // 160-250 GB); they are reported separately in stats[7].
-    double t0 = now_s(), t_alloc = 0.0;
+    double t0 = now_s(), t_alloc = 0.0, t_free = 0.0;
     std::vector<double> theta, resid;
     double t32 = 0.0;
     int64_t hv32 = 0, hv64 = 0;
@@ -998,7 +999,7 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
           t32s.basis.release();
           const double dt_free = now_s() - tf;
           t0 += dt_free;
-          t_alloc += dt_free;
+          t_free += dt_free;
         }
         const int64_t hv_rr = t64.hv_count;
         t64.init(s, QS3X_FP64, NOK, NOM64, 0, 2);
@@ -1017,7 +1018,7 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
       t32s.basis.release();
       const double dt_free = now_s() - tf;
       t0 += dt_free;
-      t_alloc += dt_free;
+      t_free += dt_free;
       t64.t_origin = t0;
     } else {
       t64.init(s, QS3X_FP64, NOK, NOM);
@@ -1062,7 +1063,9 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
       // certified by the residual bound, 2 by agreement only, 0 not accepted
       stats[6] = refined ? accept_code : 0.0;
       if (prec == QS3X_FP32) certificate()[3] = stats[6];
-      stats[7] = t_alloc;  // seconds allocating and freeing Krylov bases (not in stats[3] or the trace)
+      certificate()[4] = t_alloc;
+      certificate()[5] = t_free;
+      stats[7] = t_alloc;  // seconds allocating Krylov bases (not in stats[3] or the trace)
     }
   });
 }

@@ -318,10 +318,11 @@ def my_trlanczos_routines_complex(sector, NOE, NOK, NOM, maxitr=1000, lnc_ene_co
 def solver_certificate():
     """Certificate of the last FP32-mixed thick restart: dict of gap (spectral
     gap estimate), residual (FP64), bound (r^2 / gap), accepted (1 certified,
-    2 agreement only, 0 not accepted, -1 none)."""
-    out = np.zeros(4)
+    2 agreement only, 0 not accepted, -1 none), and the last thick restart's
+    seconds allocating and freeing its Krylov bases."""
+    out = np.zeros(6)
     _check(_lib.qs3x_solver_certificate(_p(out)))
-    return dict(zip(["gap", "residual", "bound", "accepted"], out.tolist()))
+    return dict(zip(["gap", "residual", "bound", "accepted", "seconds_alloc", "seconds_free"], out.tolist()))
 
 
 def solver_trace():

@@ -91,7 +91,9 @@ int qs3x_set_device(int32_t device);
  *                   kernel.  "hv_stencil_lines" (1..8; 0, the default: 8 in
  *                   FP32, 4 in FP64) and "hv_stencil_slabs" (0, the default:
  *                   8 in FP32, 16 in FP64) set its tile height and the
- *                   n4-slabs per task.
+ *                   n4-slabs per task; "hv_stencil_side" 1 runs its
+ *                   key-search rows on a side stream overlapping the stencil
+ *                   (default 0: after it).
  *   "verbose"       0 (default); 1: a stderr line per thick restart and per
  *                   100 Lanczos steps; 2: per convergence check.
  * The environment variable QS3X_OPTIONS="name=value,name=value" applies the
@@ -190,7 +192,8 @@ int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, i
  * seconds_total, seconds_fp32, max_residual, rr_accepted (1 certified by the
  * residual bound, 2 accepted by agreement, 0 continued in FP64), seconds_alloc];
  * seconds_total, seconds_fp32 and the solver trace exclude seconds_alloc, the
- * time spent allocating the Krylov bases and freeing the FP32 one.  In FP32 mode the FP32 Ritz vectors
+ * time spent allocating the Krylov bases, and the time freeing them
+ * (qs3x_solver_certificate).  In FP32 mode the FP32 Ritz vectors
  * go through one FP64 Rayleigh-Ritz step (NOK FP64 H.v) and, with option
  * fp64_refine > 0, the FP64 block-Krylov refinement; the result is accepted
  * (rr_accepted = 1) when, for every wanted pair, r^2/gap <= conv |theta| before
@@ -241,12 +244,15 @@ int qs3x_lanczos_group(qs3x_sector *const *secs, int32_t nsec, int32_t prec, int
                        double *ene, int64_t *iters, double *stats);
 
 /* Certificate of the last FP32-mixed qs3x_trlanczos (no counterpart in the
- * reference): out (4 doubles) = [spectral gap of the lowest pair, estimated
+ * reference): out (6 doubles) = [spectral gap of the lowest pair, estimated
  * from the FP32 projection's next distinct Ritz value; that pair's FP64
  * residual |H y - theta y|; the bound r^2 / gap on |theta - E0|; acceptance:
  * 1 certified by the bound (<= conv |theta|), 2 accepted because the FP32
  * and FP64 values agree (not certified), 0 not accepted (an FP64 thick
- * restart continued), -1 no mixed solve yet]. */
+ * restart continued), -1 no mixed solve yet; then, for the last thick
+ * restart of either precision, seconds allocating Krylov bases (stats[7])
+ * and seconds freeing them (not in stats[3] or the trace: the free of the
+ * FP32 basis of a mixed solve comes after its energies are known)]. */
 int qs3x_solver_certificate(double *out);
 
 /* Convergence trace of the last qs3x_lanczos / qs3x_trlanczos call (no
