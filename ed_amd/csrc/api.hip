@@ -1,6 +1,7 @@
 // api.hip -- C ABI entry points of libqs3x (model, sectors, H.v, timing).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -268,10 +269,34 @@ double qs3x_wavevector(int32_t K, int32_t L) {
   return 2.0 * PI * (double)K / (double)L;
 }
 
+// A wave vector the lattice's translations can carry: rk_a L_a / (2 pi) an
+// integer in every direction.  The reference builds a target sector at
+// rk + q without this check (get_expectation_values.f90:430-447), so an
+// incommensurate q (input_ex3's QZ = pi/2 on LZ = 10) gives norms and phases
+// of no translation eigenspace; this build refuses it.
+static void require_commensurate(const qs3x_model *m, const double *rk, const char *who) {
+  const double PI = std::acos(-1.0);
+  const int Ls[3] = {m->LX, m->LY, m->LZ};
+  const char *ax = "xyz";
+  for (int a = 0; a < 3; ++a) {
+    const double K = rk[a] * Ls[a] / (2.0 * PI);
+    if (std::fabs(K - std::nearbyint(K)) > 1e-8 * std::max(1.0, std::fabs(K))) {
+      char buf[256];
+      std::snprintf(buf, sizeof buf,
+                    "%s: wave vector rk%c = %.17g is not a multiple of 2 pi / L%c (L%c = %d, K = %.12g); "
+                    "snap q to 2 pi K / L with an integer K",
+                    who, ax[a], rk[a], ax[a] + ('A' - 'a'), ax[a] + ('A' - 'a'), Ls[a], K);
+      throw Error{buf};
+    }
+  }
+}
+
 int qs3x_sector_build(qs3x_model *m, int32_t nod, double rkx, double rky, double rkz,
                       qs3x_sector **out, int64_t *dim) {
   return guarded([&] {
     QS3X_REQUIRE(m && out, "qs3x_sector_build: NULL argument");
+    const double rkv[3] = {rkx, rky, rkz};
+    require_commensurate(m, rkv, "qs3x_sector_build");
     auto *s = new qs3x_sector();
     s->model = m;
     s->nod = nod;

@@ -124,7 +124,9 @@ double qs3x_wavevector(int32_t K, int32_t L);
  * (colex-minimal orbit members), their norms and a rank-addressed index.
  * Replaces allocate_lists_omp_SQ (state_lists.f90:225-292) and, with
  * rk = rk + q, allocate_lists_omp_SQ_new (state_lists.f90:418-486).
- * THS (the sector dimension) is returned through dim. */
+ * THS (the sector dimension) is returned through dim.  Every rk_a L_a / (2 pi)
+ * must be an integer (to 1e-8): an incommensurate wave vector, which the
+ * reference accepts for rk + q without a check, is an error here. */
 int qs3x_sector_build(qs3x_model *m, int32_t nod, double rkx, double rky, double rkz,
                       qs3x_sector **out, int64_t *dim);
 /* U(1)-only block: every configuration with nod down spins, rows in colex

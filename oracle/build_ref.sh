@@ -79,3 +79,12 @@ if [ -f "$SRCD" ]; then
   $FC -O2 -module-dir "$OUT/dsflan" "$SRCD" -o "$OUT/dsflan.exe"
   echo "built $OUT/dsflan.exe"
 fi
+
+# mk_input_list (source_mk_input_list/mk_input_list.f90): the reference's
+# square / triangular bond-list generator; reads its namelist from stdin.
+SRCM="$REF/source_mk_input_list/mk_input_list.f90"
+if [ -f "$SRCM" ]; then
+  mkdir -p "$OUT/mkinput"
+  $FC -O2 -module-dir "$OUT/mkinput" "$SRCM" -o "$OUT/mk_input_list.exe"
+  echo "built $OUT/mk_input_list.exe"
+fi

@@ -31,8 +31,9 @@ def test_reference_driver_links_against_shims(tmp_path):
 
 
 def _write_deck(d, NOS, LX, LY, nod, K, bonds, alg, cal_dsf=0, spsmsz=2, itr_dsf=30, q=(0.0, 0.0, 0.0),
-                rfield=0.0, nov=2, trlan=(3, 6, 16)):
-    os.makedirs(os.path.join(d, "out"), exist_ok=True)
+                rfield=0.0, nov=2, trlan=(3, 6, 16), wr_wf=0, re_wf=0, outdir="out/", filewf="out/"):
+    os.makedirs(os.path.join(d, outdir), exist_ok=True)
+    os.makedirs(os.path.join(d, filewf), exist_ok=True)
     with open(os.path.join(d, "bonds.dat"), "w") as f:
         for (a, b, jxy, jz) in bonds:
             f.write(f"{int(a):5d}{int(b):5d} {jxy: .15E} {jz: .15E}\n")
@@ -49,8 +50,8 @@ def _write_deck(d, NOS, LX, LY, nod, K, bonds, alg, cal_dsf=0, spsmsz=2, itr_dsf
     noe, nok, nom = trlan
     deck = f"""&input_parameters
 NOS = {NOS}, NOD = {nod}, LX = {LX}, LY = {LY}, LZ = 1, KX = {K[0]}, KY = {K[1]}, KZ = 0,
-NOxxz = {len(bonds)}, ALG = {alg}, cal_lm = 1, cal_cf = 1, cal_dsf = {cal_dsf}, wr_wf = 0, re_wf = 0,
-FILExxz = "bonds.dat", FILEwf = "out/", OUTDIR = "out/",
+NOxxz = {len(bonds)}, ALG = {alg}, cal_lm = 1, cal_cf = 1, cal_dsf = {cal_dsf}, wr_wf = {wr_wf}, re_wf = {re_wf},
+FILExxz = "bonds.dat", FILEwf = "{filewf}", OUTDIR = "{outdir}",
 &end
 &input_static
 NOV = {nov}, NOLM = {NOS}, NOCF = {NOS * NOS}, FILElm = "lm.dat", FILECF = "cf.dat",
@@ -70,11 +71,13 @@ NOE = {noe}, NOK = {nok}, NOM = {nom}, maxitr = 500, lnc_ene_conv = 1.0d-13, i_v
         f.write(deck)
 
 
-def _run(d):
+def _run(d, expect_ok=True):
     exe = os.path.join(FDIR, "qs3x_main.exe")
     assert os.path.exists(exe), "run ed_amd/fortran/build_fortran.sh"
     with open(os.path.join(d, "input.dat")) as fin:
         r = subprocess.run([exe], stdin=fin, cwd=d, capture_output=True, text=True, timeout=100)
+    if not expect_ok:
+        return r
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
     return r.stdout
 
@@ -138,3 +141,68 @@ def test_driver_lanczos_chain_with_dsf(tmp_path):
     assert ab[0, 5] == pytest.approx(n1, rel=1e-9)        # norm1 (gauge invariant)
     assert np.allclose(ab[:10, 3], a[:10], atol=1e-7)       # alpha_n
     assert np.allclose(ab[:10, 4], b2[:10], atol=1e-7)      # beta_n^2
+
+
+@pytest.mark.gpu
+def test_driver_ferromagnetic_state(tmp_path):
+    """N_down = 0: THS = 1 and the driver writes only E_FM = sum_b Jz_b / 4
+    (main.f90:36-46, calcu_FM_energy, eigen_solver.f90:341-347)."""
+    print("[gpu] fortran driver N_down=0", flush=True)
+    d = str(tmp_path)
+    L = 10
+    bonds = orc.chain_bonds(L, [(1, 1.0, 0.8), (2, 0.3, -0.45)])
+    _write_deck(d, L, L, 1, 0, (0, 0), bonds, alg=1)
+    _run(d)
+    en = _read_cols(os.path.join(d, "out", "energy.dat"))
+    assert en.shape == (1, 1)
+    assert en[0, 0] == pytest.approx(sum(b[3] for b in bonds) / 4.0, abs=1e-13)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("alg", [1, 2])
+def test_driver_wavefunction_round_trip(tmp_path, alg):
+    """wr_wf = 1 writes the wave function (input_param.f90:119-147, one
+    unformatted record per vector); a second run with re_wf = 1 reads it back
+    instead of solving (main.f90:66-72, 99-105) and its observables equal the
+    first run's, byte for byte in the formatted files."""
+    print(f"[gpu] fortran driver wr_wf / re_wf ALG={alg}", flush=True)
+    d = str(tmp_path)
+    L, nod = 12, 4
+    bonds = orc.chain_bonds(L, [(1, 1.0, 0.9)])
+    _write_deck(d, L, L, 1, nod, (0, 0), bonds, alg=alg, nov=1, trlan=(1, 3, 12), wr_wf=1, outdir="out1/",
+                filewf="wf/")
+    _run(d)
+    wfs = [f for f in os.listdir(os.path.join(d, "wf")) if f.strip().endswith(".bin")]
+    assert len(wfs) == 1 and os.path.exists(os.path.join(d, "wf", "energy.dat"))
+    _write_deck(d, L, L, 1, nod, (0, 0), bonds, alg=alg, nov=1, trlan=(1, 3, 12), re_wf=1, outdir="out2/",
+                filewf="wf/")
+    out = _run(d)
+    assert "Eigen solver" not in out  # the solver did not run
+    for name in ("local_mag.dat", "SzSz-corr.dat", "SpSm-corr.dat"):
+        with open(os.path.join(d, "out1", name)) as a, open(os.path.join(d, "out2", name)) as b:
+            assert a.read() == b.read(), name
+    # and the vector read back is the ground state: its correlations are the oracle's
+    m = orc.Model(L, L, 1, 1, bonds)
+    sec = m.build_sector(nod, (0.0, 0.0, 0.0))
+    w, V = np.linalg.eigh(m.full_hamiltonian(sec))
+    cf = _read_cols(os.path.join(d, "out2", "SzSz-corr.dat"))
+    pairs = [(int(a), int(b)) for a, b in cf[:, :2]]
+    sz_o, _ = m.correlations(sec, V[:, 0].astype(np.complex128), pairs)
+    assert np.allclose(cf[:, 2], sz_o, atol=1e-8)
+
+
+@pytest.mark.gpu
+def test_driver_refuses_incommensurate_q(tmp_path):
+    """A DSF momentum transfer q that is not a multiple of 2 pi / L (the
+    reference builds an inconsistent target sector at k + q without a check,
+    get_expectation_values.f90:430-447) stops the driver with an error that
+    names the wave vector."""
+    print("[gpu] fortran driver incommensurate q", flush=True)
+    d = str(tmp_path)
+    L, nod = 12, 3
+    bonds = orc.chain_bonds(L, [(1, 1.0, 1.0)])
+    q = (2 * np.pi * 2.5 / L, 0.0, 0.0)
+    _write_deck(d, L, L, 1, nod, (0, 0), bonds, alg=1, cal_dsf=1, spsmsz=2, itr_dsf=10, q=q, nov=1)
+    r = _run(d, expect_ok=False)
+    assert r.returncode != 0
+    assert "not a multiple of 2 pi" in (r.stdout + r.stderr)

@@ -11,7 +11,7 @@ from oracle import qs3 as orc
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 FILES = sorted(f for f in glob.glob(os.path.join(HERE, "golden", "*.npz"))
-               if not os.path.basename(f).startswith("dsflan_"))  # DSFLan: test_dsflan_reference.py
+               if not os.path.basename(f).startswith(("dsflan_", "mk_input_list")))  # test_dsflan_reference.py, test_inputs.py
 IDS = [os.path.basename(f)[:-4] for f in FILES]
 
 

@@ -431,3 +431,19 @@ def test_hv_stencil_matches_nn_kernel_c3():
             assert err <= tol, (K, dt, err)
             del x, y0, y1
         del s
+
+
+def test_sector_build_refuses_incommensurate_wave_vector():
+    """rk L / (2 pi) must be an integer in every direction (the target sector
+    of the DSF is built at k + q, get_expectation_values.f90:430-447)."""
+    print("[gpu] incommensurate wave vector", flush=True)
+    q3 = _q3()
+    gm = q3.Model(12, 12, 1, 1, orc.chain_bonds(12, [(1, 1.0, 1.0)]))
+    with pytest.raises(q3.QS3XError, match="not a multiple of 2 pi"):
+        q3.Sector.build(gm, 3, (2 * np.pi * 2.5 / 12, 0.0, 0.0))
+    gt = q3.Model(16, 4, 4, 1, orc.square_bonds(4, 4, 1.0))
+    with pytest.raises(q3.QS3XError, match="rky"):
+        q3.Sector.build(gt, 2, (0.0, 0.3, 0.0))
+    # commensurate vectors still build, including 2 pi K / L with K >= L
+    s = q3.Sector.build(gm, 3, (q3.wavevector(17, 12), 0.0, 0.0))
+    assert s.dim == q3.Sector.build(gm, 3, (q3.wavevector(5, 12), 0.0, 0.0)).dim
