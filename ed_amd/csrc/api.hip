@@ -115,6 +115,9 @@ int qs3x_set_option(const char *name, double value) {
     } else if (n == "hv_stencil_side") {
       QS3X_REQUIRE(value == 0 || value == 1, "hv_stencil_side must be 0 or 1");
       options().hv_stencil_side = (int)value;
+    } else if (n == "dense_solver") {
+      QS3X_REQUIRE(value == 0 || value == 1, "dense_solver must be 0 or 1");
+      options().dense_solver = (int)value;
     } else if (n == "fp32_floor") {
       QS3X_REQUIRE(value > 0, "fp32_floor must be > 0");
       options().fp32_floor = value;

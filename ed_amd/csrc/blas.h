@@ -61,6 +61,11 @@ struct Options {
   // measured no gain at L = 1000 -- the stencil fills the CUs -- and slowed
   // the stencil by 4 %)
   int hv_stencil_side = 0;
+  // dense eigensolver of qs3x_full_diag: 0 = QR iteration on the tridiagonal
+  // form (rocSOLVER zheev, the reference's LAPACK zheev algorithm; default),
+  // 1 = divide and conquer (zheevd; faster for large dim, but it has returned
+  // NaN eigenvectors with info 0 on exactly degenerate spectra)
+  int dense_solver = 0;
   int verbose = 0;            // 1: one stderr line per restart / per 100 Lanczos steps
 };
 Options &options();

@@ -72,6 +72,21 @@ module qs3x_iso_c
       complex(c_double_complex), intent(in) :: v1(*)
       integer(c_int) :: rc
     end function
+    function qs3x_make_full_hamiltonian(s, ham, ld) bind(C, name="qs3x_make_full_hamiltonian") result(rc)
+      import :: c_ptr, c_double_complex, c_int64_t, c_int
+      type(c_ptr), value :: s
+      complex(c_double_complex), intent(out) :: ham(*)
+      integer(c_int64_t), value :: ld
+      integer(c_int) :: rc
+    end function
+    function qs3x_full_diag(s, ene, psi, ld) bind(C, name="qs3x_full_diag") result(rc)
+      import :: c_ptr, c_double, c_double_complex, c_int64_t, c_int
+      type(c_ptr), value :: s
+      real(c_double), intent(out) :: ene(*)
+      complex(c_double_complex), intent(out) :: psi(*)
+      integer(c_int64_t), value :: ld
+      integer(c_int) :: rc
+    end function
     function qs3x_lanczos(s, prec, maxitr, minitr, itrint, conv, start, seed, ene, psi, iters) &
         bind(C, name="qs3x_lanczos") result(rc)
       import :: c_ptr, c_int32_t, c_int64_t, c_double, c_double_complex, c_int

@@ -306,12 +306,18 @@ contains
     call qs3x_check(qs3x_ham_to_vec_wave_vector(h, v0, v1), "ham_to_vec_wave_vector")
   end subroutine ham_to_vec_wave_vector
 
-  ! Dense H of the main sector, column by column (ham2vec_v3.f90:5-35).
+  ! Dense H (ham2vec_v3.f90:5-35).  The main sector's is assembled on the
+  ! device, one H e_j per column; any other sector of dimension lv goes
+  ! column by column through ham_to_vec_wave_vector.
   subroutine make_full_hamiltonian(lv, Ham)
     integer, intent(in) :: lv
     complex(8), intent(inout) :: Ham(lv, lv)
     complex(8), allocatable :: e(:)
     integer :: j
+    if (c_associated(qs3x_sector_h) .and. lv == THS) then
+      call qs3x_check(qs3x_make_full_hamiltonian(qs3x_sector_h, Ham, int(lv, c_int64_t)), "make_full_hamiltonian")
+      return
+    end if
     allocate(e(lv))
     do j = 1, lv
       e = (0.0d0, 0.0d0)
@@ -424,15 +430,32 @@ contains
     write(*, input_TRLan)
   end subroutine read_input_TRLan
 
-  ! ALG = 3 (dense zheev, eigen_solver.f90:51-67,263-339) is not part of the
-  ! GPU engine.
+  ! Full_diag_routines / fulldia_routines_with_momentum
+  ! (eigen_solver.f90:51-67,263-339): dense H of the main sector and all its
+  ! eigenpairs on the device; full_eigenvalues.dat and the ground state in
+  ! full_eigenvectors.dat as the reference writes them.
   subroutine Full_diag_routines(ene, psi, THS_)
     real(8), allocatable, intent(out) :: ene(:)
     complex(8), allocatable, intent(out) :: psi(:,:)
     integer, intent(in) :: THS_
-    allocate(ene(max(THS_, 2)), psi(THS_, 1))
-    write(*,*) "ALG = 3 (full diagonalisation) is not provided by the MI355X engine; use ALG = 1 or 2"
-    error stop 3
+    integer :: k
+    dim = THS_
+    allocate(ene(max(dim, 2)), psi(dim, dim))
+    ene = 99.0d0
+    print *, "make ham"
+    print *, "diag starts"
+    call qs3x_check(qs3x_full_diag(qs3x_sector_h, ene, psi, int(dim, c_int64_t)), "Full_diag_routines")
+    print *, "Eigenvalues by Full Diagonalization"
+    open(111, file=trim(adjustl(OUTDIR))//'full_eigenvalues.dat', position='append')
+    do k = 1, dim
+      write(111, '(es23.15)') ene(k)
+    end do
+    close(111)
+    open(112, file=trim(adjustl(OUTDIR))//'full_eigenvectors.dat', position='append')
+    do k = 1, dim
+      write(112, *) psi(k, 1)
+    end do
+    close(112)
   end subroutine Full_diag_routines
 
   subroutine calcu_FM_energy(ene0)

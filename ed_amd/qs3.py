@@ -6,7 +6,9 @@ read like the reference:
   state_lists  -> Model, Sector.build (allocate_lists_omp_SQ), Sector.lists
   ham2vec      -> Sector.ham_to_vec_wave_vector
   lanczos      -> lanczos_routines
-  eigen_solver -> my_trlanczos_routines_complex, calcu_FM_energy
+  eigen_solver -> my_trlanczos_routines_complex, calcu_FM_energy,
+                  Sector.full_diag (Full_diag_routines, ALG = 3)
+  ham2vec      -> Sector.make_full_hamiltonian (make_full_hamiltonian)
   get_expectation_values -> get_cf_2_wave_vector, get_lm_2_wave_vector,
                   calcu_Sq_a, calcu_DSF_wavevector
   DSFLan       -> dsf_analysis
@@ -56,6 +58,8 @@ def _load():
         "qs3x_ham_to_vec_wave_vector": [_P, _P, _P],
         "qs3x_ham_to_vec_wave_vector_noalloc": [_P, _P, _P],
         "qs3x_hv_dev": [_P, _i32, _P, _P, _P],
+        "qs3x_make_full_hamiltonian": [_P, _P, _i64],
+        "qs3x_full_diag": [_P, _P, _P, _i64],
         "qs3x_time_hv": [_P, _i32, _i32, _P],
         "qs3x_hv_last_launch": [_P, _P, _P],
         "qs3x_hv_stencil_info": [_P, _i32, _P],
@@ -227,6 +231,22 @@ class Sector:
         out = np.zeros_like(a)
         _check(_lib.qs3x_ham_to_vec_wave_vector_noalloc(self._h, _p(out), _p(a)))
         return out
+
+    def make_full_hamiltonian(self):
+        """The dense H (ham2vec_v3.f90:5-35): H[i, j] = row i's term on
+        configuration j, built as one H e_j per column on the device."""
+        h = np.zeros((self.dim, self.dim), dtype=np.complex128, order="F")
+        _check(_lib.qs3x_make_full_hamiltonian(self._h, _p(h), self.dim))
+        return h
+
+    def full_diag(self, vectors=True):
+        """Full diagonalisation (Full_diag_routines, eigen_solver.f90:51-67,
+        263-339): eigenvalues ascending and, with vectors, the orthonormal
+        eigenvectors as the columns of a (dim, dim) array."""
+        w = np.zeros(self.dim)
+        v = np.zeros((self.dim, self.dim), dtype=np.complex128, order="F") if vectors else None
+        _check(_lib.qs3x_full_diag(self._h, _p(w), _p(v), self.dim))
+        return (w, v) if vectors else w
 
     def hv_dev(self, v0_ptr, v1_ptr, prec=FP64, stream=None):
         """Device-pointer H.v (FP32: v0 = (H - e_zz) v1)."""

@@ -94,6 +94,9 @@ int qs3x_set_device(int32_t device);
  *                   n4-slabs per task; "hv_stencil_side" 1 runs its
  *                   key-search rows on a side stream overlapping the stencil
  *                   (default 0: after it).
+ *   "dense_solver"  eigensolver of qs3x_full_diag: 0 (default) QR iteration
+ *                   (rocSOLVER zheev, the algorithm of LAPACK zheev that the
+ *                   reference calls); 1 divide and conquer (zheevd).
  *   "verbose"       0 (default); 1: a stderr line per thick restart and per
  *                   100 Lanczos steps; 2: per convergence check.
  * The environment variable QS3X_OPTIONS="name=value,name=value" applies the
@@ -206,6 +209,23 @@ int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, i
 int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32_t NOM,
                    int64_t maxitr, double conv, const double *start, uint64_t seed,
                    double *ene, double *psi, double *stats);
+
+/* ------------------------------------------------ dense path (ALG = 3) */
+/* The dense H of the sector into host ham (complex double, column-major,
+ * leading dimension ld >= dim): ham(i, j) = the term of row i on configuration
+ * j.  Replaces make_full_hamiltonian (ham2vec_v3.f90:5-35).  Column j is one
+ * FP64 H . v launch on e_j, so the matrix is the matrix-free operator itself;
+ * needs 16 dim^2 bytes of device memory. */
+int qs3x_make_full_hamiltonian(qs3x_sector *s, double *ham, int64_t ld);
+/* Full diagonalisation on the device: ene (host, dim) the eigenvalues in
+ * ascending order; psi (host, column-major, leading dimension ld >= dim, may be
+ * NULL for eigenvalues only) the orthonormal eigenvectors.  Replaces
+ * Full_diag_routines / fulldia_routines_with_momentum
+ * (eigen_solver.f90:51-67,263-339): the dense H above, diagonalised from its
+ * upper triangle (as zheev('V', 'U') reads it) by rocSOLVER zheev (option
+ * "dense_solver" 1: zheevd).  A non-finite eigenvalue or eigenvector
+ * component is an error.  dim < 2^31. */
+int qs3x_full_diag(qs3x_sector *s, double *ene, double *psi, int64_t ld);
 
 /* ------------------------------------------------------ batched wave vectors */
 /* A chain sector holding every translation representative of N_down (the
