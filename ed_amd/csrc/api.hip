@@ -115,6 +115,20 @@ int qs3x_set_option(const char *name, double value) {
     } else if (n == "hv_stencil_side") {
       QS3X_REQUIRE(value == 0 || value == 1, "hv_stencil_side must be 0 or 1");
       options().hv_stencil_side = (int)value;
+    } else if (n == "block_stencil") {
+      QS3X_REQUIRE(value == 0 || value == 1, "block_stencil must be 0 or 1");
+      options().block_stencil = (int)value;
+    } else if (n == "block_stencil_lines") {
+      const int v = (int)value;
+      QS3X_REQUIRE(v == value && v >= 0 && v <= 14 && v % 2 == 0,
+                   "block_stencil_lines must be 0 (default) or even, 2..14");
+      options().block_stencil_lines = v;
+    } else if (n == "block_stencil_threads") {
+      QS3X_REQUIRE(value == 0 || value == 256 || value == 512, "block_stencil_threads must be 0, 256 or 512");
+      options().block_stencil_threads = (int)value;
+    } else if (n == "block_stencil_slabs") {
+      QS3X_REQUIRE(value >= 0 && value <= 1024, "block_stencil_slabs must be 0..1024");
+      options().block_stencil_slabs = (int)value;
     } else if (n == "dense_solver") {
       QS3X_REQUIRE(value == 0 || value == 1, "dense_solver must be 0 or 1");
       options().dense_solver = (int)value;

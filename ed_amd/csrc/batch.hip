@@ -84,6 +84,8 @@ struct BlockHvArgs {
   const void *eph;        // [L+1][kQ] complex in precision R: explist_q(ell, 1, 1); entry 0 = 1
   double *dot_partials;   // [grid][kQ] complex
   int nxcd;               // L2 domains for the chunk schedule (tile_range)
+  const int32_t *rows;    // non-null: apply only these rows (ascending), nrows of them
+  int64_t nrows;
 };
 
 // C(x, q) for 1 <= q <= N - 1 from the LDS table.
@@ -230,11 +232,12 @@ __global__ __launch_bounds__(kBThreads) void k_hv_block(BlockHvArgs a) {
   R *s_jz4 = s_jxy2 + a.nbt;
   int32_t *s_wave = (int32_t *)(((uintptr_t)(s_jz4 + a.nbt) + 15) & ~(uintptr_t)15);
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const size_t wave_words = (size_t)64 * H * 2 + 64 + 64 * (sizeof(R) / 4);
+  const size_t wave_words = (size_t)64 * H * 2 + 64 + 64 + 64 * (sizeof(R) / 4);
   int32_t *w_id = s_wave + wid * wave_words;
   int32_t *w_aux = w_id + 64 * H;
   int32_t *w_cls = w_aux + 64 * H;
-  R *w_anti = (R *)(w_cls + 64);
+  int32_t *w_row = w_cls + 64;
+  R *w_anti = (R *)(w_row + 64);
   for (int q = threadIdx.x; q < nb; q += kBThreads) {
     const int m = q / (L - 1) + 1, x = q % (L - 1);
     s_binom[q] = (int32_t)a.binom[(int64_t)x * a.kdim + m];
@@ -252,7 +255,10 @@ __global__ __launch_bounds__(kBThreads) void k_hv_block(BlockHvArgs a) {
   const C e1 = cload<R>(a.eph, (size_t)1 * kQ + lane);
   const C eL1 = cload<R>(a.eph, (size_t)(L - 1) * kQ + lane);
   double dre = 0.0, dim_ = 0.0;
-  const int64_t nchunks = (a.dim + 63) / 64;
+  // the rows applied: all of them, or the listed ones (entry i of the list
+  // stands where row i would)
+  const int64_t nit = a.rows ? a.nrows : a.dim;
+  const int64_t nchunks = (nit + 63) / 64;
   // a block takes kBWaves consecutive chunks at a time, XCD-contiguous ranges
   const TileRange tr = tile_range((nchunks + kBWaves - 1) / kBWaves, a.nxcd);
   for (int64_t bt = tr.first; bt < tr.end; bt += tr.stride) {
@@ -261,11 +267,12 @@ __global__ __launch_bounds__(kBThreads) void k_hv_block(BlockHvArgs a) {
     const int64_t base = ch * 64;
     // lanes = rows: hop structure of the chunk
     {
-      const int64_t row = base + lane;
+      const int64_t it = base + lane;
+      const int64_t row = it < nit ? (a.rows ? (int64_t)a.rows[it] : it) : 0;
       int hc[H], hcnt[H], hell[H], hbs[H];
       int cls = 0;
       R anti = 0;
-      if (row < a.dim) {
+      if (it < nit) {
         const uint64_t w = a.cfg[row];
         cls = (int)(w >> 56);
         int n[N];
@@ -295,11 +302,12 @@ __global__ __launch_bounds__(kBThreads) void k_hv_block(BlockHvArgs a) {
         w_aux[lane * H + h] = hcnt[h] | (hell[h] << 5) | (bt << 17);
       }
       w_cls[lane] = cls;
+      w_row[lane] = (int32_t)row;
       w_anti[lane] = anti;
     }
     __builtin_amdgcn_wave_barrier();
     // lanes = vectors: apply the chunk's rows to the kQ vectors, two rows at a time
-    const int nr = (int)min<int64_t>(64, a.dim - base);
+    const int nr = (int)min<int64_t>(64, nit - base);
     for (int r0 = 0; r0 < nr; r0 += 2) {
       constexpr int RR = 2;
       C xin[RR], y[RR][H];
@@ -307,14 +315,15 @@ __global__ __launch_bounds__(kBThreads) void k_hv_block(BlockHvArgs a) {
 #pragma unroll
       for (int k = 0; k < RR; ++k) {
         const int r = min(r0 + k, nr - 1);
-        xin[k] = cload<R>(a.x, (base + r) * kQ + lane);
+        const int64_t rw = w_row[r];
+        xin[k] = cload<R>(a.x, rw * kQ + lane);
 #pragma unroll
         for (int h = 0; h < H; ++h) {
           id[k][h] = w_id[r * H + h];
           aux[k][h] = w_aux[r * H + h];
         }
 #pragma unroll
-        for (int h = 0; h < H; ++h) y[k][h] = cload<R>(a.x, (int64_t)(id[k][h] >= 0 ? id[k][h] : base + r) * kQ + lane);
+        for (int h = 0; h < H; ++h) y[k][h] = cload<R>(a.x, (int64_t)(id[k][h] >= 0 ? id[k][h] : rw) * kQ + lane);
       }
 #pragma unroll
       for (int k = 0; k < RR; ++k) {
@@ -350,7 +359,7 @@ __global__ __launch_bounds__(kBThreads) void k_hv_block(BlockHvArgs a) {
         C out;
         out.x = mk * (dg * x.x + accx * scale);
         out.y = mk * (dg * x.y + accy * scale);
-        st_nt(&((C *)a.y)[(base + r) * kQ + lane], out);  // written once per step
+        st_nt(&((C *)a.y)[(int64_t)w_row[r] * kQ + lane], out);  // written once per step
         dre += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
         dim_ += (double)x.x * (double)out.y - (double)x.y * (double)out.x;
       }
@@ -379,7 +388,7 @@ size_t hv_block_lds(int L, int nbt, int prec) {
   const size_t nb = (size_t)(N - 1) * (L - 1);
   size_t b = ((nb * 4 + 15) & ~(size_t)15) + rs * ((N + 1) * (N + 1) * kQ + (N + 1) * kQ + 2 * nbt);
   b = (b + 15) & ~(size_t)15;
-  const size_t wave_words = (size_t)64 * 2 * N * 2 + 64 + 64 * (rs / 4);
+  const size_t wave_words = (size_t)64 * 2 * N * 2 + 64 + 64 + 64 * (rs / 4);
   return b + 4 * wave_words * kBWaves;
 }
 
@@ -403,54 +412,336 @@ BFn block_kernel(int N, int prec, bool key32, size_t *lds, int L, int nbt) {
   }
 }
 
-// --------------------------------------------------------- block update
-// w_q -= a_q s_q u_q + b_q s'_q up_q (b term when HASB); part[block][q] = |w_q|^2
-template <class R, bool HASB>
-__global__ __launch_bounds__(kBThreads) void k_update3_block(typename CT<R>::T *w, const typename CT<R>::T *u,
-                                                             const typename CT<R>::T *up, int64_t dim,
-                                                             const double *sc, int ia, int isa, int ib, int isb,
-                                                             double *part) {
-  using C = typename CT<R>::T;
-  const int q = threadIdx.x & 63, rs = threadIdx.x >> 6;
-  const R ca = (R)(sc[(size_t)ia * kQ + q] * (isa >= 0 ? sc[(size_t)isa * kQ + q] : 1.0));
-  const R cb = HASB ? (R)(sc[(size_t)ib * kQ + q] * (isb >= 0 ? sc[(size_t)isb * kQ + q] : 1.0)) : (R)0;
-  double acc = 0;
-  constexpr int U = 4;
-  const int64_t step = (int64_t)gridDim.x * kBWaves;
-  for (int64_t r0 = (int64_t)blockIdx.x * kBWaves + rs; r0 < dim; r0 += step * U) {
-    C x[U], a[U], b[U];
+// ---------------------------------------------- block gap-lattice stencil
+// The clear rows of an N_down = 4 all-k nearest-neighbour chain sector (the
+// rows whose hops all land on fixed lattice neighbours, see the stencil
+// kernel of hv_nn.hip) for the kQ interleaved vectors.  A clear row (1, a, b,
+// c) has stabiliser order 1 and so do its hop targets, so for every q its
+// row mask and norm ratios are 1 and only the two anchor hops carry phases,
+// explist_q(L - 1) for (a - 1, b - 1, c - 1) and explist_q(1) for (a + 1,
+// b + 1, c + 1).  A lattice point of the sweep is one 128-byte chunk of a
+// row: 16 wave vectors (FP32) or 8 (FP64), the task's vector group.  A task
+// is a tile of kSA values of a by B lines swept over its slabs c0 .. c1 - 1;
+// each slab's window (B + 2 lines of kSA + 2 points) goes into a four-slot
+// LDS ring, slab c is computed from slots c - 1, c, c + 1 while the next two
+// windows are in flight in registers, as in the single-vector stencil.  A
+// row's chunk is read about (B + 2)(kSA + 2) / (B kSA) times per H.v instead
+// of once per hop that reaches it, and no configuration word or index is
+// read.  The groups of a tile are consecutive tasks of one XCD, so the four
+// (eight) chunks of a row are requested at about the same time.
+constexpr int kSA = 16;   // a-values per tile
+constexpr int kSW = kSA + 2;
+constexpr int kSRing = 4;
+
+template <class R>
+struct Chunk16;
+template <>
+struct Chunk16<float> {
+  typedef float T __attribute__((ext_vector_type(4)));  // two complex
+  static constexpr int nq = 2;
+};
+template <>
+struct Chunk16<double> {
+  typedef double T __attribute__((ext_vector_type(2)));  // one complex
+  static constexpr int nq = 1;
+};
+
+struct BlockStencilArgs {
+  const int4 *lines;  // line table of the sector ((L + 1) c + b: first row, lowest a, highest a + 1)
+  const int4 *tasks;  // a0, b0, c0, c1 | group << 16 (c1 <= c0: nothing to do)
+  int ntasks;
+};
+
+template <class R, int B>
+constexpr size_t block_stencil_lds(int slabs) {
+  return (size_t)kSRing * (B + 2) * kSW * 128 + (size_t)(slabs + 2) * (B + 2) * sizeof(int4);
+}
+
+template <class R, int B, int NT>
+__global__ __launch_bounds__(NT) void k_hv_block_stencil(BlockHvArgs a, BlockStencilArgs sa) {
+  using V = typename Chunk16<R>::T;
+  constexpr int NQ = Chunk16<R>::nq;
+  constexpr int GQ = 8 * NQ;        // wave vectors per group (one 128-byte chunk of a row)
+  constexpr int RV = kQ / NQ;       // 16-byte chunks per row
+  constexpr int NL = B + 2;         // window lines
+  constexpr int NWIN = NL * kSW;    // window points
+  constexpr int SLOT = NWIN * 8;    // 16-byte chunks per ring slot
+  constexpr int PR = NT / 8;              // window points per round (8 chunks each)
+  constexpr int PER = (NWIN + PR - 1) / PR;  // window points per thread
+  constexpr int NOUT8 = B * kSA * 8;     // output chunks per slab
+  static_assert(NOUT8 % NT == 0, "whole rounds of output chunks");
+  static_assert(kSRing * SLOT * 16 >= NT * NQ * 2 * 8, "the partial-sum area must fit in the ring");
+  extern __shared__ __align__(16) unsigned char smem[];
+  V *s_ring = (V *)smem;
+  int4 *s_tl = (int4 *)(smem + (size_t)kSRing * SLOT * sizeof(V));
+  __shared__ R s_qt[6][kQ];         // per q: scale, row mask, explist_q(1) re, im, explist_q(L - 1) re, im
+  __shared__ double s_acc[2][kQ];   // per q: the block's sum conj(x) y
+  const int L = a.L;
+  const int tid = threadIdx.x, sub = tid & 7, ps = tid >> 3;
+  for (int q = tid; q < kQ; q += NT) {
+    s_qt[0][q] = a.iscale >= 0 ? (R)a.sc[(size_t)a.iscale * kQ + q] : (R)1;
+    s_qt[1][q] = (R)a.rowmask[1 * kQ + q];
+    const R *e = (const R *)a.eph;
+    s_qt[2][q] = e[2 * ((size_t)1 * kQ + q)];
+    s_qt[3][q] = e[2 * ((size_t)1 * kQ + q) + 1];
+    s_qt[4][q] = e[2 * ((size_t)(L - 1) * kQ + q)];
+    s_qt[5][q] = e[2 * ((size_t)(L - 1) * kQ + q) + 1];
+    s_acc[0][q] = 0.0;
+    s_acc[1][q] = 0.0;
+  }
+  const R jxy2 = (R)a.bt[0], jz4 = (R)a.bt[1], shift = (R)a.shift;
+  const V *__restrict__ x = (const V *)a.x;
+  V *__restrict__ y = (V *)a.y;
+  const V zero = (V)(R)0;
+  V preX[PER], preY[PER];
+  // window points of slab te for vector group g
+  auto fetch = [&](V (&pre)[PER], int a0, int g, const int4 *te) __attribute__((always_inline)) {
 #pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const int64_t r = r0 + k * step;
-      if (r < dim) {
-        x[k] = ld_nt(&w[r * kQ + q]);
-        a[k] = ld_nt(&u[r * kQ + q]);
-        if (HASB) b[k] = ld_nt(&up[r * kQ + q]);
+    for (int m = 0; m < PER; ++m) {
+      const int p = ps + PR * m;
+      V v = zero;
+      if (p < NWIN) {
+        const int j = p / kSW, i = p - j * kSW;
+        const int4 e = te[j];
+        const int av = a0 - 1 + i;
+        if (av >= e.y && av < e.z) v = x[(int64_t)(e.x + (av - e.y)) * RV + g * 8 + sub];
+      }
+      pre[m] = v;
+    }
+  };
+  auto store = [&](const V (&pre)[PER], int sl) __attribute__((always_inline)) {
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const int p = ps + PR * m;
+      if (p < NWIN) s_ring[(size_t)sl * SLOT + p * 8 + sub] = pre[m];
+    }
+  };
+  // this thread's wave vectors of group g
+  R qs[NQ], qm[NQ], e1r[NQ], e1i[NQ], eLr[NQ], eLi[NQ];
+  double dre[NQ], dim_[NQ];
+  auto compute = [&](int a0, int b0, int g, int c, const int4 *te0) __attribute__((always_inline)) {
+    const V *r0 = s_ring + (size_t)(c % kSRing) * SLOT;
+    const V *rm = s_ring + (size_t)((c + kSRing - 1) % kSRing) * SLOT;
+    const V *rp = s_ring + (size_t)((c + 1) % kSRing) * SLOT;
+    const int W = L + 1 - c;
+#pragma unroll
+    for (int pb = 0; pb < NOUT8; pb += NT) {
+      const int pos = (pb + tid) >> 3;
+      const int j = 1 + pos / kSA, ai = pos % kSA;  // window line, a - a0
+      const int4 e = te0[j];
+      const int av = a0 + ai, b = b0 - 1 + j;
+      const int g1 = av - 1, g2 = b - av, g3 = c - b;
+      if (av >= e.y && av < e.z && max(g1, max(g2, g3)) <= W - 3) {
+        const int r = e.x + (av - e.y);
+        const int x0 = (j * kSW + ai + 1) * 8 + sub;
+        V sm = rp[x0];                       // (a, b, c + 1)
+        const V al = rp[x0 + (kSW + 1) * 8];  // anchor to site L: (a + 1, b + 1, c + 1), explist(1)
+        V ar = zero;
+        int nanti = 2;
+        if (g2 > 1) {
+          sm += r0[x0 + 8];        // (a + 1, b, c)
+          sm += r0[x0 - kSW * 8];  // (a, b - 1, c)
+          nanti += 2;
+        }
+        if (g1 > 1) {
+          sm += r0[x0 - 8];                 // (a - 1, b, c)
+          ar = rm[x0 - (kSW + 1) * 8];      // anchor to site 2: (a - 1, b - 1, c - 1), explist(L - 1)
+          nanti += 2;
+        }
+        if (g3 > 1) {
+          sm += r0[x0 + kSW * 8];  // (a, b + 1, c)
+          sm += rm[x0];            // (a, b, c - 1)
+          nanti += 2;
+        }
+        const V xv = r0[x0];
+        const R dg = shift - (R)2 * jz4 * (R)nanti;
+        V out;
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+          const R sx = sm[2 * k] + e1r[k] * al[2 * k] - e1i[k] * al[2 * k + 1] + eLr[k] * ar[2 * k] -
+                       eLi[k] * ar[2 * k + 1];
+          const R sy = sm[2 * k + 1] + e1r[k] * al[2 * k + 1] + e1i[k] * al[2 * k] + eLr[k] * ar[2 * k + 1] +
+                       eLi[k] * ar[2 * k];
+          const R xr = xv[2 * k] * qs[k], xi = xv[2 * k + 1] * qs[k];
+          const R orr = qm[k] * (dg * xr + jxy2 * qs[k] * sx);
+          const R oi = qm[k] * (dg * xi + jxy2 * qs[k] * sy);
+          out[2 * k] = orr;
+          out[2 * k + 1] = oi;
+          dre[k] += (double)xr * (double)orr + (double)xi * (double)oi;
+          dim_[k] += (double)xr * (double)oi - (double)xi * (double)orr;
+        }
+        __builtin_nontemporal_store(out, &y[(int64_t)r * RV + g * 8 + sub]);
       }
     }
+  };
+  __syncthreads();
+  for (int t = blockIdx.x; t < sa.ntasks; t += gridDim.x) {
+    const int4 task = sa.tasks[t];
+    const int a0 = task.x, b0 = task.y, c0 = task.z, c1 = task.w & 0xffff, g = task.w >> 16;
+    if (c1 <= c0) continue;
 #pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const int64_t r = r0 + k * step;
-      if (r < dim) {
-        x[k].x -= ca * a[k].x;
-        x[k].y -= ca * a[k].y;
-        if (HASB) {
-          x[k].x -= cb * b[k].x;
-          x[k].y -= cb * b[k].y;
-        }
-        st_nt(&w[r * kQ + q], x[k]);
-        acc += (double)x[k].x * (double)x[k].x + (double)x[k].y * (double)x[k].y;
+    for (int k = 0; k < NQ; ++k) {
+      const int q = g * GQ + sub * NQ + k;
+      qs[k] = s_qt[0][q];
+      qm[k] = s_qt[1][q];
+      e1r[k] = s_qt[2][q];
+      e1i[k] = s_qt[3][q];
+      eLr[k] = s_qt[4][q];
+      eLi[k] = s_qt[5][q];
+      dre[k] = 0.0;
+      dim_[k] = 0.0;
+    }
+    // line entries of slabs c0 - 1 .. c1 (entry (cs, j) at s_tl[(cs - c0 + 1) NL + j])
+    const int nent = (c1 - c0 + 2) * NL;
+    for (int q = tid; q < nent; q += NT) {
+      const int cs = c0 - 1 + q / NL, b = b0 - 1 + q % NL;
+      int4 e = make_int4(0, 0, 0, 0);
+      if (cs >= 0 && cs <= L && b >= 0 && b <= L) e = sa.lines[(size_t)cs * (L + 1) + b];
+      s_tl[q] = e;
+    }
+    __syncthreads();
+    auto te = [&](int cs) { return s_tl + (cs - c0 + 1) * NL; };
+    {
+      V preZ[PER];
+      fetch(preX, a0, g, te(c0 - 1));
+      fetch(preY, a0, g, te(c0));
+      fetch(preZ, a0, g, te(c0 + 1));
+      store(preX, (c0 + kSRing - 1) % kSRing);
+      store(preY, c0 % kSRing);
+      store(preZ, (c0 + 1) % kSRing);
+    }
+    if (c0 + 2 <= c1) fetch(preX, a0, g, te(c0 + 2));
+    __syncthreads();
+    for (int c = c0; c < c1; c += 2) {
+      if (c + 3 <= c1) fetch(preY, a0, g, te(c + 3));
+      compute(a0, b0, g, c, te(c));
+      if (c + 2 <= c1) store(preX, (c + 2) % kSRing);
+      __syncthreads();
+      if (c + 1 < c1) {
+        if (c + 4 <= c1) fetch(preX, a0, g, te(c + 4));
+        compute(a0, b0, g, c + 1, te(c + 1));
+        if (c + 3 <= c1) store(preY, (c + 3) % kSRing);
+        __syncthreads();
       }
+    }
+    // the task's partial sums into the block's per-q totals, in a fixed order
+    // (the ring is free until the next task's prologue)
+    double *s_red = (double *)smem;  // [NT][NQ][2]
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      s_red[(tid * NQ + k) * 2] = dre[k];
+      s_red[(tid * NQ + k) * 2 + 1] = dim_[k];
+    }
+    __syncthreads();
+    if (tid < GQ) {
+      const int sb = tid / NQ, k = tid % NQ;
+      double sr = 0.0, si = 0.0;
+      for (int p = 0; p < NT / 8; ++p) {
+        sr += s_red[((p * 8 + sb) * NQ + k) * 2];
+        si += s_red[((p * 8 + sb) * NQ + k) * 2 + 1];
+      }
+      s_acc[0][g * GQ + tid] += sr;
+      s_acc[1][g * GQ + tid] += si;
+    }
+    __syncthreads();
+  }
+  if (a.dot_partials) {
+    for (int q = tid; q < kQ; q += NT) {
+      a.dot_partials[2 * ((size_t)blockIdx.x * kQ + q)] = s_acc[0][q];
+      a.dot_partials[2 * ((size_t)blockIdx.x * kQ + q) + 1] = s_acc[1][q];
     }
   }
-  __shared__ double red[kBWaves][kQ];
-  red[rs][q] = acc;
+}
+
+using BSFn = void (*)(BlockHvArgs, BlockStencilArgs);
+
+// Block stencil kernel for B lines per tile and NT threads per block (256:
+// B even, 2..14; 512: B = 4, 8, 12), or nullptr.
+template <class R>
+BSFn block_stencil_kernel_r(int B, int NT, size_t *lds, int slabs) {
+#define BSK(b, nt)                                  \
+  if (B == b && NT == nt) {                         \
+    *lds = block_stencil_lds<R, b>(slabs);          \
+    return k_hv_block_stencil<R, b, nt>;            \
+  }
+  BSK(2, 256) BSK(4, 256) BSK(6, 256) BSK(8, 256) BSK(10, 256) BSK(12, 256) BSK(14, 256)
+  BSK(4, 512) BSK(8, 512) BSK(12, 512)
+#undef BSK
+  return nullptr;
+}
+
+BSFn block_stencil_kernel(int prec, int B, int NT, size_t *lds, int slabs) {
+  return prec == QS3X_FP64 ? block_stencil_kernel_r<double>(B, NT, lds, slabs)
+                           : block_stencil_kernel_r<float>(B, NT, lds, slabs);
+}
+
+// --------------------------------------------------------- block update
+// w_q -= a_q s_q u_q + b_q s'_q up_q (b term when HASB); part[block][q] = |w_q|^2.
+// The vectors are n 16-byte chunks ([row][kQ] complex: 2 wave vectors per
+// chunk in FP32, 1 in FP64), swept in contiguous tiles of kBThreads x kU
+// chunks per block as the single-vector update does.  A row is a whole
+// number of chunks and a tile a whole number of rows' worth of lanes, so a
+// thread always holds the same chunk column, i.e. the same wave vectors.
+constexpr int kBU = 4;
+template <class R, bool HASB>
+__global__ __launch_bounds__(kBThreads) void k_update3_block(typename Chunk16<R>::T *w, const typename Chunk16<R>::T *u,
+                                                             const typename Chunk16<R>::T *up, int64_t n,
+                                                             const double *sc, int ia, int isa, int ib, int isb,
+                                                             double *part) {
+  using V = typename Chunk16<R>::T;
+  constexpr int NQ = Chunk16<R>::nq;
+  constexpr int RV = kQ / NQ;  // chunks per row
+  static_assert(kBThreads % RV == 0, "a thread keeps its chunk column");
+  const int col = threadIdx.x % RV;
+  R ca[NQ], cb[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    const int q = col * NQ + k;
+    ca[k] = (R)(sc[(size_t)ia * kQ + q] * (isa >= 0 ? sc[(size_t)isa * kQ + q] : 1.0));
+    cb[k] = HASB ? (R)(sc[(size_t)ib * kQ + q] * (isb >= 0 ? sc[(size_t)isb * kQ + q] : 1.0)) : (R)0;
+  }
+  double acc[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) acc[k] = 0.0;
+  auto one = [&](V x, V a, V b, int64_t i) __attribute__((always_inline)) {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      x[2 * k] -= ca[k] * a[2 * k];
+      x[2 * k + 1] -= ca[k] * a[2 * k + 1];
+      if (HASB) {
+        x[2 * k] -= cb[k] * b[2 * k];
+        x[2 * k + 1] -= cb[k] * b[2 * k + 1];
+      }
+      acc[k] += (double)x[2 * k] * (double)x[2 * k] + (double)x[2 * k + 1] * (double)x[2 * k + 1];
+    }
+    __builtin_nontemporal_store(x, &w[i]);
+  };
+  constexpr int TILE = kBThreads * kBU;
+  const int64_t nfull = n / TILE;
+  for (int64_t t = blockIdx.x; t < nfull; t += gridDim.x) {
+    const int64_t b0 = t * TILE + threadIdx.x;
+    V x[kBU], a[kBU], b[kBU];
+#pragma unroll
+    for (int q = 0; q < kBU; ++q) {
+      x[q] = __builtin_nontemporal_load(&w[b0 + q * kBThreads]);
+      a[q] = __builtin_nontemporal_load(&u[b0 + q * kBThreads]);
+      if (HASB) b[q] = __builtin_nontemporal_load(&up[b0 + q * kBThreads]);
+      else b[q] = a[q];
+    }
+#pragma unroll
+    for (int q = 0; q < kBU; ++q) one(x[q], a[q], b[q], b0 + q * kBThreads);
+  }
+  if (nfull * TILE < n && (int64_t)blockIdx.x == nfull % gridDim.x)
+    for (int64_t i = nfull * TILE + threadIdx.x; i < n; i += kBThreads) one(w[i], u[i], HASB ? up[i] : u[i], i);
+  __shared__ double red[kBThreads / RV][kQ];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) red[threadIdx.x / RV][col * NQ + k] = acc[k];
   __syncthreads();
-  if (rs == 0) {
-    double s = 0;
-    for (int k = 0; k < kBWaves; ++k) s += red[k][q];
-    part[2 * ((size_t)blockIdx.x * kQ + q)] = s;
-    part[2 * ((size_t)blockIdx.x * kQ + q) + 1] = 0.0;
+  if (threadIdx.x < kQ) {
+    double t = 0;
+    for (int k = 0; k < kBThreads / RV; ++k) t += red[k][threadIdx.x];
+    part[2 * ((size_t)blockIdx.x * kQ + threadIdx.x)] = t;
+    part[2 * ((size_t)blockIdx.x * kQ + threadIdx.x) + 1] = 0.0;
   }
 }
 
@@ -786,6 +1077,47 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
     a.eph = d_ephR.p;
     a.dot_partials = part.p;
     a.nxcd = options().hv_xcd > 0 ? options().hv_xcd : 8;  // per-XCD ranges unless set
+    // Clear rows through the block stencil where it applies, the others
+    // (the sector's non-clear rows, about 1 % at L = 1000) through the
+    // per-row block kernel; their partial sums follow the stencil's.
+    BSFn hvs = nullptr;
+    size_t lds_s = 0;
+    int gridSt = 0, gridSl = 0, ntasks = 0, ntS = 256;
+    DevBuf<int4> st_tasks;
+    BlockStencilArgs sarg{};
+    if (options().block_stencil && stencil_applies(&tgt)) {
+      // defaults: the fastest measured at L = 1000, N_down = 4 in both
+      // precisions (12 lines, 512 threads, 32 slabs: FP32 9.9 ms, FP64 20.5 ms
+      // per launch, against 12.3 and 25.2 ms with 6 lines, 256 threads, 8 slabs)
+      const int B = options().block_stencil_lines > 0 ? options().block_stencil_lines : 12;
+      const int slabs = options().block_stencil_slabs > 0 ? options().block_stencil_slabs : 32;
+      ntS = options().block_stencil_threads > 0 ? options().block_stencil_threads : 512;
+      hvs = block_stencil_kernel(prec, B, ntS, &lds_s, slabs);
+      QS3X_REQUIRE(hvs, "qs3x_dsf_sminus_batch: no block stencil kernel for these lines per tile and threads "
+                        "(256 threads: 2..14 lines, even; 512: 4, 8 or 12)");
+      QS3X_REQUIRE(lds_s <= 156 * 1024, "qs3x_dsf_sminus_batch: block stencil tile over 156 KB of LDS "
+                                        "(fewer lines or slabs)");
+      std::vector<int4> lines;
+      stencil_tables(&tgt, &lines);
+      const int groups = prec == QS3X_FP64 ? kQ / 8 : kQ / 16;
+      std::vector<int4> tasks = stencil_tasks(lines, L, kSA, B, slabs, groups);
+      ntasks = (int)tasks.size();
+      st_tasks.alloc(std::max<size_t>(tasks.size(), 1));
+      if (!tasks.empty()) st_tasks.upload(tasks.data(), tasks.size());
+      if (lds_s > 64 * 1024)
+        QS3X_HIP(hipFuncSetAttribute((const void *)hvs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_s));
+      int bpc = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, hvs, ntS, lds_s) != hipSuccess || bpc < 1)
+        bpc = 1;
+      // a multiple of the 8 XCDs: block g takes tasks 8 m + g % 8, its XCD's tiles
+      gridSt = (int)std::min<int64_t>((int64_t)cus() * bpc, std::max(ntasks, 8));
+      gridSt = std::max(8, gridSt / 8 * 8);
+      gridSl = tgt.nslow > 0 ? (int)std::min<int64_t>((int64_t)cus() * nbpc, ((tgt.nslow + 63) / 64 + kBWaves - 1) / kBWaves)
+                             : 0;
+      sarg = BlockStencilArgs{tgt.d_lines.p, st_tasks.p, ntasks};
+      QS3X_REQUIRE((int64_t)(gridSt + gridSl) * kQ * 2 <= (int64_t)part.n,
+                   "qs3x_dsf_sminus_batch: partial buffer (stencil)");
+    }
     const double sztot_new = (double)L / 2.0 - (double)NT;
     const double c = (prec == QS3X_FP32) ? m->ezz : 0.0;
     hipEvent_t e0, e1;
@@ -798,29 +1130,47 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
       a.y = w;
       a.iscale = SCALE + i;
       if (i == 1) QS3X_HIP(hipEventRecord(e0, st));
-      hipLaunchKernelGGL(hv, dim3(gridH), dim3(kBThreads), lds, st, a);
-      QS3X_HIP(hipGetLastError());
-      hipLaunchKernelGGL(k_finalize_q, dim3(kQ), dim3(kBThreads), 0, st, part.p, gridH, sc.p, 1, A_REC + i,
+      int nbh = gridH;
+      if (hvs) {
+        hipLaunchKernelGGL(hvs, dim3(gridSt), dim3(ntS), lds_s, st, a, sarg);
+        QS3X_HIP(hipGetLastError());
+        nbh = gridSt;
+        if (gridSl > 0) {
+          BlockHvArgs b = a;
+          b.rows = tgt.d_slow.p;
+          b.nrows = tgt.nslow;
+          b.dot_partials = part.p + (size_t)2 * kQ * gridSt;
+          hipLaunchKernelGGL(hv, dim3(gridSl), dim3(kBThreads), lds, st, b);
+          QS3X_HIP(hipGetLastError());
+          nbh += gridSl;
+        }
+      } else {
+        hipLaunchKernelGGL(hv, dim3(gridH), dim3(kBThreads), lds, st, a);
+        QS3X_HIP(hipGetLastError());
+      }
+      hipLaunchKernelGGL(k_finalize_q, dim3(kQ), dim3(kBThreads), 0, st, part.p, nbh, sc.p, 1, A_REC + i,
                          A_OUT + i, c - rfield * sztot_new);
       if (i + 1 == steps) break;
-      if (prec == QS3X_FP64) {
-        if (i > 0)
-          hipLaunchKernelGGL((k_update3_block<double, true>), dim3(gridU), dim3(kBThreads), 0, st, (double2 *)w,
-                             (const double2 *)cur, (const double2 *)prev, dim, sc.p, A_REC + i, SCALE + i, BETA + i - 1,
-                             SCALE + i - 1, part.p);
-        else
-          hipLaunchKernelGGL((k_update3_block<double, false>), dim3(gridU), dim3(kBThreads), 0, st, (double2 *)w,
-                             (const double2 *)cur, (const double2 *)prev, dim, sc.p, A_REC + i, SCALE + i, -1, -1,
-                             part.p);
-      } else {
-        if (i > 0)
-          hipLaunchKernelGGL((k_update3_block<float, true>), dim3(gridU), dim3(kBThreads), 0, st, (float2 *)w,
-                             (const float2 *)cur, (const float2 *)prev, dim, sc.p, A_REC + i, SCALE + i, BETA + i - 1,
-                             SCALE + i - 1, part.p);
-        else
-          hipLaunchKernelGGL((k_update3_block<float, false>), dim3(gridU), dim3(kBThreads), 0, st, (float2 *)w,
-                             (const float2 *)cur, (const float2 *)prev, dim, sc.p, A_REC + i, SCALE + i, -1, -1,
-                             part.p);
+      {
+        const int64_t nch = dim * (prec == QS3X_FP64 ? kQ : kQ / 2);
+        const int ib = i > 0 ? BETA + i - 1 : -1, isb = i > 0 ? SCALE + i - 1 : -1;
+        if (prec == QS3X_FP64) {
+          using V = Chunk16<double>::T;
+          if (i > 0)
+            hipLaunchKernelGGL((k_update3_block<double, true>), dim3(gridU), dim3(kBThreads), 0, st, (V *)w,
+                               (const V *)cur, (const V *)prev, nch, sc.p, A_REC + i, SCALE + i, ib, isb, part.p);
+          else
+            hipLaunchKernelGGL((k_update3_block<double, false>), dim3(gridU), dim3(kBThreads), 0, st, (V *)w,
+                               (const V *)cur, (const V *)prev, nch, sc.p, A_REC + i, SCALE + i, ib, isb, part.p);
+        } else {
+          using V = Chunk16<float>::T;
+          if (i > 0)
+            hipLaunchKernelGGL((k_update3_block<float, true>), dim3(gridU), dim3(kBThreads), 0, st, (V *)w,
+                               (const V *)cur, (const V *)prev, nch, sc.p, A_REC + i, SCALE + i, ib, isb, part.p);
+          else
+            hipLaunchKernelGGL((k_update3_block<float, false>), dim3(gridU), dim3(kBThreads), 0, st, (V *)w,
+                               (const V *)cur, (const V *)prev, nch, sc.p, A_REC + i, SCALE + i, ib, isb, part.p);
+        }
       }
       QS3X_HIP(hipGetLastError());
       hipLaunchKernelGGL(k_finalize_q, dim3(kQ), dim3(kBThreads), 0, st, part.p, gridU, sc.p, 2, BETA + i,
@@ -849,8 +1199,8 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
       stats[3] = steps > 1 ? hv_ms_total / (steps - 1) : 0.0;  // ms per block step (H.v + update), steps 2..
       stats[4] = (double)dim;
       stats[5] = (double)steps;
-      stats[6] = 0.0;
-      stats[7] = 0.0;
+      stats[6] = hvs ? 1.0 : 0.0;                        // 1: clear rows through the block stencil
+      stats[7] = hvs ? (double)tgt.nslow : 0.0;          // rows through the per-row block kernel then
     }
   });
 }

@@ -61,6 +61,14 @@ struct Options {
   // measured no gain at L = 1000 -- the stencil fills the CUs -- and slowed
   // the stencil by 4 %)
   int hv_stencil_side = 0;
+  // Batched 64-q DSF block H.v: the gap-lattice stencil for the clear rows of
+  // N_down = 4 nearest-neighbour chains (1, default) or the per-row block
+  // kernel for every row (0); its lines per tile (even, 2..14; 0: 12) and
+  // slabs per task (0: 32).
+  int block_stencil = 1;
+  int block_stencil_lines = 0;
+  int block_stencil_slabs = 0;
+  int block_stencil_threads = 0;  // threads per block of the block stencil (0: 512; 256 or 512)
   // dense eigensolver of qs3x_full_diag: 0 = QR iteration on the tridiagonal
   // form (rocSOLVER zheev, the reference's LAPACK zheev algorithm; default),
   // 1 = divide and conquer (zheevd; faster for large dim, but it has returned

@@ -759,7 +759,7 @@ constexpr int kMaxLines = 8;
 
 struct StencilArgs {
   const int4 *lines;    // [(L + 1) * c + b]: first row, lowest a, highest a + 1 (lowest = highest + 1: empty)
-  const int4 *tasks;    // a0, b0, c0, c1 (c1 <= c0: nothing to do)
+  const int4 *tasks;    // a0, b0, c0, c1 in bits 0..15 of .w (c1 <= c0: nothing to do)
   int ntasks;
   int B;                // lines per tile
 };
@@ -898,7 +898,7 @@ __global__ __launch_bounds__(kThreads) void k_hv_stencil4(HvArgs a, StencilArgs 
   __syncthreads();
   for (int t = blockIdx.x; t < sa.ntasks; t += gridDim.x) {
     const int4 task = sa.tasks[t];
-    const int a0 = task.x, b0 = task.y, c0 = task.z, c1 = task.w;
+    const int a0 = task.x, b0 = task.y, c0 = task.z, c1 = task.w & 0xffff;
     if (c1 <= c0) continue;
     // line entries of slabs c0 - 1 .. c1 (entry (cs, j) at s_tl[(cs - c0 + 1) NL + j])
     const int nent = (c1 - c0 + 2) * NL;
@@ -1102,24 +1102,26 @@ int stencil_slabs(int prec) {
   return prec == QS3X_FP64 ? 16 : 8;
 }
 
-// Plan for one precision: tiles of kSegA values of a by B lines, each swept
-// over the slabs where it holds rows, in tasks of up to `slabs` slabs.  Tasks
-// are dealt to the 8 XCDs by tile: task 8 m + x belongs to XCD x, whose tiles
-// are one contiguous range in (b, a) order holding about an eighth of the
-// tasks, so the halo lines and columns that neighbouring tiles share stay in
-// one L2.
-void stencil_plan(int prec, qs3x_sector::StencilPlan &pl, const std::vector<int4> &lines, int L) {
+}  // namespace
+
+// Tasks of a gap-lattice stencil sweep: tiles of segA values of a by B lines,
+// each swept over the slabs where it holds rows, in tasks of up to `slabs`
+// slabs, each task repeated for `groups` consecutive vector groups (group in
+// bits 16.. of .w, the end slab in bits 0..15).  Tasks are dealt to the 8
+// XCDs by tile: task 8 m + x belongs to XCD x, whose tiles are one contiguous
+// range in (b, a) order holding about an eighth of the tasks, so the halo
+// lines and columns that neighbouring tiles share stay in one L2, and the
+// groups of one tile run at about the same time there.
+std::vector<int4> stencil_tasks(const std::vector<int4> &lines, int L, int segA, int B, int slabs, int groups) {
   const int W1 = L + 1;
-  const int slabs = stencil_slabs(prec);
-  const int B = stencil_lines(prec);
-  const int na = (L + kSegA - 1) / kSegA + 1, nb = (L + B - 1) / B + 1;
+  const int na = (L + segA - 1) / segA + 1, nb = (L + B - 1) / B + 1;
   std::vector<int> cmin((size_t)na * nb, 0x7fffffff), cmax((size_t)na * nb, -1);
   for (int c = 0; c <= L; ++c)
     for (int b = 3; b < c; ++b) {
       const int4 e = lines[(size_t)c * W1 + b];
       if (e.z <= e.y) continue;
       const int tb = (b - 3) / B;
-      for (int ta = (e.y - 2) / kSegA; ta <= (e.z - 1 - 2) / kSegA; ++ta) {
+      for (int ta = (e.y - 2) / segA; ta <= (e.z - 1 - 2) / segA; ++ta) {
         const size_t t = (size_t)tb * na + ta;
         cmin[t] = std::min(cmin[t], c);
         cmax[t] = std::max(cmax[t], c);
@@ -1133,7 +1135,8 @@ void stencil_plan(int prec, qs3x_sector::StencilPlan &pl, const std::vector<int4
       if (cmax[t] < cmin[t]) continue;
       std::vector<int4> tl;
       for (int c0 = cmin[t]; c0 <= cmax[t]; c0 += slabs)
-        tl.push_back(make_int4(2 + ta * kSegA, 3 + tb * B, c0, std::min(c0 + slabs, cmax[t] + 1)));
+        for (int g = 0; g < groups; ++g)
+          tl.push_back(make_int4(2 + ta * segA, 3 + tb * B, c0, std::min(c0 + slabs, cmax[t] + 1) | (g << 16)));
       total += tl.size();
       per_tile.push_back(tl);
     }
@@ -1149,12 +1152,13 @@ void stencil_plan(int prec, qs3x_sector::StencilPlan &pl, const std::vector<int4
       gt[x].push_back(&tl);
       acc += tl.size();
     }
+    // depth d of every tile in turn, keeping the groups of one task together
     for (int g = 0; g < nx; ++g) {
       size_t depth = 0;
       for (auto *tl : gt[g]) depth = std::max(depth, tl->size());
-      for (size_t d = 0; d < depth; ++d)
+      for (size_t d = 0; d < depth; d += groups)
         for (auto *tl : gt[g])
-          if (d < tl->size()) grp[g].push_back((*tl)[d]);
+          for (int k = 0; k < groups && d + k < tl->size(); ++k) grp[g].push_back((*tl)[d + k]);
     }
   }
   size_t mx = 0;
@@ -1162,6 +1166,16 @@ void stencil_plan(int prec, qs3x_sector::StencilPlan &pl, const std::vector<int4
   std::vector<int4> tasks(mx * nx, make_int4(0, 0, 0, 0));
   for (int x = 0; x < nx; ++x)
     for (size_t m = 0; m < grp[x].size(); ++m) tasks[m * nx + x] = grp[x][m];
+  return tasks;
+}
+
+namespace {
+
+// Plan for one precision of the single-vector stencil (groups = 1).
+void stencil_plan(int prec, qs3x_sector::StencilPlan &pl, const std::vector<int4> &lines, int L) {
+  const int slabs = stencil_slabs(prec);
+  const int B = stencil_lines(prec);
+  std::vector<int4> tasks = stencil_tasks(lines, L, kSegA, B, slabs, 1);
   const size_t es = prec == QS3X_FP64 ? 16 : 8;
   pl.nlines = 0;
   for (const int4 &e : lines) pl.nlines += e.z > e.y ? 1 : 0;
@@ -1199,33 +1213,44 @@ void stencil_slow_list(qs3x_sector *s) {
 
 }  // namespace
 
+bool stencil_applies(const qs3x_sector *s) {
+  return s->nod == 4 && s->chain && s->mode == kIndexAnchored && s->model->nn_chain &&
+         s->model->bt_jxy2.size() == 1 && s->d_rclass.p != nullptr && s->dim < ((int64_t)1 << 31) &&
+         binom_host(s->model->NOS - 2, 3) < ((int64_t)1 << 31) && s->model->NOS < 32768;
+}
+
+void stencil_tables(qs3x_sector *s, std::vector<int4> *lines_out) {
+  const int L = s->model->NOS;
+  if (!s->d_lines.p) {
+    const size_t n = (size_t)(L + 1) * (L + 1);
+    s->d_lines.alloc(n);
+    QS3X_HIP(hipMemsetAsync(s->d_lines.p, 0, n * sizeof(int4), stream()));
+    hipLaunchKernelGGL(k_stencil_lines, dim3((unsigned)((s->dim + 255) / 256)), dim3(256), 0, stream(),
+                       s->d_cfg.p, s->dim, s->bits, L, s->d_lines.p);
+    QS3X_HIP(hipGetLastError());
+    std::vector<int32_t> b32((size_t)3 * (L - 1));
+    for (int q = 1; q <= 3; ++q)
+      for (int x = 0; x < L - 1; ++x) b32[(size_t)(q - 1) * (L - 1) + x] = (int32_t)binom_host(x, q);
+    s->d_binom32.alloc(b32.size());
+    s->d_binom32.upload(b32.data(), b32.size());
+    stencil_slow_list(s);
+  }
+  if (lines_out) {
+    lines_out->assign((size_t)(L + 1) * (L + 1), make_int4(0, 0, 0, 0));
+    s->d_lines.download(lines_out->data(), lines_out->size());
+  }
+}
+
 bool launch_hv_stencil(qs3x_sector *s, int prec, const HvArgs &a, hipStream_t st, int *grid_out, size_t *lds_out) {
-  if (!options().hv_stencil || s->nod != 4 || !s->chain || s->mode != kIndexAnchored || !s->model->nn_chain ||
-      s->model->bt_jxy2.size() != 1 || s->d_rclass.p == nullptr || s->dim >= ((int64_t)1 << 31) ||
-      binom_host(s->model->NOS - 2, 3) >= ((int64_t)1 << 31))
-    return false;
+  if (!options().hv_stencil || !stencil_applies(s)) return false;
   const int L = s->model->NOS;
   qs3x_sector::StencilPlan &pl = s->stencil[prec == QS3X_FP64 ? 0 : 1];
   if (pl.tried && (pl.slabs != stencil_slabs(prec) || pl.B != stencil_lines(prec))) pl.tried = false;  // options changed
   if (!pl.tried) {
     pl.tried = true;
     pl.ok = false;
-    if (!s->d_lines.p) {
-      const size_t n = (size_t)(L + 1) * (L + 1);
-      s->d_lines.alloc(n);
-      QS3X_HIP(hipMemsetAsync(s->d_lines.p, 0, n * sizeof(int4), stream()));
-      hipLaunchKernelGGL(k_stencil_lines, dim3((unsigned)((s->dim + 255) / 256)), dim3(256), 0, stream(),
-                         s->d_cfg.p, s->dim, s->bits, L, s->d_lines.p);
-      QS3X_HIP(hipGetLastError());
-      std::vector<int32_t> b32((size_t)3 * (L - 1));
-      for (int q = 1; q <= 3; ++q)
-        for (int x = 0; x < L - 1; ++x) b32[(size_t)(q - 1) * (L - 1) + x] = (int32_t)binom_host(x, q);
-      s->d_binom32.alloc(b32.size());
-      s->d_binom32.upload(b32.data(), b32.size());
-      stencil_slow_list(s);
-    }
-    std::vector<int4> lines((size_t)(L + 1) * (L + 1));
-    s->d_lines.download(lines.data(), lines.size());
+    std::vector<int4> lines;
+    stencil_tables(s, &lines);
     stencil_plan(prec, pl, lines, L);
     // the plan's tables went up on the library stream; the launch below may
     // be on the caller's stream, which nothing orders after it

@@ -183,6 +183,15 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
                double *dot_partials, int *nblocks_out, hipStream_t st);
 int64_t count_nnz(qs3x_sector *s);
 
+// Gap-lattice stencil (hv_nn.hip): whether a sector qualifies (anchored
+// nearest-neighbour chain with uniform bonds and N_down = 4); its line table,
+// binomials and non-clear row list (built once per sector; with lines_out the
+// line table is also copied to the host); and the XCD-dealt task list of a
+// sweep (see the definition).
+bool stencil_applies(const qs3x_sector *s);
+void stencil_tables(qs3x_sector *s, std::vector<int4> *lines_out);
+std::vector<int4> stencil_tasks(const std::vector<int4> &lines, int L, int segA, int B, int slabs, int groups);
+
 }  // namespace qs3x
 
 // ------------------------------------------------------------ device utils

@@ -415,8 +415,9 @@ def dsf_sminus_batch(gs, K, pos, psi, itr_dsf, rfield=0.0, prec=FP32):
     stats = np.zeros(8)
     _check(_lib.qs3x_dsf_sminus_batch(gs._h, nq, _p(Ks), _p(ps), _p(v), int(itr_dsf), float(rfield), int(prec),
                                       _p(alpha), _p(beta2), _p(norm1), _p(stats)))
-    keys = ["seconds_setup", "seconds_sminus", "seconds_recurrence", "ms_per_block_step", "dim_all_k", "steps"]
-    return alpha, beta2, norm1, dict(zip(keys, stats[:6].tolist()))
+    keys = ["seconds_setup", "seconds_sminus", "seconds_recurrence", "ms_per_block_step", "dim_all_k", "steps",
+            "block_stencil", "rows_per_row_kernel"]
+    return alpha, beta2, norm1, dict(zip(keys, stats[:8].tolist()))
 
 
 def dsf_analysis(alpha, beta2, norm1, E0, omega, eta=0.04, reference_quirk=True):

@@ -180,3 +180,48 @@ def test_lanczos_group_long_chain():
         e1, _, _ = q3.lanczos_routines(s, maxitr=400, minitr=10, itrint=5, lnc_ene_conv=1e-12, prec=q3.FP64, seed=5,
                                        want_vector=False)
         assert abs(e - e1) <= 1e-9 * max(1.0, abs(e1)), (e, e1)
+
+
+@pytest.mark.parametrize("L,steps", [(40, 30), (120, 20), (300, 12)])
+@pytest.mark.parametrize("prec_name", ["fp64", "fp32"])
+def test_block_stencil_matches_per_row_block_kernel(L, steps, prec_name):
+    """The block gap-lattice stencil (clear rows of the N_down = 4 all-k
+    sector) plus the per-row block kernel (the other rows) against the
+    per-row block kernel on every row: same recurrence
+    (get_expectation_values.f90:481-592) for all 64 q.  FP64 coefficients to
+    1e-11 over the first steps; FP32 to the FP32 bar of this file."""
+    print(f"[gpu] block stencil vs per-row block kernel L={L} {prec_name}", flush=True)
+    q3 = _q3()
+    prec = q3.FP64 if prec_name == "fp64" else q3.FP32
+    gm = q3.Model(L, L, 1, 1, q3.chain_bonds(L, [(1, 1.0, 0.8)]))
+    gs = q3.Sector.build(gm, 3, (0.0, 0.0, 0.0))
+    E0, psi, _ = q3.lanczos_routines(gs, maxitr=2000, lnc_ene_conv=1e-13, prec=q3.FP64, seed=5)
+    pos = np.array([[i, 0.0, 0.0] for i in range(L)], dtype=float)
+    Ks = [int(round(L * m / 64)) % L for m in range(64)]
+    try:
+        q3.set_option("block_stencil", 0)
+        a0, b0, n0, st0 = q3.dsf_sminus_batch(gs, Ks, pos, psi, steps, prec=prec)
+    finally:
+        q3.set_option("block_stencil", 1)
+    a1, b1, n1, st1 = q3.dsf_sminus_batch(gs, Ks, pos, psi, steps, prec=prec)
+    # a second tile shape: 6 lines per tile, 256 threads, 8 slabs per task
+    try:
+        q3.set_option("block_stencil_lines", 6)
+        q3.set_option("block_stencil_threads", 256)
+        q3.set_option("block_stencil_slabs", 8)
+        a2, b2, n2, st2 = q3.dsf_sminus_batch(gs, Ks, pos, psi, steps, prec=prec)
+    finally:
+        for o in ("block_stencil_lines", "block_stencil_threads", "block_stencil_slabs"):
+            q3.set_option(o, 0)
+    assert st0["block_stencil"] == 0 and st1["block_stencil"] == 1 and st2["block_stencil"] == 1
+    assert 0 < st1["rows_per_row_kernel"] < st1["dim_all_k"]
+    assert np.array_equal(n0, n1)
+    k = 6 if prec == q3.FP64 else 4
+    tol_a, tol_b = (1e-11, 1e-11) if prec == q3.FP64 else (1e-6, 1e-4)
+    for j in range(64):
+        if n0[j] <= 1e-20:
+            continue
+        for a_, b_ in ((a1, b1), (a2, b2)):
+            ea = np.max(np.abs(a_[j, :k] - a0[j, :k])) / max(1.0, np.max(np.abs(a0[j, :k])))
+            eb = np.max(np.abs(b_[j, :k] - b0[j, :k])) / max(1e-300, np.max(np.abs(b0[j, :k])))
+            assert ea <= tol_a and eb <= tol_b, (j, Ks[j], ea, eb)
