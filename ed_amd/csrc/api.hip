@@ -107,14 +107,21 @@ int qs3x_set_option(const char *name, double value) {
       QS3X_REQUIRE(value == 0 || value == 1, "hv_stencil must be 0 or 1");
       options().hv_stencil = (int)value;
     } else if (n == "hv_stencil_lines") {
-      QS3X_REQUIRE(value >= 0 && value <= 8, "hv_stencil_lines must be 0..8");
+      QS3X_REQUIRE(value == 0 || value == 2 || value == 4 || value == 6 || value == 8,
+                   "hv_stencil_lines must be 0 (per precision), 2, 4, 6 or 8");
       options().hv_stencil_lines = (int)value;
     } else if (n == "hv_stencil_slabs") {
       QS3X_REQUIRE(value >= 0 && value <= 1024, "hv_stencil_slabs must be 0..1024");
       options().hv_stencil_slabs = (int)value;
+    } else if (n == "hv_stencil_depth") {
+      QS3X_REQUIRE(value == 0 || value == 2 || value == 3, "hv_stencil_depth must be 0 (per precision), 2 or 3");
+      options().hv_stencil_depth = (int)value;
     } else if (n == "hv_stencil_side") {
       QS3X_REQUIRE(value == 0 || value == 1, "hv_stencil_side must be 0 or 1");
       options().hv_stencil_side = (int)value;
+    } else if (n == "lanczos_fused") {
+      QS3X_REQUIRE(value == 0 || value == 1 || value == 2, "lanczos_fused must be 0, 1 (FP64) or 2 (both)");
+      options().lanczos_fused = (int)value;
     } else if (n == "block_stencil") {
       QS3X_REQUIRE(value == 0 || value == 1, "block_stencil must be 0 or 1");
       options().block_stencil = (int)value;

@@ -19,6 +19,10 @@ void vec_copy_scale(int pd, void *dst, int ps, const void *src, int64_t n, const
 int vec_dot(int px, const void *x, int py, const void *y, int64_t n, double *part, hipStream_t st);
 // see k_finalize modes in blas.hip
 void finalize(const double *part, int nb, double *sc, int mode, int i0, int i1, double shift, hipStream_t st);
+// alpha and beta of a fused Lanczos step from its 4 partial sums per block
+// (k_finalize_fused in blas.hip)
+void finalize_fused(const double *part, int nb, double *sc, int ia, int io, int ib, int is, double shift,
+                    hipStream_t st);
 int vec_update3(int prec, void *w, const void *u, const void *up, int64_t n, const double *sc, int ia, int isa,
                 int ib, int isb, double *part, hipStream_t st);
 void vec_axpy_acc(int prec, double *psi, const void *u, int64_t n, double z, const double *sc, int is,
@@ -54,8 +58,9 @@ struct Options {
   // Gap-lattice stencil H.v (nearest-neighbour chains, N_down = 4, uniform
   // bonds): 1 = use it where it applies (default), 0 = never.
   int hv_stencil = 1;
-  int hv_stencil_lines = 0;   // n3-lines per tile of the stencil kernel (0: per precision, 1..8)
+  int hv_stencil_lines = 0;   // n3-lines per tile of the stencil kernel (0: per precision; 2, 4, 6 or 8)
   int hv_stencil_slabs = 0;   // n4-slabs per stencil task (0: per precision)
+  int hv_stencil_depth = 0;   // slab windows in flight per block (0: per precision; 2 or 3)
   // stencil: key-search rows on a side stream, overlapping the stencil launch
   // (1), or after it on the caller's stream (0, the default: the overlap
   // measured no gain at L = 1000 -- the stencil fills the CUs -- and slowed
@@ -65,6 +70,15 @@ struct Options {
   // N_down = 4 nearest-neighbour chains (1, default) or the per-row block
   // kernel for every row (0); its lines per tile (even, 2..14; 0: 12) and
   // slabs per task (0: 32).
+  // Plain Lanczos on stencil sectors: the fused step (H.v and the three-term
+  // update in one pass over the two previous vectors) in FP64 only (1,
+  // default), in both precisions (2), or never (0: the H.v and update
+  // kernels).  At L = 1000, N_down = 4 the fused step is faster in FP64
+  // (0.736 against 0.784 ms per step) and slower in FP32 (0.464 against
+  // 0.421 ms: the FP32 stencil is bound by its VALU work and memory latency,
+  // and the fused window reads two vectors where the FP32 update streams
+  // them at 6.6 TB/s).
+  int lanczos_fused = 1;
   int block_stencil = 1;
   int block_stencil_lines = 0;
   int block_stencil_slabs = 0;

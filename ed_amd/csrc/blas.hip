@@ -172,6 +172,31 @@ __global__ __launch_bounds__(kVecThreads) void k_finalize(const double *part, in
   }
 }
 
+// Fused Lanczos step (see launch_hv_fused): part holds 4 sums per block,
+// Re<v, Hv>, |P|^2, Re<v, P>, |v|^2.  alpha = the first; the next Lanczos
+// vector is (P - alpha v) / beta with beta^2 = |P|^2 - 2 alpha Re<v, P> +
+// alpha^2 |v|^2.  Writes sc[ia] = alpha, sc[io] = alpha + shift, sc[ib] =
+// beta, sc[is] = 1 / beta (0 if beta = 0).
+__global__ __launch_bounds__(kVecThreads) void k_finalize_fused(const double *part, int nb, double *sc, int ia, int io,
+                                                                int ib, int is, double shift) {
+  double t[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t[k] += part[4 * b + k];
+  __shared__ double o[4][2];
+  block_sum2(t[0], t[1], &o[0][0], &o[1][0]);
+  block_sum2(t[2], t[3], &o[2][0], &o[3][0]);
+  if (threadIdx.x == 0) {
+    const double a = o[0][0], pp = o[1][0], vp = o[2][0], vv = o[3][0];
+    const double b2 = pp - 2.0 * a * vp + a * a * vv;
+    const double b = sqrt(b2 > 0 ? b2 : 0.0);
+    sc[ia] = a;
+    if (io >= 0) sc[io] = a + shift;
+    sc[ib] = b;
+    sc[is] = b > 0 ? 1.0 / b : 0.0;
+  }
+}
+
 // w = w - a*sa*u - b*sb*up (the b term only when HASB); part = |w|^2.
 template <class R, bool HASB>
 __global__ __launch_bounds__(kVecThreads) void k_update3(typename CT<R>::T *w, const typename CT<R>::T *u,
@@ -498,6 +523,12 @@ int vec_dot(int px, const void *x, int py, const void *y, int64_t n, double *par
     hipLaunchKernelGGL((k_dot<float, float>), GRID(n), (const float2 *)x, (const float2 *)y, n, part);
   QS3X_HIP(hipGetLastError());
   return nb;
+}
+
+void finalize_fused(const double *part, int nb, double *sc, int ia, int io, int ib, int is, double shift,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_finalize_fused, dim3(1), dim3(kVecThreads), 0, st, part, nb, sc, ia, io, ib, is, shift);
+  QS3X_HIP(hipGetLastError());
 }
 
 void finalize(const double *part, int nb, double *sc, int mode, int i0, int i1, double shift, hipStream_t st) {

@@ -793,6 +793,33 @@ static void allow_lds(KFn f, size_t lds) {
   done[f] = lds;
 }
 
+bool hv_fused_available(const qs3x_sector *s, int prec) {
+  const int f = options().lanczos_fused;
+  if (f == 0 || (f == 1 && prec != QS3X_FP64)) return false;
+  if (!options().hv_stencil || s->dim == 0 || !stencil_applies(s)) return false;
+  const qs3x_sector::HvTune &tn = s->tune[prec == QS3X_FP64 ? 0 : 1];
+  const int pick = tn.set ? tn.kernel : options().hv_kernel;
+  return pick <= 0 && s->d_bitmap.p != nullptr;
+}
+
+bool launch_hv_fused(qs3x_sector *s, int prec, void *pout, void *xout, const void *pprev, const void *vprev,
+                     const double *fsc, int fis, int fia, int fib, double *part4, int *nblocks_out, hipStream_t st) {
+  if (!hv_fused_available(s, prec)) return false;
+  HvArgs a = make_args(s, pout, pprev, nullptr, part4, prec);
+  a.ezz = 0.0;  // H - e_zz in both precisions (the caller adds e_zz to alpha)
+  a.vprev = vprev;
+  a.xout = xout;
+  a.fsc = fsc;
+  a.fis = fis;
+  a.fia = fia;
+  a.fib = fib;
+  int g = 0;
+  if (!launch_hv_stencil_fused(s, prec, a, st, &g)) return false;
+  if (nblocks_out) *nblocks_out = g;
+  s->last_family = 6;
+  return true;
+}
+
 void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double *scale_dev,
                double *dot_partials, int *nblocks_out, hipStream_t st) {
   if (s->dim == 0) {

@@ -48,6 +48,15 @@ struct HvArgs {
   unsigned long long *nnz_out;
   int nxcd;             // tile ranges per L2 domain (1: plain round-robin over blocks)
   int chunk_log2;       // nearest-neighbour kernels: interleaved chunks of 2^c tiles per domain (0: one range)
+  // Fused Lanczos step (stencil sectors, launch_hv_fused).  The input vector
+  // is v = c1 v1 - c2 vprev (v1: P of the previous step, vprev: its Lanczos
+  // vector; vprev null: v = c1 v1) with c1 = fsc[fis], c2 = fsc[fia] fsc[fis];
+  // the kernels write v to xout and P = H v - c3 vprev to v0 (c3 = fsc[fib]),
+  // and 4 partial sums per block: Re<v, Hv>, |P|^2, Re<v, P>, |v|^2.
+  const void *vprev;
+  void *xout;
+  const double *fsc;
+  int fis, fia, fib;
 };
 
 // Persistent-grid tile schedule.  Workgroups are dispatched round-robin over
@@ -131,6 +140,60 @@ __device__ inline void block_reduce_store(double re, double im, double *out) {
   }
 }
 
+// Block reduction of four doubles into out[4*blockIdx.x .. +3] (fixed order).
+__device__ inline void block_reduce_store4(const double (&v)[4], double *out) {
+  __shared__ double red4[4][kThreads / 64];
+  double t[4] = {v[0], v[1], v[2], v[3]};
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t[k] += __shfl_down(t[k], d, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) red4[k][wid] = t[k];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double a = 0;
+    for (int w = 0; w < kThreads / 64; ++w) a += red4[threadIdx.x][w];
+    out[4 * blockIdx.x + threadIdx.x] = a;
+  }
+}
+
+// Fused Lanczos step coefficients (see HvArgs): c1 = 1/beta, c2 = alpha/beta,
+// c3 = beta of the previous step; first step (fib < 0): c1 = the start
+// vector's scale, c2 = c3 = 0.
+template <class R>
+struct FusedCoef {
+  R c1, c2, c3;
+  bool hasv;
+};
+template <class R>
+__device__ __forceinline__ FusedCoef<R> fused_coef(const HvArgs &a) {
+  FusedCoef<R> f;
+  f.c1 = (R)a.fsc[a.fis];
+  f.hasv = a.fib >= 0 && a.vprev != nullptr;
+  f.c2 = f.hasv ? (R)(a.fsc[a.fia] * a.fsc[a.fis]) : (R)0;
+  f.c3 = f.hasv ? (R)a.fsc[a.fib] : (R)0;
+  return f;
+}
+__device__ __forceinline__ float fz_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fz_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// v = c1 p - c2 w, one rounding order for every kernel that forms it
+template <class R>
+__device__ __forceinline__ typename CT<R>::T fused_v(const FusedCoef<R> &f, typename CT<R>::T p,
+                                                      typename CT<R>::T w) {
+  typename CT<R>::T v;
+  if (f.hasv) {
+    v.x = fz_fma(f.c1, p.x, -(f.c2 * w.x));
+    v.y = fz_fma(f.c1, p.y, -(f.c2 * w.y));
+  } else {
+    v.x = f.c1 * p.x;
+    v.y = f.c1 * p.y;
+  }
+  return v;
+}
+
 using KFn = void (*)(HvArgs);
 
 }  // namespace hvk
@@ -147,5 +210,8 @@ size_t chain_nn_lds_bytes(int N, int L, int nbt, int prec);
 // hv_stencil), with its grid and dynamic LDS; false leaves nothing launched.
 bool launch_hv_stencil(qs3x_sector *s, int prec, const hvk::HvArgs &a, hipStream_t st, int *grid_out,
                        size_t *lds_out);
+// The fused Lanczos step on the stencil (a.xout set, see HvArgs): both
+// kernels write 4 partial sums per block into a.dot_partials (if set).
+bool launch_hv_stencil_fused(qs3x_sector *s, int prec, const hvk::HvArgs &a, hipStream_t st, int *grid_out);
 
 }  // namespace qs3x

@@ -278,15 +278,13 @@ __device__ __forceinline__ void finish_rows(const HvArgs &a, const int64_t *row,
   }
 }
 
-// A queued row: every hop through the key search.
-template <int N, class R, bool REALPH, bool UNI, class KeyT>
-__device__ __forceinline__ void slow_row(const HvArgs &a, const NnCtx<N, R> &k, int64_t row, R scale, double &dre,
-                                         double &dim_) {
-  using C = typename CT<R>::T;
+// Hop slots of a row through the key search: target anchored rank (-1: no
+// hop), coefficient, explist index; anti = the antiparallel bonds' Jz/4 sum.
+template <int N, class R, bool UNI, class KeyT>
+__device__ __forceinline__ void slow_row_hops(const HvArgs &a, const NnCtx<N, R> &k, uint64_t w, int *hc, R *hcf,
+                                              int *hei, R &anti) {
   constexpr int H = 2 * N;
   const int L = k.L;
-  const uint64_t w = a.cfg[row];
-  const C xin = ((const C *)a.v1)[row];
   int n[N];
 #pragma unroll
   for (int q = 0; q < N; ++q) n[q] = (int)((w >> (q * a.bits)) & ((1ull << a.bits) - 1ull));
@@ -295,9 +293,7 @@ __device__ __forceinline__ void slow_row(const HvArgs &a, const NnCtx<N, R> &k, 
   for (int q = 0; q < N - 1; ++q) g[q] = n[q + 1] - n[q];
   g[N - 1] = L + 1 - n[N - 1];
   const R rinv = k.s_cfc[(int)(w >> 56)] / k.s_rcl[1];
-  R anti = 0;
-  int hc[H], hei[H];
-  R hcf[H];
+  anti = 0;
 #pragma unroll
   for (int h = 0; h < H; ++h) {
     const int m = h >> 1;
@@ -325,8 +321,84 @@ __device__ __forceinline__ void slow_row(const HvArgs &a, const NnCtx<N, R> &k, 
     hcf[h] = k.s_jxy2[bt] * k.s_rcl[count] * rinv;
     hei[h] = ei;
   }
+}
+
+// A queued row: every hop through the key search.
+template <int N, class R, bool REALPH, bool UNI, class KeyT>
+__device__ __forceinline__ void slow_row(const HvArgs &a, const NnCtx<N, R> &k, int64_t row, R scale, double &dre,
+                                         double &dim_) {
+  using C = typename CT<R>::T;
+  constexpr int H = 2 * N;
+  const uint64_t w = a.cfg[row];
+  const C xin = ((const C *)a.v1)[row];
+  R anti = 0;
+  int hc[H], hei[H];
+  R hcf[H];
+  slow_row_hops<N, R, UNI, KeyT>(a, k, w, hc, hcf, hei, anti);
   const bool store = true;
   finish_rows<N, R, REALPH, 1>(a, &row, &xin, &store, &hc, &hcf, &hei, &anti, scale, dre, dim_);
+}
+
+// The fused Lanczos step of HvArgs for a queued row: every element it reads
+// is formed as v = c1 P - c2 v' from the two input vectors.
+template <int N, class R, bool REALPH, class KeyT>
+__device__ __forceinline__ void slow_row_fused(const HvArgs &a, const NnCtx<N, R> &k, const FusedCoef<R> &fc,
+                                               int64_t row, double (&acc4)[4]) {
+  using C = typename CT<R>::T;
+  constexpr int H = 2 * N;
+  const C *__restrict__ v1 = (const C *)a.v1;
+  const C *__restrict__ vp = (const C *)a.vprev;
+  const uint64_t w = a.cfg[row];
+  R anti = 0;
+  int hc[H], hei[H];
+  R hcf[H];
+  slow_row_hops<N, R, true, KeyT>(a, k, w, hc, hcf, hei, anti);
+  uint2 wd[H];
+#pragma unroll
+  for (int h = 0; h < H; ++h) wd[h] = a.bitmap[(uint32_t)(hc[h] >= 0 ? hc[h] : 0) >> 5];
+  C vo, p[H + 1], q[H + 1];
+  vo.x = vo.y = (R)0;
+  bool hin[H];
+#pragma unroll
+  for (int h = 0; h < H; ++h) {
+    const uint32_t b = (uint32_t)hc[h] & 31u;
+    hin[h] = hc[h] >= 0 && ((wd[h].x >> b) & 1u);
+    const uint32_t id = hin[h] ? wd[h].y + (uint32_t)__popc(wd[h].x & ((1u << b) - 1u)) : (uint32_t)row;
+    p[h] = v1[id];
+    q[h].x = q[h].y = (R)0;
+    if (fc.hasv) q[h] = vp[id];
+  }
+  p[H] = v1[row];
+  q[H].x = q[H].y = (R)0;
+  if (fc.hasv) q[H] = vp[row];
+  vo = q[H];
+  const C x = fused_v<R>(fc, p[H], q[H]);
+  R accx = 0, accy = 0;
+#pragma unroll
+  for (int h = 0; h < H; ++h) {
+    const C y = fused_v<R>(fc, p[h], q[h]);
+    const R cf = hin[h] ? hcf[h] : (R)0;
+    if constexpr (REALPH) {
+      accx += cf * y.x;
+      accy += cf * y.y;
+    } else {
+      const R pr = (R)a.explist[2 * hei[h]] * cf, pi = (R)a.explist[2 * hei[h] + 1] * cf;
+      accx += pr * y.x - pi * y.y;
+      accy += pr * y.y + pi * y.x;
+    }
+  }
+  const R dg = (sizeof(R) == 8 ? (R)a.ezz : (R)0) - (R)2 * anti;
+  C out, pn;
+  out.x = dg * x.x + accx;
+  out.y = dg * x.y + accy;
+  pn.x = out.x - fc.c3 * vo.x;
+  pn.y = out.y - fc.c3 * vo.y;
+  st_nt(&((C *)a.xout)[row], x);
+  st_nt(&((C *)a.v0)[row], pn);
+  acc4[0] += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
+  acc4[1] += (double)pn.x * (double)pn.x + (double)pn.y * (double)pn.y;
+  acc4[2] += (double)x.x * (double)pn.x + (double)x.y * (double)pn.y;
+  acc4[3] += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
 }
 
 // Hop slots of a clear row (see the file comment); returns false, leaving
@@ -764,19 +836,28 @@ struct StencilArgs {
   int B;                // lines per tile
 };
 
-template <class R, bool REALPH>
+// FUSED: the fused Lanczos step of HvArgs -- the window holds v = c1 P - c2 v'
+// formed from two loaded vectors when it is stored to LDS, and a row writes v
+// and P_new = H v - c3 v' (v' of its own row read at the start of the slab).
+// BT: lines per tile (sa.B == BT), which sizes the window and round registers.
+// D: slab windows in flight per block (2 or 3) while a slab is computed.
+template <class R, bool REALPH, bool FUSED, int BT, int D>
 __global__ __launch_bounds__(kThreads) void k_hv_stencil4(HvArgs a, StencilArgs sa) {
   using C = typename CT<R>::T;
   extern __shared__ __align__(16) unsigned char smem[];
   const int L = a.L;
-  const int B = sa.B, NL = B + 2;
+  constexpr int B = BT, NL = B + 2;
   // LDS: ring [kRing][NL][kSegW] elements | line entries of the task's slabs
   // c0 - 1 .. c1: [slabs + 2][NL] int4
   C *s_ring = (C *)smem;
   int4 *s_tl = (int4 *)(smem + (size_t)kRing * NL * kSegW * sizeof(C));
   const C *__restrict__ v1 = (const C *)a.v1;
+  const C *__restrict__ vp = (const C *)a.vprev;
   C *__restrict__ v0 = (C *)a.v0;
-  const R scale = a.scale ? (R)(*a.scale) : (R)1;
+  C *__restrict__ xo = (C *)a.xout;
+  FusedCoef<R> fc{(R)1, (R)0, (R)0, false};
+  if constexpr (FUSED) fc = fused_coef<R>(a);
+  const R scale = (!FUSED && a.scale) ? (R)(*a.scale) : (R)1;
   const R jxy2 = (R)a.bt[0], jz4 = (R)a.bt[1];
   const int ei_ar = (L - 1) + 3 * (L + 1), ei_al = 1 + 3 * (L + 1);
   R ph_ar_r = (R)1, ph_ar_i = (R)0, ph_al_r = (R)1, ph_al_i = (R)0;
@@ -788,108 +869,152 @@ __global__ __launch_bounds__(kThreads) void k_hv_stencil4(HvArgs a, StencilArgs 
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   double dre = 0.0, dim_ = 0.0;
+  double acc4[4] = {0.0, 0.0, 0.0, 0.0};  // FUSED: Re<v,Hv>, |P|^2, Re<v,P>, |v|^2
   const int W1 = L + 1;
-  constexpr int kPer = (kMaxLines + 2 + 3) / 4;  // window lines per wavefront
+  constexpr int kPer = (BT + 2 + 3) / 4;  // window lines per wavefront
+  constexpr int kRounds = (BT * kSegA + kThreads - 1) / kThreads;  // output rounds per slab
 
   // Registers of slab windows: wavefront w holds lines w, w + 4, ...; lane =
   // position a0 - 1 + lane.  Line entries come from LDS, so the loads issue at
   // once.  Two windows are in flight (sets X and Y, alternating) while a slab
   // is computed, so a block keeps about two slabs of loads outstanding.
-  C preX[kPer], preY[kPer];
-  auto fetch = [&](C (&pre)[kPer], int a0, const int4 *te) __attribute__((always_inline)) {
+  C pre[D][kPer], pw[D][kPer];
+  auto fetch = [&](C (&pre)[kPer], C (&pw)[kPer], int a0, const int4 *te) __attribute__((always_inline)) {
 #pragma unroll
     for (int m = 0; m < kPer; ++m) {
       const int j = wid + 4 * m;
-      C v;
-      v.x = v.y = (R)0;
+      C v, w;
+      v.x = v.y = w.x = w.y = (R)0;
       if (j < NL) {
         const int4 e = te[j];
         const int av = a0 - 1 + lane;
-        if (av >= e.y && av < e.z) v = v1[e.x + (av - e.y)];
+        if (av >= e.y && av < e.z) {
+          v = v1[e.x + (av - e.y)];
+          if (FUSED && fc.hasv) w = vp[e.x + (av - e.y)];
+        }
       }
       pre[m] = v;
+      if constexpr (FUSED) pw[m] = w;
     }
   };
-  auto store = [&](const C (&pre)[kPer], int sl) __attribute__((always_inline)) {
+  auto store = [&](const C (&pre)[kPer], const C (&pw)[kPer], int sl) __attribute__((always_inline)) {
 #pragma unroll
     for (int m = 0; m < kPer; ++m) {
       const int j = wid + 4 * m;
-      if (j < NL) s_ring[((size_t)sl * NL + j) * kSegW + lane] = pre[m];
+      if (j < NL) s_ring[((size_t)sl * NL + j) * kSegW + lane] = FUSED ? fused_v<R>(fc, pre[m], pw[m]) : pre[m];
+    }
+  };
+  // The rows of slab c that this thread computes (clear rows only: the
+  // others are in the sector's slow-row list), in output round k.
+  auto own_row = [&](int a0, int b0, int c, const int4 *te0, int k, int &r, int &j, int &ai)
+                     __attribute__((always_inline)) -> bool {
+    const int q = k * kThreads + (int)threadIdx.x;
+    j = 1 + q / kSegA;
+    ai = q % kSegA;  // window line, a - a0
+    r = 0;
+    if (q >= B * kSegA) return false;
+    const int4 e = te0[j];
+    const int av = a0 + ai;
+    const int b = b0 - 1 + j;
+    const int g1 = av - 1, g2 = b - av, g3 = c - b;
+    r = e.x + (av - e.y);
+    return av >= e.y && av < e.z && max(g1, max(g2, g3)) <= L + 1 - c - 3;
+  };
+  // FUSED: v' of slab c's own rows.  Issued right after the slab's window is
+  // stored, in the set of that window, so at compute time they are older
+  // than the window loads in flight (vector-memory loads retire in order)
+  // and the wait for them does not wait for the prefetch.
+  C vos[D][kRounds];
+  auto load_vo = [&](C (&vo)[kRounds], int a0, int b0, int c, const int4 *te0) __attribute__((always_inline)) {
+#pragma unroll
+    for (int k = 0; k < kRounds; ++k) {
+      int r, j, ai;
+      const bool ok = own_row(a0, b0, c, te0, k, r, j, ai);
+      vo[k].x = vo[k].y = (R)0;
+      if (fc.hasv && ok) vo[k] = vp[r];
     }
   };
   // slab c from slots c - 1, c, c + 1 of the ring
-  auto compute = [&](int a0, int b0, int c, const int4 *te0) __attribute__((always_inline)) {
+  auto compute = [&](int a0, int b0, int c, const int4 *te0, const C (&vo)[kRounds]) __attribute__((always_inline)) {
     const int s0 = c % kRing, sm = (c + kRing - 1) % kRing, sp = (c + 1) % kRing;
     const C *r0 = s_ring + (size_t)s0 * NL * kSegW, *rm = s_ring + (size_t)sm * NL * kSegW,
             *rp = s_ring + (size_t)sp * NL * kSegW;
-    const int W = L + 1 - c;
-    const int npos = B * kSegA;
-    for (int pb = 0; pb < npos; pb += kThreads) {
-      const int q = pb + (int)threadIdx.x;
-      const int j = 1 + q / kSegA, ai = q % kSegA;  // window line, a - a0
-      if (q < npos) {
-        const int4 e = te0[j];
-        const int av = a0 + ai;
-        const int b = b0 - 1 + j;
-        const int g1 = av - 1, g2 = b - av, g3 = c - b;
-        // clear rows only: the others are in the sector's slow-row list
-        if (av >= e.y && av < e.z && max(g1, max(g2, g3)) <= W - 3) {
-          const int r = e.x + (av - e.y);
-          const int x0 = j * kSegW + ai + 1;  // window index of (a, b) in a slot
-          C x = r0[x0];
-          R sx = 0, sy = 0, arx = 0, ary = 0, alx, aly;
-          int nanti = 2;  // spin c to the right and the anchor to site L: W > 1
-          C y = rp[x0];   // (a, b, c + 1)
-          sx += y.x;
-          sy += y.y;
-          y = rp[x0 + kSegW + 1];  // anchor to site L: (a + 1, b + 1, c + 1)
-          alx = y.x;
-          aly = y.y;
-          if (g2 > 1) {
-            y = r0[x0 + 1];  // (a + 1, b, c)
-            sx += y.x;
-            sy += y.y;
-            y = r0[x0 - kSegW];  // (a, b - 1, c)
-            sx += y.x;
-            sy += y.y;
-            nanti += 2;
-          }
-          if (g1 > 1) {
-            y = r0[x0 - 1];  // (a - 1, b, c)
-            sx += y.x;
-            sy += y.y;
-            y = rm[x0 - kSegW - 1];  // anchor to site 2: (a - 1, b - 1, c - 1)
-            arx = y.x;
-            ary = y.y;
-            nanti += 2;
-          }
-          if (g3 > 1) {
-            y = r0[x0 + kSegW];  // (a, b + 1, c)
-            sx += y.x;
-            sy += y.y;
-            y = rm[x0];  // (a, b, c - 1)
-            sx += y.x;
-            sy += y.y;
-            nanti += 2;
-          }
-          if constexpr (REALPH) {
-            sx += arx + alx;
-            sy += ary + aly;
-          } else {
-            sx += ph_ar_r * arx - ph_ar_i * ary + ph_al_r * alx - ph_al_i * aly;
-            sy += ph_ar_r * ary + ph_ar_i * arx + ph_al_r * aly + ph_al_i * alx;
-          }
-          x.x *= scale;
-          x.y *= scale;
-          const R dg = (sizeof(R) == 8 ? (R)a.ezz : (R)0) - (R)2 * jz4 * (R)nanti;
-          C out;
-          out.x = dg * x.x + jxy2 * scale * sx;
-          out.y = dg * x.y + jxy2 * scale * sy;
-          st_nt(&v0[r], out);
-          if (a.dot_partials) {
-            dre += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
-            dim_ += (double)x.x * (double)out.y - (double)x.y * (double)out.x;
-          }
+    int rr[kRounds], rj[kRounds], rai[kRounds];
+    bool rok[kRounds];
+#pragma unroll
+    for (int k = 0; k < kRounds; ++k) rok[k] = own_row(a0, b0, c, te0, k, rr[k], rj[k], rai[k]);
+#pragma unroll
+    for (int k = 0; k < kRounds; ++k) {
+      if (!rok[k]) continue;
+      const int j = rj[k], ai = rai[k], r = rr[k];
+      const int av = a0 + ai;
+      const int b = b0 - 1 + j;
+      const int g1 = av - 1, g2 = b - av, g3 = c - b;
+      const int x0 = j * kSegW + ai + 1;  // window index of (a, b) in a slot
+      C x = r0[x0];
+      R sx = 0, sy = 0, arx = 0, ary = 0, alx, aly;
+      int nanti = 2;  // spin c to the right and the anchor to site L: W > 1
+      C y = rp[x0];   // (a, b, c + 1)
+      sx += y.x;
+      sy += y.y;
+      y = rp[x0 + kSegW + 1];  // anchor to site L: (a + 1, b + 1, c + 1)
+      alx = y.x;
+      aly = y.y;
+      if (g2 > 1) {
+        y = r0[x0 + 1];  // (a + 1, b, c)
+        sx += y.x;
+        sy += y.y;
+        y = r0[x0 - kSegW];  // (a, b - 1, c)
+        sx += y.x;
+        sy += y.y;
+        nanti += 2;
+      }
+      if (g1 > 1) {
+        y = r0[x0 - 1];  // (a - 1, b, c)
+        sx += y.x;
+        sy += y.y;
+        y = rm[x0 - kSegW - 1];  // anchor to site 2: (a - 1, b - 1, c - 1)
+        arx = y.x;
+        ary = y.y;
+        nanti += 2;
+      }
+      if (g3 > 1) {
+        y = r0[x0 + kSegW];  // (a, b + 1, c)
+        sx += y.x;
+        sy += y.y;
+        y = rm[x0];  // (a, b, c - 1)
+        sx += y.x;
+        sy += y.y;
+        nanti += 2;
+      }
+      if constexpr (REALPH) {
+        sx += arx + alx;
+        sy += ary + aly;
+      } else {
+        sx += ph_ar_r * arx - ph_ar_i * ary + ph_al_r * alx - ph_al_i * aly;
+        sy += ph_ar_r * ary + ph_ar_i * arx + ph_al_r * aly + ph_al_i * alx;
+      }
+      x.x *= scale;
+      x.y *= scale;
+      const R dg = (sizeof(R) == 8 ? (R)a.ezz : (R)0) - (R)2 * jz4 * (R)nanti;
+      C out;
+      out.x = dg * x.x + jxy2 * scale * sx;
+      out.y = dg * x.y + jxy2 * scale * sy;
+      if constexpr (FUSED) {
+        C pn;
+        pn.x = out.x - fc.c3 * vo[k].x;
+        pn.y = out.y - fc.c3 * vo[k].y;
+        st_nt(&xo[r], x);
+        st_nt(&v0[r], pn);
+        acc4[0] += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
+        acc4[1] += (double)pn.x * (double)pn.x + (double)pn.y * (double)pn.y;
+        acc4[2] += (double)x.x * (double)pn.x + (double)x.y * (double)pn.y;
+        acc4[3] += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
+      } else {
+        st_nt(&v0[r], out);
+        if (a.dot_partials) {
+          dre += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
+          dim_ += (double)x.x * (double)out.y - (double)x.y * (double)out.x;
         }
       }
     }
@@ -913,32 +1038,46 @@ __global__ __launch_bounds__(kThreads) void k_hv_stencil4(HvArgs a, StencilArgs 
     // prologue: slabs c0 - 1, c0, c0 + 1 loaded together into their slots,
     // then c0 + 2 in flight (X)
     {
-      C preZ[kPer];
-      fetch(preX, a0, te(c0 - 1));
-      fetch(preY, a0, te(c0));
-      fetch(preZ, a0, te(c0 + 1));
-      store(preX, (c0 + kRing - 1) % kRing);
-      store(preY, c0 % kRing);
-      store(preZ, (c0 + 1) % kRing);
+      C preZ[kPer], pwZ[kPer];
+      fetch(pre[0], pw[0], a0, te(c0 - 1));
+      fetch(pre[1 % D], pw[1 % D], a0, te(c0));
+      fetch(preZ, pwZ, a0, te(c0 + 1));
+      store(pre[0], pw[0], (c0 + kRing - 1) % kRing);
+      store(pre[1 % D], pw[1 % D], c0 % kRing);
+      store(preZ, pwZ, (c0 + 1) % kRing);
     }
-    if (c0 + 2 <= c1) fetch(preX, a0, te(c0 + 2));
+    if constexpr (FUSED) {
+      load_vo(vos[0], a0, b0, c0, te(c0));
+      if (c0 + 1 < c1) load_vo(vos[1 % D], a0, b0, c0 + 1, te(c0 + 1));
+    }
+    // slab s travels in set (s - c0) % D: slabs c0 + 2 .. c0 + D in flight
+#pragma unroll
+    for (int k = 2; k <= D; ++k)
+      if (c0 + k <= c1) fetch(pre[k % D], pw[k % D], a0, te(c0 + k));
     __syncthreads();
-    // slab c: slab c + 3 goes in flight in one set while slab c + 2 (the other
-    // set) is stored after the slab is computed
-    for (int c = c0; c < c1; c += 2) {
-      if (c + 3 <= c1) fetch(preY, a0, te(c + 3));
-      compute(a0, b0, c, te(c));
-      if (c + 2 <= c1) store(preX, (c + 2) % kRing);
-      __syncthreads();
-      if (c + 1 < c1) {
-        if (c + 4 <= c1) fetch(preX, a0, te(c + 4));
-        compute(a0, b0, c + 1, te(c + 1));
-        if (c + 3 <= c1) store(preY, (c + 3) % kRing);
-        __syncthreads();
+    // slab c: slab c + D + 1 goes in flight in the set that slab c + 1 left,
+    // slab c is computed, and slab c + 2 (D slabs ago in flight) is stored
+    for (int c = c0; c < c1; c += D) {
+#pragma unroll
+      for (int u = 0; u < D; ++u) {
+        const int cc = c + u;
+        if (cc < c1) {
+          if (cc + D + 1 <= c1) fetch(pre[(u + 1) % D], pw[(u + 1) % D], a0, te(cc + D + 1));
+          compute(a0, b0, cc, te(cc), vos[u]);
+          if (cc + 2 <= c1) {
+            store(pre[(u + 2) % D], pw[(u + 2) % D], (cc + 2) % kRing);
+            if (FUSED && cc + 2 < c1) load_vo(vos[(u + 2) % D], a0, b0, cc + 2, te(cc + 2));
+          }
+          __syncthreads();
+        }
       }
     }
   }
-  if (a.dot_partials) block_reduce_store(dre, dim_, a.dot_partials);
+  if constexpr (FUSED) {
+    if (a.dot_partials) block_reduce_store4(acc4, a.dot_partials);
+  } else {
+    if (a.dot_partials) block_reduce_store(dre, dim_, a.dot_partials);
+  }
 }
 
 // Rows of an N_down = 4 stencil sector that are not clear (the stencil kernel
@@ -985,7 +1124,7 @@ __global__ void k_slow_write(const uint64_t *__restrict__ cfg, int64_t dim, int 
   }
 }
 // H.v of the listed rows through the key search of slow_row (one row per thread).
-template <class R, bool REALPH, class KeyT>
+template <class R, bool REALPH, class KeyT, bool FUSED>
 __global__ __launch_bounds__(kThreads) void k_hv_slow_rows(HvArgs a, const int32_t *__restrict__ rows, int nrows,
                                                            const int32_t *__restrict__ binom32) {
   constexpr int N = 4;
@@ -1001,12 +1140,21 @@ __global__ __launch_bounds__(kThreads) void k_hv_slow_rows(HvArgs a, const int32
   __syncthreads();
   const NnCtx<N, R> k{a.L, a.kbits, binom32, s_tab, s_tab + (N + 1), s_tab + 2 * (N + 1), s_tab + 2 * (N + 1) + 1,
                       nullptr};
-  const R scale = a.scale ? (R)(*a.scale) : (R)1;
-  double dre = 0.0, dim_ = 0.0;
-  for (int i = blockIdx.x * kThreads + threadIdx.x; i - (int)threadIdx.x < nrows; i += gridDim.x * kThreads) {
-    if (i < nrows) slow_row<N, R, REALPH, true, KeyT>(a, k, rows[i], scale, dre, dim_);
+  if constexpr (FUSED) {
+    const FusedCoef<R> fc = fused_coef<R>(a);
+    double acc4[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i - (int)threadIdx.x < nrows; i += gridDim.x * kThreads) {
+      if (i < nrows) slow_row_fused<N, R, REALPH, KeyT>(a, k, fc, rows[i], acc4);
+    }
+    if (a.dot_partials) block_reduce_store4(acc4, a.dot_partials);
+  } else {
+    const R scale = a.scale ? (R)(*a.scale) : (R)1;
+    double dre = 0.0, dim_ = 0.0;
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i - (int)threadIdx.x < nrows; i += gridDim.x * kThreads) {
+      if (i < nrows) slow_row<N, R, REALPH, true, KeyT>(a, k, rows[i], scale, dre, dim_);
+    }
+    if (a.dot_partials) block_reduce_store(dre, dim_, a.dot_partials);
   }
-  if (a.dot_partials) block_reduce_store(dre, dim_, a.dot_partials);
 }
 
 // Line table of an anchored N_down = 4 chain sector: for every row, the first
@@ -1075,17 +1223,42 @@ namespace {
 using SFn = void (*)(HvArgs, StencilArgs);
 using SlowFn = void (*)(HvArgs, const int32_t *, int, const int32_t *);
 
-SFn stencil_kernel(int prec, bool realph) {
-  if (prec == QS3X_FP64) return realph ? k_hv_stencil4<double, true> : k_hv_stencil4<double, false>;
-  return realph ? k_hv_stencil4<float, true> : k_hv_stencil4<float, false>;
-}
-SlowFn slow_kernel(int prec, bool realph, bool key32) {
-  if (prec == QS3X_FP64) {
-    if (realph) return key32 ? k_hv_slow_rows<double, true, uint32_t> : k_hv_slow_rows<double, true, uint64_t>;
-    return key32 ? k_hv_slow_rows<double, false, uint32_t> : k_hv_slow_rows<double, false, uint64_t>;
+template <int BT, int D>
+SFn stencil_kernel_b(int prec, bool realph, bool fused) {
+  if (fused) {
+    if (prec == QS3X_FP64)
+      return realph ? k_hv_stencil4<double, true, true, BT, D> : k_hv_stencil4<double, false, true, BT, D>;
+    return realph ? k_hv_stencil4<float, true, true, BT, D> : k_hv_stencil4<float, false, true, BT, D>;
   }
-  if (realph) return key32 ? k_hv_slow_rows<float, true, uint32_t> : k_hv_slow_rows<float, true, uint64_t>;
-  return key32 ? k_hv_slow_rows<float, false, uint32_t> : k_hv_slow_rows<float, false, uint64_t>;
+  if (prec == QS3X_FP64)
+    return realph ? k_hv_stencil4<double, true, false, BT, D> : k_hv_stencil4<double, false, false, BT, D>;
+  return realph ? k_hv_stencil4<float, true, false, BT, D> : k_hv_stencil4<float, false, false, BT, D>;
+}
+template <int D>
+SFn stencil_kernel_d(int prec, bool realph, bool fused, int B) {
+  switch (B) {
+    case 2: return stencil_kernel_b<2, D>(prec, realph, fused);
+    case 4: return stencil_kernel_b<4, D>(prec, realph, fused);
+    case 6: return stencil_kernel_b<6, D>(prec, realph, fused);
+    default: return stencil_kernel_b<8, D>(prec, realph, fused);
+  }
+}
+// lines per tile: 2, 4, 6 or 8; windows in flight: 2 or 3
+SFn stencil_kernel(int prec, bool realph, bool fused, int B, int depth) {
+  return depth >= 3 ? stencil_kernel_d<3>(prec, realph, fused, B) : stencil_kernel_d<2>(prec, realph, fused, B);
+}
+template <bool FU>
+SlowFn slow_kernel_t(int prec, bool realph, bool key32) {
+  if (prec == QS3X_FP64) {
+    if (realph)
+      return key32 ? k_hv_slow_rows<double, true, uint32_t, FU> : k_hv_slow_rows<double, true, uint64_t, FU>;
+    return key32 ? k_hv_slow_rows<double, false, uint32_t, FU> : k_hv_slow_rows<double, false, uint64_t, FU>;
+  }
+  if (realph) return key32 ? k_hv_slow_rows<float, true, uint32_t, FU> : k_hv_slow_rows<float, true, uint64_t, FU>;
+  return key32 ? k_hv_slow_rows<float, false, uint32_t, FU> : k_hv_slow_rows<float, false, uint64_t, FU>;
+}
+SlowFn slow_kernel(int prec, bool realph, bool key32, bool fused = false) {
+  return fused ? slow_kernel_t<true>(prec, realph, key32) : slow_kernel_t<false>(prec, realph, key32);
 }
 
 // Band width and task length: options hv_stencil_lines / hv_stencil_slabs,
@@ -1096,6 +1269,10 @@ SlowFn slow_kernel(int prec, bool realph, bool key32) {
 int stencil_lines(int prec) {
   if (options().hv_stencil_lines > 0) return std::min(options().hv_stencil_lines, kMaxLines);
   return prec == QS3X_FP64 ? 4 : 8;
+}
+int stencil_depth(int prec, bool fused) {
+  if (options().hv_stencil_depth > 0) return options().hv_stencil_depth;
+  return 2;
 }
 int stencil_slabs(int prec) {
   if (options().hv_stencil_slabs > 0) return options().hv_stencil_slabs;
@@ -1185,7 +1362,7 @@ void stencil_plan(int prec, qs3x_sector::StencilPlan &pl, const std::vector<int4
   pl.ntasks = (int)tasks.size();
   pl.tasks.alloc(std::max<size_t>(tasks.size(), 1));
   if (!tasks.empty()) pl.tasks.upload(tasks.data(), tasks.size());
-  pl.grid = 0;  // at the first launch
+  pl.grid = pl.grid_fused = 0;  // at the first launch
 }
 
 // The rows the stencil kernel skips, in row order (deterministic).
@@ -1241,7 +1418,8 @@ void stencil_tables(qs3x_sector *s, std::vector<int4> *lines_out) {
   }
 }
 
-bool launch_hv_stencil(qs3x_sector *s, int prec, const HvArgs &a, hipStream_t st, int *grid_out, size_t *lds_out) {
+static bool launch_stencil_impl(qs3x_sector *s, int prec, const HvArgs &a, hipStream_t st, int *grid_out,
+                                size_t *lds_out, bool fused) {
   if (!options().hv_stencil || !stencil_applies(s)) return false;
   const int L = s->model->NOS;
   qs3x_sector::StencilPlan &pl = s->stencil[prec == QS3X_FP64 ? 0 : 1];
@@ -1260,8 +1438,10 @@ bool launch_hv_stencil(qs3x_sector *s, int prec, const HvArgs &a, hipStream_t st
   if (!pl.ok) return false;
   int kb = 1;
   while ((1 << kb) <= L) ++kb;
-  SFn f = stencil_kernel(prec, s->real_phase);
-  SlowFn fs = slow_kernel(prec, s->real_phase, 3 * kb <= 32);
+  SFn f = stencil_kernel(prec, s->real_phase, fused, pl.B, stencil_depth(prec, fused));
+  SlowFn fs = slow_kernel(prec, s->real_phase, 3 * kb <= 32, fused);
+  // partial sums per block: 2 (complex <v, Hv>) or 4 in the fused step
+  const size_t np = fused ? 4 : 2;
   if (pl.lds > 64 * 1024) {
     static std::mutex mu;
     static std::map<SFn, size_t> done;
@@ -1273,7 +1453,8 @@ bool launch_hv_stencil(qs3x_sector *s, int prec, const HvArgs &a, hipStream_t st
     }
   }
   const int slow_grid = s->nslow > 0 ? (int)std::min<int64_t>((s->nslow + kThreads - 1) / kThreads, 512) : 0;
-  if (pl.grid == 0) {
+  int &pgrid = fused ? pl.grid_fused : pl.grid;
+  if (pgrid == 0) {
     // resident blocks per CU, from the kernel's registers and LDS (160 KB per
     // CU, 512 VGPRs per lane slot on each of 4 SIMDs, at most 8 wavefronts
     // per SIMD), a multiple of the 8 XCDs so that block g (on XCD g % 8) takes
@@ -1291,7 +1472,7 @@ bool launch_hv_stencil(qs3x_sector *s, int prec, const HvArgs &a, hipStream_t st
     const int per_cu = std::min(by_regs, by_lds);
     int grid = std::min<int64_t>(pl.ntasks, (int64_t)cus * per_cu);
     grid = std::min(grid, kMaxBlocks - 512);
-    pl.grid = std::max(8, (grid / 8) * 8);
+    pgrid = std::max(8, (grid / 8) * 8);
   }
   StencilArgs sa{s->d_lines.p, pl.tasks.p, pl.ntasks, pl.B};
   // The slow rows (a latency-bound tail of about 1 % of the rows) run after
@@ -1299,8 +1480,8 @@ bool launch_hv_stencil(qs3x_sector *s, int prec, const HvArgs &a, hipStream_t st
   // from and joined back into st; the two write disjoint rows and partial sums.
   if (slow_grid > 0 && !options().hv_stencil_side) {
     HvArgs b = a;
-    if (b.dot_partials) b.dot_partials += 2 * (size_t)pl.grid;
-    hipLaunchKernelGGL(f, dim3(pl.grid), dim3(kThreads), pl.lds, st, a, sa);
+    if (b.dot_partials) b.dot_partials += np * (size_t)pgrid;
+    hipLaunchKernelGGL(f, dim3(pgrid), dim3(kThreads), pl.lds, st, a, sa);
     QS3X_HIP(hipGetLastError());
     hipLaunchKernelGGL(fs, dim3(slow_grid), dim3(kThreads), 0, st, b, s->d_slow.p, (int)s->nslow, s->d_binom32.p);
     QS3X_HIP(hipGetLastError());
@@ -1317,21 +1498,30 @@ bool launch_hv_stencil(qs3x_sector *s, int prec, const HvArgs &a, hipStream_t st
     QS3X_HIP(hipEventRecord(fork[dev], st));
     QS3X_HIP(hipStreamWaitEvent(side[dev], fork[dev], 0));
     HvArgs b = a;
-    if (b.dot_partials) b.dot_partials += 2 * (size_t)pl.grid;
+    if (b.dot_partials) b.dot_partials += np * (size_t)pgrid;
     hipLaunchKernelGGL(fs, dim3(slow_grid), dim3(kThreads), 0, side[dev], b, s->d_slow.p, (int)s->nslow,
                        s->d_binom32.p);
     QS3X_HIP(hipGetLastError());
     QS3X_HIP(hipEventRecord(join[dev], side[dev]));
-    hipLaunchKernelGGL(f, dim3(pl.grid), dim3(kThreads), pl.lds, st, a, sa);
+    hipLaunchKernelGGL(f, dim3(pgrid), dim3(kThreads), pl.lds, st, a, sa);
     QS3X_HIP(hipGetLastError());
     QS3X_HIP(hipStreamWaitEvent(st, join[dev], 0));
   } else {
-    hipLaunchKernelGGL(f, dim3(pl.grid), dim3(kThreads), pl.lds, st, a, sa);
+    hipLaunchKernelGGL(f, dim3(pgrid), dim3(kThreads), pl.lds, st, a, sa);
     QS3X_HIP(hipGetLastError());
   }
-  if (grid_out) *grid_out = pl.grid + slow_grid;
+  if (grid_out) *grid_out = pgrid + slow_grid;
   if (lds_out) *lds_out = pl.lds;
   return true;
+}
+
+bool launch_hv_stencil(qs3x_sector *s, int prec, const HvArgs &a, hipStream_t st, int *grid_out, size_t *lds_out) {
+  return launch_stencil_impl(s, prec, a, st, grid_out, lds_out, false);
+}
+
+bool launch_hv_stencil_fused(qs3x_sector *s, int prec, const HvArgs &a, hipStream_t st, int *grid_out) {
+  size_t lds = 0;
+  return launch_stencil_impl(s, prec, a, st, grid_out, &lds, true);
 }
 
 }  // namespace qs3x

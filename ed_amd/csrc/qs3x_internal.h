@@ -148,7 +148,7 @@ struct qs3x_sector {
   // precision, the task list of its launch plan.
   struct StencilPlan {
     bool tried = false, ok = false;
-    int B = 0, slabs = 0, ntasks = 0, grid = 0;
+    int B = 0, slabs = 0, ntasks = 0, grid = 0, grid_fused = 0;
     int64_t nlines = 0;  // non-empty (n3, n4) lines
     size_t lds = 0;
     qs3x::DevBuf<int4> tasks;
@@ -182,6 +182,16 @@ double chain_ro(int L, int count, const double rk[3]);
 void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double *scale_dev,
                double *dot_partials, int *nblocks_out, hipStream_t st);
 int64_t count_nnz(qs3x_sector *s);
+// Fused Lanczos step on stencil sectors (hv.hip): v = c1 pprev - c2 vprev
+// (c1 = fsc[fis], c2 = fsc[fia] fsc[fis]; vprev null at the first step, then
+// v = c1 pprev) is written to xout and P = (H - e_zz) v - fsc[fib] vprev to
+// pout, with 4 partial sums per block in part4 (may be null): Re<v, Hv>,
+// |P|^2, Re<v, P>, |v|^2 (finalize_fused turns them into alpha and beta).
+// Returns false, launching nothing, where the stencil does not apply or
+// option lanczos_fused excludes the precision.
+bool hv_fused_available(const qs3x_sector *s, int prec);
+bool launch_hv_fused(qs3x_sector *s, int prec, void *pout, void *xout, const void *pprev, const void *vprev,
+                     const double *fsc, int fis, int fia, int fib, double *part4, int *nblocks_out, hipStream_t st);
 
 // Gap-lattice stencil (hv_nn.hip): whether a sector qualifies (anchored
 // nearest-neighbour chain with uniform bonds and N_down = 4); its line table,

@@ -129,18 +129,45 @@ int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, i
     // never lie below the lowest eigenvalue, and the change test still stops it.
     const int64_t mitr = std::min<int64_t>(maxitr, prec == QS3X_FP32 ? 2 * dim + 8 : dim);
     const size_t es = elem_bytes(prec);
-    DevBuf<unsigned char> U[3];
-    for (auto &u : U) u.alloc(es * dim);
+    // Fused steps (stencil sectors): one kernel pass forms the Lanczos vector
+    // v_i = (P_{i-1} - alpha_{i-1} v_{i-1}) / beta_{i-1} from the previous
+    // step's P and v while applying H, and writes v_i and P_i = (H - e_zz) v_i
+    // - beta_{i-1} v_{i-1}; alpha_i and beta_i come from its partial sums.
+    // Buffers: U[0], U[1] hold P (U[1]: the start vector), U[2], U[3] hold v.
+    // Otherwise the H.v and update kernels run on three buffers U[0..2].
+    const bool fused = hv_fused_available(s, prec);
+    DevBuf<unsigned char> U[4];
+    for (int k = 0; k < (fused ? 4 : 3); ++k) U[k].alloc(es * dim);
     // scalar slots
     const int64_t A_REC = 0, A_OUT = mitr, BETA = 2 * mitr, SCALE = 3 * mitr + 1;
     DevBuf<double> sc((size_t)4 * mitr + 4);
-    DevBuf<double> part((size_t)2 * kMaxBlocks);
-    const double shift = (prec == QS3X_FP32) ? s->model->ezz : 0.0;
+    DevBuf<double> part((size_t)4 * kMaxBlocks);
+    const double shift = (prec == QS3X_FP32 || fused) ? s->model->ezz : 0.0;
     trace().clear();
     const double t_start = now_s();
-    load_start(prec, U[0].p, dim, start, seed, st);
-    int nb = vec_dot(prec, U[0].p, prec, U[0].p, dim, part.p, st);
+    void *u0 = fused ? U[1].p : U[0].p;
+    load_start(prec, u0, dim, start, seed, st);
+    int nb = vec_dot(prec, u0, prec, u0, dim, part.p, st);
     finalize(part.p, nb, sc.p, 2, (int)(SCALE + mitr + 1), (int)SCALE, 0.0, st);  // scale[0] = 1/|u0|
+    // one step: alpha_i, beta_i and scale_{i+1} = 1/beta_i into sc
+    auto step = [&](int64_t i, double *pt) {
+      if (fused) {
+        void *pprev = U[(i + 1) % 2].p, *pout = U[i % 2].p;
+        void *vprev = i > 0 ? U[2 + (i + 1) % 2].p : nullptr, *xout = U[2 + i % 2].p;
+        QS3X_REQUIRE(launch_hv_fused(s, prec, pout, xout, pprev, vprev, sc.p, (int)(SCALE + i), (int)(A_REC + i - 1),
+                                     i > 0 ? (int)(BETA + i - 1) : -1, pt, &nb, st),
+                     "qs3x_lanczos: fused step unavailable");
+        if (pt) finalize_fused(pt, nb, sc.p, (int)(A_REC + i), (int)(A_OUT + i), (int)(BETA + i), (int)(SCALE + i + 1),
+                               shift, st);
+        return;
+      }
+      void *cur = U[i % 3].p, *prev = U[(i + 2) % 3].p, *w = U[(i + 1) % 3].p;
+      launch_hv(s, prec, w, cur, sc.p + SCALE + i, pt, &nb, st);
+      if (pt) finalize(pt, nb, sc.p, 1, (int)(A_REC + i), (int)(A_OUT + i), shift, st);
+      nb = vec_update3(prec, w, cur, prev, dim, sc.p, (int)(A_REC + i), (int)(SCALE + i),
+                       i > 0 ? (int)(BETA + i - 1) : -1, (int)(SCALE + i - 1), pt, st);
+      if (pt) finalize(pt, nb, sc.p, 2, (int)(BETA + i), (int)(SCALE + i + 1), 0.0, st);
+    };
     std::vector<double> ha, hb;
     double E = 0.0, tmp = 0.0;
     int64_t ite = 0;
@@ -197,12 +224,7 @@ int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, i
       tmp = E;
     };
     for (int64_t i = 0; i < mitr && !done; ++i) {
-      void *cur = U[i % 3].p, *prev = U[(i + 2) % 3].p, *w = U[(i + 1) % 3].p;
-      launch_hv(s, prec, w, cur, sc.p + SCALE + i, part.p, &nb, st);
-      finalize(part.p, nb, sc.p, 1, (int)(A_REC + i), (int)(A_OUT + i), shift, st);
-      nb = vec_update3(prec, w, cur, prev, dim, sc.p, (int)(A_REC + i), (int)(SCALE + i),
-                       i > 0 ? (int)(BETA + i - 1) : -1, (int)(SCALE + i - 1), part.p, st);
-      finalize(part.p, nb, sc.p, 2, (int)(BETA + i), (int)(SCALE + i + 1), 0.0, st);
+      step(i, part.p);
       ite = i + 1;
       if (ite == mitr || (ite >= minitr && ite % itrint == 0)) {
         Pending &p = pend[slot];
@@ -235,14 +257,20 @@ int qs3x_lanczos(qs3x_sector *s, int32_t prec, int64_t maxitr, int64_t minitr, i
       std::vector<double> z(ite);
       if (ite == 1) z[0] = 1.0;
       else tridiag_vector(ha.data(), hb.data(), (int)ite, E, z.data());
-      load_start(prec, U[0].p, dim, start, seed, st);
-      vec_axpy_acc(prec, p64.p, U[0].p, dim, z[0], sc.p, (int)SCALE, st);
-      for (int64_t i = 0; i + 1 < ite; ++i) {
-        void *cur = U[i % 3].p, *prev = U[(i + 2) % 3].p, *w = U[(i + 1) % 3].p;
-        launch_hv(s, prec, w, cur, sc.p + SCALE + i, nullptr, nullptr, st);
-        vec_update3(prec, w, cur, prev, dim, sc.p, (int)(A_REC + i), (int)(SCALE + i),
-                    i > 0 ? (int)(BETA + i - 1) : -1, (int)(SCALE + i - 1), nullptr, st);
-        vec_axpy_acc(prec, p64.p, w, dim, z[i + 1], sc.p, (int)(SCALE + i + 1), st);
+      load_start(prec, u0, dim, start, seed, st);
+      if (fused) {
+        // the second pass repeats the steps with the stored alpha and beta;
+        // step i writes v_i itself (normalised) into U[2 + i % 2]
+        for (int64_t i = 0; i < ite; ++i) {
+          step(i, nullptr);
+          vec_axpy_acc(prec, p64.p, U[2 + i % 2].p, dim, z[i], sc.p, -1, st);
+        }
+      } else {
+        vec_axpy_acc(prec, p64.p, U[0].p, dim, z[0], sc.p, (int)SCALE, st);
+        for (int64_t i = 0; i + 1 < ite; ++i) {
+          step(i, nullptr);
+          vec_axpy_acc(prec, p64.p, U[(i + 1) % 3].p, dim, z[i + 1], sc.p, (int)(SCALE + i + 1), st);
+        }
       }
       nb = vec_dot(QS3X_FP64, p64.p, QS3X_FP64, p64.p, dim, part.p, st);
       finalize(part.p, nb, sc.p, 2, 0, 1, 0.0, st);

@@ -256,7 +256,8 @@ class Sector:
     def hv_last_launch(self):
         """(kernel family, dynamic LDS bytes) of the last H.v launch: 0
         nearest-neighbour chain, 1 batched-load chain, 2 per-hop chain, 3
-        general lattice, 4 U(1)-only."""
+        general lattice, 4 U(1)-only, 5 gap-lattice stencil, 6 the fused
+        Lanczos step on the stencil."""
         f = np.zeros(1, dtype=np.int32)
         lds = np.zeros(1, dtype=np.int64)
         _check(_lib.qs3x_hv_last_launch(self._h, _p(f), _p(lds)))

@@ -333,8 +333,9 @@ int qs3x_continued_fraction(int32_t nlev, const double *alpha, const double *bet
 /* ---------------------------------------------------------- measurement */
 /* The kernel family of the sector's last H . v launch (no counterpart in the
  * reference): 0 nearest-neighbour chain, 1 batched-load chain, 2 per-hop
- * chain, 3 general lattice, 4 U(1)-only, 5 gap-lattice stencil; -1 before
- * the first launch.
+ * chain, 3 general lattice, 4 U(1)-only, 5 gap-lattice stencil, 6 the fused
+ * Lanczos step on the stencil (qs3x_lanczos on such a sector, option
+ * lanczos_fused); -1 before the first launch.
  * lds_bytes: its dynamic LDS. */
 int qs3x_hv_last_launch(const qs3x_sector *s, int32_t *family, int64_t *lds_bytes);
 /* The stencil H . v's plan for this sector and precision, after its first

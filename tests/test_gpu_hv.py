@@ -351,23 +351,23 @@ def test_hv_long_j1j2_chain_matches_general(K):
 
 
 STENCIL_CASES = [
-    # L, K, Jz, hv_stencil_lines, hv_stencil_slabs
-    (40, 0, 0.8, 0, 16),
-    (40, 11, 0.8, 3, 5),
-    (64, 3, 1.0, 1, 1),
-    (64, 0, 1.0, 8, 16),
-    (128, 0, 1.0, 0, 0),
-    (128, 17, 0.6, 4, 7),
+    # L, K, Jz, hv_stencil_lines, hv_stencil_slabs, hv_stencil_depth
+    (40, 0, 0.8, 0, 16, 0),
+    (40, 11, 0.8, 6, 5, 3),
+    (64, 3, 1.0, 2, 1, 0),
+    (64, 0, 1.0, 8, 16, 3),
+    (128, 0, 1.0, 0, 0, 0),
+    (128, 17, 0.6, 4, 7, 3),
 ]
 
 
-@pytest.mark.parametrize("L,K,jz,B,slabs", STENCIL_CASES)
-def test_hv_stencil_parity(L, K, jz, B, slabs):
+@pytest.mark.parametrize("L,K,jz,B,slabs,depth", STENCIL_CASES)
+def test_hv_stencil_parity(L, K, jz, B, slabs, depth):
     """Gap-lattice stencil H.v (nearest-neighbour chains, uniform bonds,
     N_down = 4): FP64 within 1e-12 of the oracle's ham_to_vec_wave_vector
     (ham2vec_v3.f90:37-72), FP32 (H - e_zz) v within 2e-6, for several band
-    widths and task lengths, with 16-byte aligned and unaligned FP32 vectors;
-    the stencil kernel is the one that ran."""
+    widths, task lengths and prefetch depths, with 16-byte aligned and
+    unaligned FP32 vectors; the stencil kernel is the one that ran."""
     print(f"[gpu] hv stencil L={L} K={K} B={B} slabs={slabs}", flush=True)
     q3 = _q3()
     bonds = orc.chain_bonds(L, [(1, 1.0, jz)])
@@ -377,6 +377,7 @@ def test_hv_stencil_parity(L, K, jz, B, slabs):
     gm = q3.Model(L, L, 1, 1, bonds)
     q3.set_option("hv_stencil_lines", B)
     q3.set_option("hv_stencil_slabs", slabs)
+    q3.set_option("hv_stencil_depth", depth)
     try:
         s = q3.Sector.build(gm, 4, rk)
         rng = np.random.default_rng(L + K)
@@ -399,6 +400,7 @@ def test_hv_stencil_parity(L, K, jz, B, slabs):
     finally:
         q3.set_option("hv_stencil_lines", 0)
         q3.set_option("hv_stencil_slabs", 0)
+        q3.set_option("hv_stencil_depth", 0)
 
 
 def test_hv_stencil_matches_nn_kernel_c3():
