@@ -422,10 +422,12 @@ __device__ __forceinline__ int shift_site(int s, int i, int j, int k, int LX, in
 // 0-based rank r = sum_q C(p_q, q) over its 0-based down sites p_1 < ... < p_N
 // (list_fly / inv_list of source_only_u1/state_lists.f90, shifted to 0-based
 // sites and ranks).  A hop's target row is the rank of the flipped
-// configuration: no representative, norm or phase.  The occupation is one
-// 64-bit word (NOS <= 64), so membership tests are bit tests and the target
-// rank is a sum over the set bits of the flipped word.
-template <class R, bool COUNT>
+// configuration: no representative, norm or phase.  Up to 64 sites the
+// occupation is one 64-bit word, so membership tests are bit tests and the
+// target rank is a sum over the set bits of the flipped word; with WIDE (any
+// NOS) membership is a scan of the N sorted sites and the target rank a
+// merge of the remaining sites with the new one.
+template <class R, bool COUNT, bool WIDE>
 __global__ __launch_bounds__(kThreads) void k_hv_u1(HvArgs a, int N) {
   using C = typename CT<R>::T;
   const C *__restrict__ v1 = (const C *)a.v1;
@@ -452,7 +454,7 @@ __global__ __launch_bounds__(kThreads) void k_hv_u1(HvArgs a, int N) {
       }
       p[q - 1] = lo;
       r -= B[(int64_t)lo * kd + q];
-      occ |= 1ull << lo;
+      if (!WIDE) occ |= 1ull << lo;
       hi = lo - 1;
     }
     C x = v1[row];
@@ -464,15 +466,37 @@ __global__ __launch_bounds__(kThreads) void k_hv_u1(HvArgs a, int N) {
       for (int e = a.adj_ptr[s1]; e < a.adj_ptr[s1 + 1]; ++e) {
         const int ent = a.adj[e];
         const int t = (ent & 0xffff) - 1, btp = ent >> 16;
-        if ((occ >> t) & 1ull) continue;  // parallel bond
+        bool occupied;
+        if constexpr (WIDE) {
+          occupied = false;
+          for (int k = 0; k < N; ++k) occupied |= (p[k] == t);
+        } else {
+          occupied = (occ >> t) & 1ull;
+        }
+        if (occupied) continue;  // parallel bond
         anti += (R)a.bt[2 * btp + 1];
         if constexpr (COUNT) {
           ++nnz;
           continue;
         }
-        uint64_t w = occ ^ (1ull << p[m]) ^ (1ull << t);
         int64_t id = 0;
-        for (int q = 1; w; ++q, w &= w - 1) id += B[(int64_t)__builtin_ctzll(w) * kd + q];
+        if constexpr (WIDE) {
+          // the sites p minus p[m], with t inserted in order
+          int q = 1;
+          bool ins = false;
+          for (int k = 0; k < N; ++k) {
+            if (k == m) continue;
+            if (!ins && t < p[k]) {
+              id += B[(int64_t)t * kd + q++];
+              ins = true;
+            }
+            id += B[(int64_t)p[k] * kd + q++];
+          }
+          if (!ins) id += B[(int64_t)t * kd + q];
+        } else {
+          uint64_t w = occ ^ (1ull << p[m]) ^ (1ull << t);
+          for (int q = 1; w; ++q, w &= w - 1) id += B[(int64_t)__builtin_ctzll(w) * kd + q];
+        }
         const R cf = (R)a.bt[2 * btp];
         const C y = v1[id];
         accx += cf * y.x;
@@ -835,10 +859,14 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   int grid = tuned(persistent_grid(a.ntiles));
   if (nblocks_out) *nblocks_out = grid;
   if (s->mode == kIndexU1) {
-    if (prec == QS3X_FP64)
-      hipLaunchKernelGGL((k_hv_u1<double, false>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
-    else
-      hipLaunchKernelGGL((k_hv_u1<float, false>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
+    const bool wide = s->model->NOS > 64;
+    if (prec == QS3X_FP64) {
+      if (wide) hipLaunchKernelGGL((k_hv_u1<double, false, true>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
+      else hipLaunchKernelGGL((k_hv_u1<double, false, false>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
+    } else {
+      if (wide) hipLaunchKernelGGL((k_hv_u1<float, false, true>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
+      else hipLaunchKernelGGL((k_hv_u1<float, false, false>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
+    }
     QS3X_HIP(hipGetLastError());
     s->last_family = 4;
     s->last_lds = 0;
@@ -928,7 +956,10 @@ int64_t count_nnz(qs3x_sector *s) {
   KFn f = (s->mode != kIndexU1 && use_chain(s, QS3X_FP64))
               ? chain_kernel(s->nod, QS3X_FP64, true, normlist, true, false) : nullptr;
   if (s->mode == kIndexU1)
-    hipLaunchKernelGGL((k_hv_u1<double, true>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
+    if (s->model->NOS > 64)
+      hipLaunchKernelGGL((k_hv_u1<double, true, true>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
+    else
+      hipLaunchKernelGGL((k_hv_u1<double, true, false>), dim3(grid), dim3(kThreads), 0, st, a, s->nod);
   else if (f) {
     allow_lds(f, chain_lds_bytes(s, QS3X_FP64));
     hipLaunchKernelGGL(f, dim3(grid), dim3(kThreads), chain_lds_bytes(s, QS3X_FP64), st, a);

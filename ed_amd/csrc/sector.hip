@@ -451,7 +451,9 @@ __global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v) {
 
 }  // namespace
 
-static void upload_common(qs3x_sector *s, int kmax) {
+// packed: the sector keeps one packed configuration word per row (every
+// sector but the U(1)-only block, which unranks rows instead).
+static void upload_common(qs3x_sector *s, int kmax, bool packed = true) {
   qs3x_model *m = s->model;
   s->nbinom_n = m->NOS + 1;
   s->nbinom_k = kmax + 1;
@@ -476,7 +478,7 @@ static void upload_common(qs3x_sector *s, int kmax) {
       }
   s->real_phase = real;
   s->bits = bits_for(m->NOS);
-  QS3X_REQUIRE(s->nod * s->bits <= 56,
+  QS3X_REQUIRE(!packed || s->nod * s->bits <= 56,
                "sector: N_down * ceil(log2(NOS+1)) must be <= 56 bits for the packed rows");
 }
 
@@ -494,9 +496,11 @@ void build_sector_u1(qs3x_sector *s) {
   const int N = s->nod;
   QS3X_REQUIRE(N >= 1 && N <= kMaxNod, "U(1) sector: N_down must be in 1..16");
   QS3X_REQUIRE(N <= m->NOS, "U(1) sector: N_down > NOS");
-  QS3X_REQUIRE(m->NOS <= 64, "U(1) sector: NOS must be at most 64 (one 64-bit occupation word per row)");
+  // rows are 32-bit gather targets in the H.v kernels' int64 arithmetic, so
+  // only the row count bounds the block
+  QS3X_REQUIRE(m->NOS <= 65535, "U(1) sector: NOS must be below 65536 (16-bit site fields of the adjacency)");
   s->rk[0] = s->rk[1] = s->rk[2] = 0.0;
-  upload_common(s, N + 1);
+  upload_common(s, N + 1, false);
   s->mode = kIndexU1;
   s->chain = false;
   s->real_phase = true;

@@ -137,7 +137,9 @@ int qs3x_sector_build(qs3x_model *m, int32_t nod, double rkx, double rky, double
  * list setup of the U(1)-only code (source_only_u1/main.f90:23-29, THS =
  * combination(NOS, NOD)); H . v follows source_only_u1/ham2vec_v3.f90:35-65.
  * Such a sector supports H . v, the Lanczos solvers and the DSF recurrence;
- * qs3x_sector_lists returns list_s(i) = i and unit norms.  NOS <= 64. */
+ * qs3x_sector_lists returns list_s(i) = i and unit norms.  Any NOS below
+ * 65536 (up to 64 sites a row's occupation is one 64-bit word; beyond, the
+ * kernel works from the sorted down-spin sites). */
 int qs3x_sector_build_u1(qs3x_model *m, int32_t nod, qs3x_sector **out, int64_t *dim);
 /* Same sector from caller-supplied lists (e.g. the reference's own list_s /
  * list_r / explist), for drop-in use behind ham_to_vec_wave_vector: list_s

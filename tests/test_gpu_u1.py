@@ -31,6 +31,11 @@ CASES = [
     ("chain20_n1", 20, 20, 1, orc.chain_bonds(20, [(1, 1.0, 1.0)]), 1),
     ("torus4x4_n6", 16, 4, 4, orc.square_bonds(4, 4, 1.0), 6),
     ("torus4x3_n5", 12, 4, 3, orc.square_bonds(4, 3, 1.0), 5),
+    # beyond 64 sites: the kernel that works from the sorted down-spin sites
+    ("chain70_n3_j1j2", 70, 70, 1, orc.chain_bonds(70, [(1, 1.0, 0.9), (2, 0.4, 0.3)]), 3),
+    ("chain130_n2", 130, 130, 1, orc.chain_bonds(130, [(1, 1.0, 1.0)]), 2),
+    ("torus9x8_n2", 72, 9, 8, orc.square_bonds(9, 8, 1.0), 2),
+    ("chain65_n4", 65, 65, 1, orc.chain_bonds(65, [(1, 1.0, 0.6)]), 4),
 ]
 
 
@@ -106,3 +111,25 @@ def test_u1_rejects_symmetric_only_entry_points():
     v = np.ones(gs.dim, dtype=np.complex128) / np.sqrt(gs.dim)
     with pytest.raises(q3.QS3XError, match="U\\(1\\)-only"):
         q3.get_cf_2_wave_vector(gs, v, [(1, 2)])
+
+
+def test_u1_ground_state_beyond_64_sites():
+    """L = 66 J1-J2 chain, N = 3 (C(66, 3) = 45,760 rows, the kernel that
+    works from the sorted down-spin sites): E0 of the U(1) block = min over k
+    of the translation-symmetric sector E0 (FP64, 1e-10)."""
+    print("[gpu] u1 ground state L=66", flush=True)
+    q3 = _q3()
+    L, N = 66, 3
+    bonds = orc.chain_bonds(L, [(1, 1.0, 1.0), (2, 0.3, 0.3)])
+    gm = q3.Model(L, L, 1, 1, bonds)
+    e_k = []
+    for K in range(L):
+        s = q3.Sector.build(gm, N, (q3.wavevector(K, L), 0.0, 0.0))
+        if s.dim:
+            e, _, _ = q3.lanczos_routines(s, maxitr=2000, lnc_ene_conv=1e-14, seed=2, want_vector=False)
+            e_k.append(e)
+    gs = q3.Sector.build_u1(gm, N)
+    assert gs.dim == 45760
+    e, _, _ = q3.lanczos_routines(gs, maxitr=3000, lnc_ene_conv=1e-14, seed=2, want_vector=False)
+    assert gs.hv_last_launch()[0] == 4
+    assert abs(e - min(e_k)) <= 1e-10 * max(1.0, abs(e)), (e, min(e_k))

@@ -162,6 +162,9 @@ U1_CASES = [
     (12, 5, [(1, 1.0, 0.7), (2, 0.5, 0.5)]),
     (14, 7, [(1, 1.0, 0.8)]),
     (10, 1, [(1, 1.0, 1.0)]),
+    # beyond 64 sites (the reference's lists have no such limit)
+    (70, 3, [(1, 1.0, 0.9), (2, 0.4, 0.3)]),
+    (130, 2, [(1, 1.0, 1.0)]),
 ]
 
 
