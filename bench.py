@@ -138,6 +138,9 @@ def dsf_workload(args):
     Ks = [int(round(1000 * m / 64)) for m in range(64)]
     steps = args.dsf_steps
     out = {}
+    block_tune = {}
+    if args.tune:
+        q3.set_option("block_autotune", 1)
     for prec, name in ((q3.FP32, "fp32"), (q3.FP64, "fp64")):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -145,6 +148,12 @@ def dsf_workload(args):
         torch.cuda.synchronize()
         out[name] = (a, b, n, st, time.perf_counter() - t0)
         log(f"batch {name}: {out[name][4]:.2f} s, {st}")
+        if args.tune:
+            tab = q3.block_tune_last()
+            block_tune[name] = {"chosen": [int(x) for x in tab[-1][:3]] + [round(float(tab[-1][3]), 3)],
+                                "candidates": [[int(x) for x in r[:3]] + [round(float(r[3]), 3)] for r in tab[:-1]]}
+            log(f"block stencil autotune {name}: (lines, threads, slabs, ms) {block_tune[name]['chosen']}")
+    q3.set_option("block_autotune", 0)
     a32, b32, n32, st32, t32 = out["fp32"]
     a64, b64, n64, st64, t64 = out["fp64"]
     om = np.linspace(0.0, 5.0, 2001)
@@ -194,7 +203,7 @@ def dsf_workload(args):
            "fp32_vs_fp64_S_eta0.1_rel_err_max": float(np.max(errs_wide)),
            "fp32_vs_fp64_first10_coef_rel_err_median": float(np.median(coef)),
            "fp32_vs_fp64_first10_coef_rel_err_max": float(np.max(coef)),
-           "ground_state_seconds": t_gs, "E0": E0}
+           "ground_state_seconds": t_gs, "E0": E0, "block_autotune": block_tune}
     print(json.dumps(res), flush=True)
 
 
@@ -363,13 +372,13 @@ def main():
     # timed H.v and the solvers below use the chosen configuration)
     tune = {}
     if args.tune:
-        log("autotuning the H.v launch (kernel, tile schedule, grid) ...")
+        log("autotuning the H.v launch (kernel family, then its tile / schedule parameters) ...")
         for name, prec in (("fp32", q3.FP32), ("fp64", q3.FP64)):
             best, bms, table = sec.hv_autotune(prec, n_rep=5)
-            tune[name] = {"kernel": best[0], "xcd": best[1], "grid_quarters": best[2], "ms": bms,
-                          "candidates": [[int(r[0]), int(r[1]), int(r[2]), round(float(r[3]), 4)] for r in table]}
-            log(f"autotune {name}: kernel {best[0]} xcd {best[1]} grid {best[2]}/4: {bms:.4f} ms "
-                f"(default config {float(table[0][3]):.4f} ms)")
+            tune[name] = dict(best, ms=bms, candidates=[[int(x) for x in r[:10]] + [round(float(r[10]), 4)]
+                                                        for r in table])
+            log(f"autotune {name}: {best}: {bms:.4f} ms (default config {float(table[0][10]):.4f} ms, "
+                f"{len(table)} candidates)")
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)
     res = {}

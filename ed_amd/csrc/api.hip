@@ -133,6 +133,9 @@ int qs3x_set_option(const char *name, double value) {
     } else if (n == "block_stencil_threads") {
       QS3X_REQUIRE(value == 0 || value == 256 || value == 512, "block_stencil_threads must be 0, 256 or 512");
       options().block_stencil_threads = (int)value;
+    } else if (n == "block_autotune") {
+      QS3X_REQUIRE(value == 0 || value == 1, "block_autotune must be 0 or 1");
+      options().block_autotune = (int)value;
     } else if (n == "block_stencil_slabs") {
       QS3X_REQUIRE(value >= 0 && value <= 1024, "block_stencil_slabs must be 0..1024");
       options().block_stencil_slabs = (int)value;
@@ -486,27 +489,6 @@ int qs3x_hv_autotune(qs3x_sector *s, int32_t prec, int32_t n_rep, int32_t *best,
     hipEvent_t e0, e1;
     QS3X_HIP(hipEventCreate(&e0));
     QS3X_HIP(hipEventCreate(&e1));
-    // Candidates: every H.v kernel that applies (the launcher falls through
-    // to the next one when a kernel does not apply, so duplicates are timed
-    // and dropped by the order of the list), two tile schedules (the default,
-    // and round robin or 8 XCDs), four grid sizes (4/4, 3/4, 2/4, 1/4 of
-    // the resident grid: fewer resident blocks can keep more of a large
-    // vector's gathers in cache).
-    std::vector<qs3x_sector::HvTune> cand;
-    const int xd = hv_xcd_for(prec);  // the default schedule first
-    for (int k = 0; k <= 3; ++k)
-      for (int x : {xd, xd == 1 ? 8 : 1})
-        for (int q : {4, 3, 2, 1}) {
-          qs3x_sector::HvTune c;
-          c.set = true;
-          c.kernel = k;
-          c.xcd = x;
-          c.grid_q = q;
-          cand.push_back(c);
-        }
-    double best_t = 1e300;
-    qs3x_sector::HvTune best_c;
-    int nres = 0;
     auto timed = [&](int reps) {
       QS3X_HIP(hipEventRecord(e0, st));
       for (int r = 0; r < reps; ++r) launch_hv(s, prec, b.p, a.p, nullptr, nullptr, nullptr, st);
@@ -516,45 +498,110 @@ int qs3x_hv_autotune(qs3x_sector *s, int32_t prec, int32_t n_rep, int32_t *best,
       QS3X_HIP(hipEventElapsedTime(&t, e0, e1));
       return (double)t / reps;
     };
-    // A candidate whose first (timed) launch is over 3x the best so far is not
-    // repeated (its time is recorded negated); when that happens to a
-    // kernel's first configuration, its other configurations are not run
-    // (time 0).  Slow kernels (e.g. the general-lattice kernel on a long
-    // chain) then cost one launch each.
-    int skip_kernel = -1;
-    for (const auto &c : cand) {
-      double ms = 0.0;
-      if (c.kernel != skip_kernel) {
-        tn = c;
-        const double first = timed(1);
-        if (first > 3.0 * best_t) {
-          ms = -first;
-          if (c.xcd == xd && c.grid_q == 4) skip_kernel = c.kernel;  // the kernel's first configuration
-        } else {
-          ms = timed(n_rep);
-          // a later candidate must win by 2 %, so that run-to-run noise does
-          // not move the choice away from the earlier (default) configurations
-          if (ms < 0.98 * best_t) {
-            best_t = ms;
-            best_c = c;
-          }
+    const int xd = hv_xcd_for(prec);  // the default schedule first
+    const int xalt = xd == 1 ? 8 : 1;
+    double best_t = 1e300;
+    qs3x_sector::HvTune best_c;
+    int best_family = -1;
+    int nres = 0;
+    // One candidate: a launch that is warm (its first launch builds the
+    // kernel's plan) is timed once; over 3x the best so far it is recorded
+    // negated and not repeated, else timed over n_rep launches.  A later
+    // candidate must win by 2 %, so that noise does not move the choice off
+    // the earlier (default) configurations.  Returns the family that ran.
+    auto run = [&](const qs3x_sector::HvTune &c, bool warm) {
+      tn = c;
+      if (!warm) launch_hv(s, prec, b.p, a.p, nullptr, nullptr, nullptr, st);  // plans, LDS attributes
+      const int fam = s->last_family;
+      const double first = timed(1);
+      double ms;
+      if (first > 3.0 * best_t) {
+        ms = -first;
+      } else {
+        ms = timed(n_rep);
+        if (ms < 0.98 * best_t) {
+          best_t = ms;
+          best_c = c;
+          best_family = fam;
         }
       }
       if (results && nres < max_results) {
-        results[4 * nres] = c.kernel;
-        results[4 * nres + 1] = c.xcd;
-        results[4 * nres + 2] = c.grid_q;
-        results[4 * nres + 3] = ms;
+        double *r = results + 11 * (size_t)nres;
+        const double row[11] = {(double)fam, (double)c.kernel, (double)c.stencil, (double)c.st_lines,
+                                (double)c.st_slabs, (double)c.st_depth, (double)c.pairs, (double)c.chunk,
+                                (double)c.xcd, (double)c.grid_q, ms};
+        for (int k = 0; k < 11; ++k) r[k] = row[k];
       }
       ++nres;
+      return fam;
+    };
+    auto base = [&](int kernel, int stencil) {
+      qs3x_sector::HvTune c;
+      c.set = true;
+      c.kernel = kernel;
+      c.stencil = stencil;
+      c.xcd = xd;
+      c.grid_q = 4;
+      return c;
+    };
+    // Stage 1: the kernel families, each in its default configuration.  The
+    // launcher falls through to the next family where one does not apply; a
+    // family seen before is not timed again.
+    bool seen[8] = {false, false, false, false, false, false, false, false};
+    for (int k = -1; k <= 3; ++k) {
+      const qs3x_sector::HvTune c = base(std::max(k, 0), k < 0 ? 1 : (k == 0 ? 0 : 1));
+      tn = c;
+      launch_hv(s, prec, b.p, a.p, nullptr, nullptr, nullptr, st);
+      const int fam = s->last_family;
+      if (fam < 0 || fam >= 8 || seen[fam]) continue;
+      seen[fam] = true;
+      run(c, true);  // the launch above was its warm-up
+    }
+    // Stage 2: the winning family's own parameters.
+    const qs3x_sector::HvTune w = best_c;
+    if (best_family == 5) {
+      // stencil: tile height, task slabs, slab windows in flight
+      for (int B : {2, 4, 6, 8})
+        for (int sl : {4, 8, 16, 32})
+          for (int d : {2, 3}) {
+            qs3x_sector::HvTune c = w;
+            c.st_lines = B;
+            c.st_slabs = sl;
+            c.st_depth = d;
+            run(c, false);
+          }
+    } else if (best_family == 0) {
+      // nearest-neighbour kernel: rows per lane (FP32), interleaved chunk,
+      // schedule, grid
+      for (int pr : {-1, 0}) {
+        if (pr == 0 && prec == QS3X_FP64) continue;
+        for (int ch : {0, 3, 5, 7})
+          for (int x : {xd, xalt})
+            for (int q : {4, 2}) {
+              qs3x_sector::HvTune c = w;
+              c.pairs = pr;
+              c.chunk = ch;
+              c.xcd = x;
+              c.grid_q = q;
+              run(c, false);
+            }
+      }
+    } else if (best_family >= 0) {
+      for (int x : {xd, xalt})
+        for (int q : {4, 3, 2, 1}) {
+          qs3x_sector::HvTune c = w;
+          c.xcd = x;
+          c.grid_q = q;
+          run(c, false);
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     tn = best_c;
     if (best) {
-      best[0] = best_c.kernel;
-      best[1] = best_c.xcd;
-      best[2] = best_c.grid_q;
+      const int v[10] = {best_family, best_c.kernel, best_c.stencil, best_c.st_lines, best_c.st_slabs,
+                         best_c.st_depth, best_c.pairs, best_c.chunk, best_c.xcd, best_c.grid_q};
+      for (int k = 0; k < 10; ++k) best[k] = v[k];
     }
     if (best_ms) *best_ms = best_t;
     if (n_results) *n_results = std::min(nres, max_results);

@@ -24,6 +24,7 @@
 //   conj(explist_q(ell)).
 // * The recurrence keeps per-q scalars in one device array [slot][q].
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -896,6 +897,13 @@ void launch_sminus(int prec, int grid, hipStream_t st, const SecView &gs, const 
                        (float2 *)F, norm, part);
 }
 
+// The last block-stencil tuning: rows (lines, threads, slabs, ms), the
+// chosen shape last.
+std::vector<std::array<double, 4>> &block_tune_table() {
+  static std::vector<std::array<double, 4>> t;
+  return t;
+}
+
 int cus() {
   static int c = 0;
   if (!c) {
@@ -913,6 +921,16 @@ int cus() {
 using namespace qs3x;
 
 extern "C" {
+
+int qs3x_block_tune_last(int32_t max_rows, int32_t *n_rows, double *rows) {
+  return guarded4([&] {
+    QS3X_REQUIRE(n_rows && (rows || max_rows == 0) && max_rows >= 0, "qs3x_block_tune_last: bad argument");
+    const auto &t = block_tune_table();
+    *n_rows = (int32_t)std::min<size_t>(t.size(), (size_t)max_rows);
+    for (int32_t i = 0; i < *n_rows; ++i)
+      for (int k = 0; k < 4; ++k) rows[4 * i + k] = t[i][k];
+  });
+}
 
 int qs3x_sector_build_all_k(qs3x_model *m, int32_t nod, qs3x_sector **out) {
   return guarded4([&] {
@@ -1085,44 +1103,82 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
     int gridSt = 0, gridSl = 0, ntasks = 0, ntS = 256;
     DevBuf<int4> st_tasks;
     BlockStencilArgs sarg{};
-    if (options().block_stencil && stencil_applies(&tgt)) {
-      // defaults: the fastest measured at L = 1000, N_down = 4 in both
-      // precisions (12 lines, 512 threads, 32 slabs: FP32 9.9 ms, FP64 20.5 ms
-      // per launch, against 12.3 and 25.2 ms with 6 lines, 256 threads, 8 slabs)
-      const int B = options().block_stencil_lines > 0 ? options().block_stencil_lines : 12;
-      const int slabs = options().block_stencil_slabs > 0 ? options().block_stencil_slabs : 32;
-      ntS = options().block_stencil_threads > 0 ? options().block_stencil_threads : 512;
+    std::vector<int4> st_lines;
+    const bool use_stencil = options().block_stencil && stencil_applies(&tgt);
+    // one tile shape: kernel, task list, grid
+    auto setup_stencil = [&](int B, int nt, int slabs) {
+      ntS = nt;
       hvs = block_stencil_kernel(prec, B, ntS, &lds_s, slabs);
       QS3X_REQUIRE(hvs, "qs3x_dsf_sminus_batch: no block stencil kernel for these lines per tile and threads "
                         "(256 threads: 2..14 lines, even; 512: 4, 8 or 12)");
       QS3X_REQUIRE(lds_s <= 156 * 1024, "qs3x_dsf_sminus_batch: block stencil tile over 156 KB of LDS "
                                         "(fewer lines or slabs)");
-      std::vector<int4> lines;
-      stencil_tables(&tgt, &lines);
+      if (st_lines.empty()) stencil_tables(&tgt, &st_lines);
       const int groups = prec == QS3X_FP64 ? kQ / 8 : kQ / 16;
-      std::vector<int4> tasks = stencil_tasks(lines, L, kSA, B, slabs, groups);
+      std::vector<int4> tasks = stencil_tasks(st_lines, L, kSA, B, slabs, groups);
       ntasks = (int)tasks.size();
       st_tasks.alloc(std::max<size_t>(tasks.size(), 1));
       if (!tasks.empty()) st_tasks.upload(tasks.data(), tasks.size());
       if (lds_s > 64 * 1024)
         QS3X_HIP(hipFuncSetAttribute((const void *)hvs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_s));
       int bpc = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, hvs, ntS, lds_s) != hipSuccess || bpc < 1)
-        bpc = 1;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, hvs, ntS, lds_s) != hipSuccess || bpc < 1) bpc = 1;
       // a multiple of the 8 XCDs: block g takes tasks 8 m + g % 8, its XCD's tiles
       gridSt = (int)std::min<int64_t>((int64_t)cus() * bpc, std::max(ntasks, 8));
       gridSt = std::max(8, gridSt / 8 * 8);
-      gridSl = tgt.nslow > 0 ? (int)std::min<int64_t>((int64_t)cus() * nbpc, ((tgt.nslow + 63) / 64 + kBWaves - 1) / kBWaves)
+      gridSl = tgt.nslow > 0 ? (int)std::min<int64_t>((int64_t)cus() * nbpc,
+                                                      ((tgt.nslow + 63) / 64 + kBWaves - 1) / kBWaves)
                              : 0;
       sarg = BlockStencilArgs{tgt.d_lines.p, st_tasks.p, ntasks};
       QS3X_REQUIRE((int64_t)(gridSt + gridSl) * kQ * 2 <= (int64_t)part.n,
                    "qs3x_dsf_sminus_batch: partial buffer (stencil)");
+    };
+    if (use_stencil) {
+      // defaults: the fastest measured at L = 1000, N_down = 4 in both
+      // precisions (12 lines, 512 threads, 32 slabs: FP32 9.9 ms, FP64 20.5 ms
+      // per launch, against 12.3 and 25.2 ms with 6 lines, 256 threads, 8 slabs)
+      setup_stencil(options().block_stencil_lines > 0 ? options().block_stencil_lines : 12,
+                    options().block_stencil_threads > 0 ? options().block_stencil_threads : 512,
+                    options().block_stencil_slabs > 0 ? options().block_stencil_slabs : 32);
     }
     const double sztot_new = (double)L / 2.0 - (double)NT;
     const double c = (prec == QS3X_FP32) ? m->ezz : 0.0;
     hipEvent_t e0, e1;
     QS3X_HIP(hipEventCreate(&e0));
     QS3X_HIP(hipEventCreate(&e1));
+    // Option block_autotune: time one block stencil launch per tile shape
+    // (lines, threads, slabs) on the S^- vectors and keep the fastest for the
+    // recurrence (qs3x_block_tune_last reports the table).
+    block_tune_table().clear();
+    if (use_stencil && options().block_autotune) {
+      static const int kCand[][3] = {{12, 512, 32}, {12, 512, 16}, {12, 512, 64}, {8, 512, 32}, {4, 512, 32},
+                                     {6, 256, 32}, {8, 256, 32}, {14, 256, 32}, {10, 256, 16}, {6, 256, 8}};
+      double best_ms = 1e300;
+      int bi = 0;
+      for (int k = 0; k < (int)(sizeof(kCand) / sizeof(kCand[0])); ++k) {
+        setup_stencil(kCand[k][0], kCand[k][1], kCand[k][2]);
+        BlockHvArgs t = a;
+        t.x = U[0].p;
+        t.y = U[1].p;
+        t.iscale = SCALE;
+        hipLaunchKernelGGL(hvs, dim3(gridSt), dim3(ntS), lds_s, st, t, sarg);  // warm-up
+        QS3X_HIP(hipGetLastError());
+        QS3X_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < 2; ++r) hipLaunchKernelGGL(hvs, dim3(gridSt), dim3(ntS), lds_s, st, t, sarg);
+        QS3X_HIP(hipEventRecord(e1, st));
+        QS3X_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        QS3X_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const double per = ms / 2.0;
+        block_tune_table().push_back({(double)kCand[k][0], (double)kCand[k][1], (double)kCand[k][2], per});
+        if (per < 0.98 * best_ms) {
+          best_ms = per;
+          bi = k;
+        }
+      }
+      setup_stencil(kCand[bi][0], kCand[bi][1], kCand[bi][2]);
+      block_tune_table().push_back({(double)kCand[bi][0], (double)kCand[bi][1], (double)kCand[bi][2], best_ms});
+    }
     float hv_ms_total = 0;
     for (int i = 0; i < steps; ++i) {
       void *cur = U[i % 3].p, *prev = U[(i + 2) % 3].p, *w = U[(i + 1) % 3].p;

@@ -853,6 +853,7 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   HvArgs a = make_args(s, v0, v1, scale_dev, dot_partials, prec);
   const qs3x_sector::HvTune &tn = s->tune[prec == QS3X_FP64 ? 0 : 1];
   if (tn.set) a.nxcd = tn.xcd;
+  if (tn.set && tn.chunk >= 0) a.chunk_log2 = tn.chunk;
   // a tuned configuration divides the resident grid (never enlarges it, so the
   // callers' partial-sum buffers keep their size)
   auto tuned = [&](int g) { return tn.set ? std::max(1, g * tn.grid_q / 4) : g; };
@@ -880,7 +881,7 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
   if (pick <= 0 && s->chain && s->mode == kIndexAnchored && !normlist && s->model->nn_chain && s->d_bitmap.p) {
     int sg = 0;
     size_t slds = 0;
-    if (launch_hv_stencil(s, prec, a, st, &sg, &slds)) {
+    if (!(tn.set && tn.stencil == 0) && launch_hv_stencil(s, prec, a, st, &sg, &slds)) {
       if (nblocks_out) *nblocks_out = sg;
       s->last_family = 5;
       s->last_lds = (int64_t)slds;
@@ -888,7 +889,7 @@ void launch_hv(qs3x_sector *s, int prec, void *v0, const void *v1, const double 
     }
     const size_t lds = chain_nn_lds_bytes(s->nod, s->model->NOS, (int)s->model->bt_jxy2.size(), prec);
     const bool uniform = s->model->bt_jxy2.size() == 1;
-    const bool pairs = (((uintptr_t)a.v0 | (uintptr_t)a.v1) & 15u) == 0;
+    const bool pairs = (((uintptr_t)a.v0 | (uintptr_t)a.v1) & 15u) == 0 && !(tn.set && tn.pairs == 0);
     KFn f = (lds <= kNnMaxLds) ? chain_nn_kernel(s->nod, prec, s->real_phase, uniform, key32(s), pairs) : nullptr;
     if (f) {
       allow_lds(f, lds);

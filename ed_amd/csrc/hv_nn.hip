@@ -1262,13 +1262,12 @@ SlowFn slow_kernel(int prec, bool realph, bool key32, bool fused = false) {
 }
 
 // Band width and task length: options hv_stencil_lines / hv_stencil_slabs,
-// or by default the fastest measured at L = 1000 (FP32: 8 lines, 8 slabs,
-// 0.223 ms; FP64: 4 lines, 16 slabs, 0.383 ms -- FP64 is limited by its
-// registers, 5 wavefronts per SIMD, so narrower bands keep more blocks
-// resident).
+// or by default the fastest measured at L = 1000 by the autotuner: 8 lines
+// in both precisions, 8 slabs per task in FP32 (0.215 ms) and 16 in FP64
+// (0.362 ms, against 0.379 ms with 4 lines).
 int stencil_lines(int prec) {
   if (options().hv_stencil_lines > 0) return std::min(options().hv_stencil_lines, kMaxLines);
-  return prec == QS3X_FP64 ? 4 : 8;
+  return 8;
 }
 int stencil_depth(int prec, bool fused) {
   if (options().hv_stencil_depth > 0) return options().hv_stencil_depth;
@@ -1349,9 +1348,7 @@ std::vector<int4> stencil_tasks(const std::vector<int4> &lines, int L, int segA,
 namespace {
 
 // Plan for one precision of the single-vector stencil (groups = 1).
-void stencil_plan(int prec, qs3x_sector::StencilPlan &pl, const std::vector<int4> &lines, int L) {
-  const int slabs = stencil_slabs(prec);
-  const int B = stencil_lines(prec);
+void stencil_plan(int prec, qs3x_sector::StencilPlan &pl, const std::vector<int4> &lines, int L, int B, int slabs) {
   std::vector<int4> tasks = stencil_tasks(lines, L, kSegA, B, slabs, 1);
   const size_t es = prec == QS3X_FP64 ? 16 : 8;
   pl.nlines = 0;
@@ -1423,13 +1420,19 @@ static bool launch_stencil_impl(qs3x_sector *s, int prec, const HvArgs &a, hipSt
   if (!options().hv_stencil || !stencil_applies(s)) return false;
   const int L = s->model->NOS;
   qs3x_sector::StencilPlan &pl = s->stencil[prec == QS3X_FP64 ? 0 : 1];
-  if (pl.tried && (pl.slabs != stencil_slabs(prec) || pl.B != stencil_lines(prec))) pl.tried = false;  // options changed
+  // tile height, task slabs and windows in flight: the sector's autotuned
+  // values, else the options, else the per-precision defaults
+  const qs3x_sector::HvTune &tn = s->tune[prec == QS3X_FP64 ? 0 : 1];
+  const int want_B = (tn.set && tn.st_lines > 0) ? tn.st_lines : stencil_lines(prec);
+  const int want_slabs = (tn.set && tn.st_slabs > 0) ? tn.st_slabs : stencil_slabs(prec);
+  const int depth = (tn.set && tn.st_depth > 0) ? tn.st_depth : stencil_depth(prec, fused);
+  if (pl.tried && (pl.slabs != want_slabs || pl.B != want_B)) pl.tried = false;  // another tile
   if (!pl.tried) {
     pl.tried = true;
     pl.ok = false;
     std::vector<int4> lines;
     stencil_tables(s, &lines);
-    stencil_plan(prec, pl, lines, L);
+    stencil_plan(prec, pl, lines, L, want_B, want_slabs);
     // the plan's tables went up on the library stream; the launch below may
     // be on the caller's stream, which nothing orders after it
     QS3X_HIP(hipStreamSynchronize(stream()));
@@ -1438,7 +1441,7 @@ static bool launch_stencil_impl(qs3x_sector *s, int prec, const HvArgs &a, hipSt
   if (!pl.ok) return false;
   int kb = 1;
   while ((1 << kb) <= L) ++kb;
-  SFn f = stencil_kernel(prec, s->real_phase, fused, pl.B, stencil_depth(prec, fused));
+  SFn f = stencil_kernel(prec, s->real_phase, fused, pl.B, depth);
   SlowFn fs = slow_kernel(prec, s->real_phase, 3 * kb <= 32, fused);
   // partial sums per block: 2 (complex <v, Hv>) or 4 in the fused step
   const size_t np = fused ? 4 : 2;
@@ -1454,7 +1457,9 @@ static bool launch_stencil_impl(qs3x_sector *s, int prec, const HvArgs &a, hipSt
   }
   const int slow_grid = s->nslow > 0 ? (int)std::min<int64_t>((s->nslow + kThreads - 1) / kThreads, 512) : 0;
   int &pgrid = fused ? pl.grid_fused : pl.grid;
-  if (pgrid == 0) {
+  const void *&pfor = fused ? pl.grid_fused_for : pl.grid_for;
+  if (pgrid == 0 || pfor != (const void *)f) {
+    pfor = (const void *)f;
     // resident blocks per CU, from the kernel's registers and LDS (160 KB per
     // CU, 512 VGPRs per lane slot on each of 4 SIMDs, at most 8 wavefronts
     // per SIMD), a multiple of the 8 XCDs so that block g (on XCD g % 8) takes

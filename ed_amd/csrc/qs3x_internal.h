@@ -137,6 +137,10 @@ struct qs3x_sector {
     int kernel = 0;  // first kernel tried, as option hv_kernel
     int xcd = 1;     // L2 domains of the tile schedule, as option hv_xcd
     int grid_q = 4;  // quarters of the resident grid launched (4: all of it)
+    int stencil = 1;  // kernel 0 on a stencil sector: 1 the stencil, 0 the nearest-neighbour kernel
+    int st_lines = 0, st_slabs = 0, st_depth = 0;  // stencil tile height, task slabs, windows in flight (0: options)
+    int chunk = -1;   // nearest-neighbour kernels: log2 of the interleaved XCD chunk (-1: option hv_chunk)
+    int pairs = -1;   // FP32 nearest-neighbour kernel: 0 one row per lane, -1 the default rule (two where it applies)
   } tune[2];
   // the last H.v launch: kernel family (0 nearest-neighbour chain, 1
   // batched-load chain, 2 per-hop chain, 3 general lattice, 4 U(1)-only) and
@@ -149,6 +153,7 @@ struct qs3x_sector {
   struct StencilPlan {
     bool tried = false, ok = false;
     int B = 0, slabs = 0, ntasks = 0, grid = 0, grid_fused = 0;
+    const void *grid_for = nullptr, *grid_fused_for = nullptr;  // the kernels the grids were sized for
     int64_t nlines = 0;  // non-empty (n3, n4) lines
     size_t lds = 0;
     qs3x::DevBuf<int4> tasks;

@@ -65,6 +65,7 @@ def _load():
         "qs3x_hv_stencil_info": [_P, _i32, _P],
         "qs3x_hv_autotune": [_P, _i32, _i32, _P, _P, _i32, _P, _P],
         "qs3x_hv_tune_clear": [_P],
+        "qs3x_block_tune_last": [_i32, _P, _P],
         "qs3x_lanczos": [_P, _i32, _i64, _i64, _i64, _dbl, _P, _u64, _P, _P, _P],
         "qs3x_trlanczos": [_P, _i32, _i32, _i32, _i32, _i64, _dbl, _P, _u64, _P, _P, _P],
         "qs3x_solver_trace": [_P, _i64, _P],
@@ -276,15 +277,17 @@ class Sector:
         return float(ms[0])
 
     def hv_autotune(self, prec=FP64, n_rep=5):
-        """qs3x_hv_autotune: times every launch candidate and keeps the fastest
-        for this sector and precision.  Returns (best (kernel, xcd, grid
-        quarters), best ms, array of rows (kernel, xcd, quarters, ms))."""
-        best = np.zeros(3, dtype=np.int32)
+        """qs3x_hv_autotune: times the kernel families, then the winner's own
+        parameters, and keeps the fastest for this sector and precision.
+        Returns (best dict, best ms, array of rows (family, kernel, stencil,
+        lines, slabs, depth, pairs, chunk, xcd, quarters, ms))."""
+        best = np.zeros(10, dtype=np.int32)
         ms = np.zeros(1)
-        res = np.zeros((64, 4))
+        res = np.zeros((128, 11))
         n = np.zeros(1, dtype=np.int32)
-        _check(_lib.qs3x_hv_autotune(self._h, int(prec), int(n_rep), _p(best), _p(ms), 64, _p(n), _p(res)))
-        return tuple(int(x) for x in best), float(ms[0]), res[:int(n[0])].copy()
+        _check(_lib.qs3x_hv_autotune(self._h, int(prec), int(n_rep), _p(best), _p(ms), 128, _p(n), _p(res)))
+        keys = ("family", "kernel", "stencil", "lines", "slabs", "depth", "pairs", "chunk", "xcd", "grid_quarters")
+        return dict(zip(keys, (int(x) for x in best))), float(ms[0]), res[:int(n[0])].copy()
 
     def hv_tune_clear(self):
         _check(_lib.qs3x_hv_tune_clear(self._h))
@@ -419,6 +422,16 @@ def dsf_sminus_batch(gs, K, pos, psi, itr_dsf, rfield=0.0, prec=FP32):
     keys = ["seconds_setup", "seconds_sminus", "seconds_recurrence", "ms_per_block_step", "dim_all_k", "steps",
             "block_stencil", "rows_per_row_kernel"]
     return alpha, beta2, norm1, dict(zip(keys, stats[:8].tolist()))
+
+
+def block_tune_last():
+    """qs3x_block_tune_last: rows (lines, threads, slabs, ms) of the last
+    batched DSF block-stencil tuning (option block_autotune), the chosen
+    shape last; empty without it."""
+    rows = np.zeros((32, 4))
+    n = np.zeros(1, dtype=np.int32)
+    _check(_lib.qs3x_block_tune_last(32, _p(n), _p(rows)))
+    return rows[:int(n[0])].copy()
 
 
 def dsf_analysis(alpha, beta2, norm1, E0, omega, eta=0.04, reference_quirk=True):

@@ -254,6 +254,12 @@ int qs3x_sector_build_all_k(qs3x_model *m, int32_t nod, qs3x_sector **out);
 int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const double *pos,
                           const double *psi, int32_t itr_dsf, double rfield, int32_t prec,
                           double *alpha, double *beta2, double *norm1, double *stats);
+/* The tile shapes the last qs3x_dsf_sminus_batch timed with option
+ * block_autotune 1 (no counterpart in the reference): rows of 4 doubles
+ * (lines per tile, threads per block, slabs per task, ms per block stencil
+ * launch), the chosen shape repeated as the last row; none without the
+ * option or where the block stencil does not apply. */
+int qs3x_block_tune_last(int32_t max_rows, int32_t *n_rows, double *rows);
 
 /* Plain Lanczos for nsec sectors of one chain model at once.  Replaces nsec
  * calls of lanczos_routines (lanczos.f90:14-152; energies only, no second
@@ -348,19 +354,25 @@ int qs3x_hv_stencil_info(const qs3x_sector *s, int32_t prec, int64_t *info);
 /* Times n_rep device H . v applications (events on the library stream) and
  * returns the mean milliseconds per application. */
 int qs3x_time_hv(qs3x_sector *s, int32_t prec, int32_t n_rep, double *ms);
-/* Launch autotuner for the sector's H . v (no counterpart in the reference):
- * times n_rep applications of every candidate configuration -- first kernel
- * tried (as option hv_kernel 0..3), tile schedule (hv_xcd 8 or 1), grid (4,
- * 3, 2 or 1 quarters of the resident grid) -- on a random vector, and keeps the
- * fastest (a later candidate must beat the best so far by 2 %) for this
- * sector and precision; every later H . v of the sector
- * (solvers included) uses it.  best (3 ints, may be NULL): kernel, xcd,
- * grid quarters; best_ms its time; results (4 doubles per candidate, up to
- * max_results): kernel, xcd, quarters, ms, where ms < 0 is the single timed
- * launch of a candidate over 3x slower than the best so far and ms = 0 a
- * candidate not run (its kernel's first configuration was that slow).  All
- * candidates compute the same H . v (each kernel is checked against the
- * oracle by the test suite). */
+/* Launch autotuner for the sector's H . v (no counterpart in the reference).
+ * Times n_rep applications on a random vector of, first, every kernel family
+ * that applies in its default configuration (stencil, nearest-neighbour
+ * chain, batched-load chain, per-hop chain, general lattice), then the
+ * winning family's own parameters: for the stencil its tile height (2, 4, 6,
+ * 8 lines), task length (4, 8, 16, 32 slabs) and slab windows in flight (2,
+ * 3); for the nearest-neighbour kernel rows per lane (FP32: two where they
+ * apply, or one), interleaved XCD chunk (2^0, 2^3, 2^5, 2^7 tiles), tile
+ * schedule (per-XCD or round robin) and grid (4 or 2 quarters of the
+ * resident grid); for the others the schedule and grid (4..1 quarters).  It
+ * keeps the fastest (a later candidate must beat the best so far by 2 %) for
+ * this sector and precision; every later H . v of the sector (solvers
+ * included) uses it.  best (10 ints, may be NULL): family (as
+ * qs3x_hv_last_launch), kernel, stencil, lines, slabs, depth, pairs, chunk,
+ * xcd, grid quarters; best_ms its time; results (11 doubles per candidate, up
+ * to max_results): the same 10 fields and ms, where ms < 0 is the single
+ * timed launch of a candidate over 3x slower than the best so far.  Every
+ * candidate computes the same H . v (each kernel and parameter is checked
+ * against the oracle by the test suite). */
 int qs3x_hv_autotune(qs3x_sector *s, int32_t prec, int32_t n_rep, int32_t *best, double *best_ms,
                      int32_t max_results, int32_t *n_results, double *results);
 /* Forgets the autotuned configurations of the sector (both precisions). */

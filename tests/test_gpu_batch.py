@@ -225,3 +225,37 @@ def test_block_stencil_matches_per_row_block_kernel(L, steps, prec_name):
             ea = np.max(np.abs(a_[j, :k] - a0[j, :k])) / max(1.0, np.max(np.abs(a0[j, :k])))
             eb = np.max(np.abs(b_[j, :k] - b0[j, :k])) / max(1e-300, np.max(np.abs(b0[j, :k])))
             assert ea <= tol_a and eb <= tol_b, (j, Ks[j], ea, eb)
+
+
+def test_block_stencil_autotune():
+    """Option block_autotune: the batch times every block stencil tile shape
+    on its S^- vectors, keeps the fastest (a later shape must win by 2 %),
+    and its coefficients equal those of the default shape (FP64, 1e-11 over
+    the first steps)."""
+    print("[gpu] block stencil autotune", flush=True)
+    q3 = _q3()
+    L, steps = 120, 12
+    gm = q3.Model(L, L, 1, 1, q3.chain_bonds(L, [(1, 1.0, 0.8)]))
+    gs = q3.Sector.build(gm, 3, (0.0, 0.0, 0.0))
+    E0, psi, _ = q3.lanczos_routines(gs, maxitr=2000, lnc_ene_conv=1e-13, prec=q3.FP64, seed=5)
+    pos = np.array([[i, 0.0, 0.0] for i in range(L)], dtype=float)
+    Ks = [int(round(L * m / 64)) % L for m in range(64)]
+    a0, b0, n0, _ = q3.dsf_sminus_batch(gs, Ks, pos, psi, steps, prec=q3.FP64)
+    assert len(q3.block_tune_last()) == 0
+    q3.set_option("block_autotune", 1)
+    try:
+        a1, b1, n1, st1 = q3.dsf_sminus_batch(gs, Ks, pos, psi, steps, prec=q3.FP64)
+    finally:
+        q3.set_option("block_autotune", 0)
+    tab = q3.block_tune_last()
+    assert st1["block_stencil"] == 1 and len(tab) >= 3
+    timed, chosen = tab[:-1], tab[-1]
+    assert all(r[3] > 0 for r in timed)
+    assert chosen[3] <= min(r[3] for r in timed) / 0.98
+    assert any(np.array_equal(chosen[:3], r[:3]) for r in timed)
+    assert np.array_equal(n0, n1)
+    for j in range(64):
+        if n0[j] <= 1e-20:
+            continue
+        assert np.max(np.abs(a1[j, :6] - a0[j, :6])) <= 1e-11 * max(1.0, np.max(np.abs(a0[j, :6])))
+        assert np.max(np.abs(b1[j, :6] - b0[j, :6])) <= 1e-11 * max(1e-300, np.max(np.abs(b0[j, :6])))
