@@ -268,16 +268,21 @@ def sectors_workload(args):
     print(json.dumps(out), flush=True)
 
 
-def load_traffic(kernel):
+def load_traffic(kernel, launch=None):
     """HBM bytes per launch of the given kernel from the committed counter
     profile (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a plain copy), or
-    None when the profile covers another kernel or launch configuration."""
+    None when the profile covers another kernel or launch configuration
+    (tile height and task slabs of the stencil)."""
     p = os.path.join(ROOT, "profiles", "hv_stencil_L1000_N4_traffic.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
-        t = json.load(f)
-    return t.get(kernel)
+        t = json.load(f).get(kernel)
+    if t and launch is not None:
+        for k, v in t.get("launch", {}).items():
+            if launch.get(k) != v:
+                return None
+    return t
 
 
 def hv_bytes(sec, prec, sv):
@@ -316,6 +321,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", action="store_true",
                     help="run the H.v launch autotuner first (the timed H.v and the solves then use its choice)")
+    ap.add_argument("--lanczos-steps", type=int, default=200,
+                    help="plain Lanczos steps timed per precision (ms per step of the solver path)")
     ap.add_argument("--NOK", type=int, default=4)
     # one first cycle holds the whole convergence (the spectrum's low end is a
     # quasi-continuum, level spacing ~1e-4, so small bases restart many
@@ -420,6 +427,23 @@ def main():
         del x, y
         torch.cuda.empty_cache()
 
+    # A fixed number of plain Lanczos steps per precision, without the
+    # convergence checks' effect on the count (ms per step of the path the
+    # solvers take: the fused step in FP64, the H.v and update kernels in FP32)
+    lz = {}
+    for name, prec in (("fp32", q3.FP32), ("fp64", q3.FP64)):
+        kw = dict(minitr=10 ** 6, itrint=10 ** 6, lnc_ene_conv=1e-30, prec=prec, seed=1, want_vector=False)
+        q3.lanczos_routines(sec, maxitr=20, **kw)  # buffers, plans
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, _, n = q3.lanczos_routines(sec, maxitr=args.lanczos_steps, **kw)
+        torch.cuda.synchronize()
+        fam = sec.hv_last_launch()[0]
+        lz[name] = {"ms_per_step": (time.perf_counter() - t0) * 1e3 / n, "steps": n,
+                    "path": "fused step (k_hv_stencil4 FUSED + k_finalize_fused)" if fam == 6 else
+                            "H.v + k_update3 (+ finalize)"}
+        log(f"plain Lanczos {name}: {lz[name]['ms_per_step']:.4f} ms per step over {n} steps, {lz[name]['path']}")
+
     solve = {}
     if not args.no_solve:
         q3.set_option("verbose", 1)  # one stderr line per restart / 100 Lanczos steps
@@ -522,10 +546,7 @@ def main():
             log(f"cpu baseline unavailable: {e}")
 
     f32 = res["fp32"]
-    traffic = load_traffic(f32["kernel"])
-    if traffic and traffic.get("launch") != {k: f32["launch"][k] for k in ("tile_lines", "slabs", "workgroups")
-                                             if k in f32["launch"]}:
-        traffic = None
+    traffic = load_traffic(f32["kernel"], f32["launch"])
     tb = (traffic or {}).get("bytes_per_launch")
     roof = {"bound": "hbm", "achieved": f32["alg_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": f32["alg_GBs"] / HBM_PEAK_GBS, "traffic": tb,
@@ -538,6 +559,7 @@ def main():
             "all_gathers_model_bytes": f32["all_gathers_bytes"],
             "kernel": f32["kernel"], "launch_config": f32["launch"], "avg_launch_ms": f32["ms"]}
     f64 = res["fp64"]
+    tb64 = (load_traffic(f64["kernel"], f64["launch"]) or {}).get("bytes_per_launch")
     out = {
         "metric": METRIC,
         "value": f32["hv_per_s"],
@@ -560,9 +582,11 @@ def main():
         "ms_per_hv_fp64": f64["ms"],
         "roofline_fp64": {"achieved": f64["alg_GBs"], "frac": f64["alg_GBs"] / HBM_PEAK_GBS,
                           "alg_bytes_per_launch": f64["alg_bytes"], "kernel": f64["kernel"],
-                          "launch_config": f64["launch"]},
+                          "launch_config": f64["launch"], "traffic": tb64,
+                          "counter_GBs": (tb64 / (f64["ms"] * 1e-3) / 1e9) if tb64 else None},
         "sector_build_s": t_build,
         "hv_autotune": tune,
+        "plain_lanczos_step": lz,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
