@@ -335,6 +335,9 @@ def main():
     ap.add_argument("--reorth", type=int, default=2,
                     help="thick-restart reorthogonalisation: 0 full, 1 locked+local, 2 locked at restart only")
     ap.add_argument("--cgs", type=int, default=1, help="Gram-Schmidt passes per thick-restart step")
+    ap.add_argument("--fp64-lanczos", type=int, default=0,
+                    help="FP32-mixed thick restart: at most this many FP64 Lanczos steps from the Rayleigh-Ritz "
+                         "vector until the residual bound certifies E0 (0: the block-Krylov refinement instead)")
     ap.add_argument("--fp32-floor", type=float, default=1e-7)
     ap.add_argument("--check-every", type=int, default=5,
                     help="thick restart: convergence check every n steps inside cycles (0: at restarts)")
@@ -451,6 +454,7 @@ def main():
         q3.set_option("tr_cgs_passes", args.cgs)
         q3.set_option("fp32_floor", args.fp32_floor)
         q3.set_option("tr_check_every", args.check_every)
+        q3.set_option("fp64_lanczos", args.fp64_lanczos)
 
         def thick(prec, nom):
             torch.cuda.synchronize()
@@ -511,6 +515,7 @@ def main():
                  "fp32_mixed_certificate": cert,
                  "fp32_mixed_settings": {"NOK": args.NOK, "NOM": args.NOM, "tr_reorth": args.reorth,
                                          "cgs_passes": args.cgs, "fp32_floor": args.fp32_floor,
+                                         "fp64_lanczos": args.fp64_lanczos,
                                          "conv": args.conv_mixed, "check_every": args.check_every},
                  "fp64_plain_settings": {"conv": args.conv_plain, "itrint": 5, "minitr": 20}}
         if args.NOM64 > 0:

@@ -88,6 +88,9 @@ int qs3x_set_option(const char *name, double value) {
     } else if (n == "fp64_refine") {
       QS3X_REQUIRE(value >= 0 && value <= 32, "fp64_refine must be 0..32");
       options().fp64_refine = (int)value;
+    } else if (n == "fp64_lanczos") {
+      QS3X_REQUIRE(value >= 0 && value <= 100000, "fp64_lanczos must be 0..100000");
+      options().fp64_lanczos = (int)value;
     } else if (n == "rot_vec") {
       QS3X_REQUIRE(value == 0 || value == 1, "rot_vec must be 0 or 1");
       options().rot_vec = (int)value;

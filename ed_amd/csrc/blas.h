@@ -40,6 +40,10 @@ struct Options {
   int tr_cgs_passes = 2;      // classical Gram-Schmidt passes per step
   double fp32_floor = 1e-6;   // |delta theta| that ends the FP32 phase of the mixed thick restart
   int fp64_refine = 4;         // FP32-mixed thick restart: block Krylov steps of the final FP64 refinement (0: none)
+  // FP32-mixed thick restart with one wanted pair: instead of the block
+  // Krylov refinement, at most this many plain FP64 Lanczos steps from the
+  // Rayleigh-Ritz vector, until the residual bound certifies the value (0: off)
+  int fp64_lanczos = 0;
   int rot_vec = 1;            // Ritz rotation: 16-byte loads (aligned slots) instead of 4/8-byte loads
   int tr_check_every = 0;     // FP32 thick-restart stage: convergence check every n steps inside cycles (0: at restarts)
   // H.v kernel choice: 0 = fastest available (nearest-neighbour chain, then

@@ -688,6 +688,118 @@ TRResult tr_cycles_checked(TR &t, int64_t maxitr, double tol, double abs_floor, 
   return r;
 }
 
+// FP64 Lanczos refinement of one Ritz vector (option fp64_lanczos): the
+// plain FP64 recurrence of lanczos.f90:50-92 (fused steps where they apply)
+// started from y0 (device, complex double), without storing its basis.
+// Every 5 steps from step 10 the tridiagonal's lowest eigenvalue theta and
+// the residual norm of its Ritz vector, r = beta_m |s_m| (beta_m the last
+// step's beta, s_m the last component of the normalised tridiagonal
+// eigenvector: an identity of the recurrence, no vector needed), give the
+// Kato-Temple bound r^2 / gap; the run stops when it is <= conv |theta| or
+// after maxsteps.  With out (device, complex double) the Ritz vector of the
+// accepted step is rebuilt by a second pass of the same steps.
+struct Lz64Result {
+  double theta = 0.0, resid = 0.0;
+  int64_t steps = 0;
+  bool certified = false;
+};
+
+Lz64Result lanczos_refine64(qs3x_sector *s, const void *y0, int64_t maxsteps, double gap, double conv,
+                            double *out, hipStream_t st, double t_origin, int64_t hv_base) {
+  const int prec = QS3X_FP64;
+  const int64_t dim = s->dim;
+  const int64_t mitr = std::max<int64_t>(std::min<int64_t>(maxsteps, dim), 1);
+  const bool fused = hv_fused_available(s, prec);
+  DevBuf<unsigned char> U[4];
+  for (int k = 0; k < (fused ? 4 : 3); ++k) U[k].alloc(16 * (size_t)dim);
+  const int64_t A_REC = 0, A_OUT = mitr, BETA = 2 * mitr, SCALE = 3 * mitr + 1;
+  DevBuf<double> sc((size_t)4 * mitr + 4);
+  DevBuf<double> part((size_t)4 * kMaxBlocks);
+  const double shift = fused ? s->model->ezz : 0.0;
+  void *u0 = fused ? U[1].p : U[0].p;
+  auto start = [&] {
+    vec_copy_scale(prec, u0, QS3X_FP64, y0, dim, nullptr, -1, 1.0, st);
+    const int nb = vec_dot(prec, u0, prec, u0, dim, part.p, st);
+    finalize(part.p, nb, sc.p, 2, (int)(SCALE + mitr + 1), (int)SCALE, 0.0, st);
+  };
+  int nb = 0;
+  auto step = [&](int64_t i, double *pt) {
+    if (fused) {
+      void *pprev = U[(i + 1) % 2].p, *pout = U[i % 2].p;
+      void *vprev = i > 0 ? U[2 + (i + 1) % 2].p : nullptr, *xout = U[2 + i % 2].p;
+      QS3X_REQUIRE(launch_hv_fused(s, prec, pout, xout, pprev, vprev, sc.p, (int)(SCALE + i), (int)(A_REC + i - 1),
+                                   i > 0 ? (int)(BETA + i - 1) : -1, pt, &nb, st),
+                   "lanczos_refine64: fused step unavailable");
+      if (pt) finalize_fused(pt, nb, sc.p, (int)(A_REC + i), (int)(A_OUT + i), (int)(BETA + i), (int)(SCALE + i + 1),
+                             shift, st);
+      return;
+    }
+    void *cur = U[i % 3].p, *prev = U[(i + 2) % 3].p, *w = U[(i + 1) % 3].p;
+    launch_hv(s, prec, w, cur, sc.p + SCALE + i, pt, &nb, st);
+    if (pt) finalize(pt, nb, sc.p, 1, (int)(A_REC + i), (int)(A_OUT + i), shift, st);
+    nb = vec_update3(prec, w, cur, prev, dim, sc.p, (int)(A_REC + i), (int)(SCALE + i),
+                     i > 0 ? (int)(BETA + i - 1) : -1, (int)(SCALE + i - 1), pt, st);
+    if (pt) finalize(pt, nb, sc.p, 2, (int)(BETA + i), (int)(SCALE + i + 1), 0.0, st);
+  };
+  Lz64Result r;
+  std::vector<double> ha, hb, z;
+  start();
+  for (int64_t i = 0; i < mitr; ++i) {
+    step(i, part.p);
+    const int64_t n = i + 1;
+    if (!(n == mitr || (n >= 10 && n % 5 == 0))) continue;
+    ha.resize(n);
+    hb.resize(n);
+    QS3X_HIP(hipMemcpyAsync(ha.data(), sc.p + A_OUT, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    QS3X_HIP(hipMemcpyAsync(hb.data(), sc.p + BETA, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    QS3X_HIP(hipStreamSynchronize(st));
+    const double E = n == 1 ? ha[0] : tridiag_lowest(ha.data(), hb.data(), (int)n);
+    z.assign(n, 0.0);
+    if (n == 1) z[0] = 1.0;
+    else tridiag_vector(ha.data(), hb.data(), (int)n, E, z.data());
+    double zn = 0.0;
+    for (double x : z) zn += x * x;
+    const double res = hb[n - 1] * std::fabs(z[n - 1]) / std::sqrt(std::max(zn, 1e-300));
+    r.theta = E;
+    r.resid = res;
+    r.steps = n;
+    trace_point(t_origin, hv_base + n, E, QS3X_FP64);
+    if (options().verbose)
+      std::fprintf(stderr, "[qs3x] fp64 lanczos refinement step %lld: theta %.15g residual %.3e bound %.3e\n",
+                   (long long)n, E, res, res * res / std::max(gap, 1e-300));
+    if (gap > 0.0 && res * res / gap <= conv * std::fabs(E)) {
+      r.certified = true;
+      break;
+    }
+  }
+  if (out) {
+    // second pass: psi = sum_i z_i v_i over the accepted steps
+    const int64_t n = r.steps;
+    z.assign(n, 0.0);
+    if (n == 1) z[0] = 1.0;
+    else tridiag_vector(ha.data(), hb.data(), (int)n, r.theta, z.data());
+    QS3X_HIP(hipMemsetAsync(out, 0, 16 * (size_t)dim, st));
+    start();
+    if (fused) {
+      for (int64_t i = 0; i < n; ++i) {
+        step(i, nullptr);
+        vec_axpy_acc(prec, out, U[2 + i % 2].p, dim, z[i], sc.p, -1, st);
+      }
+    } else {
+      vec_axpy_acc(prec, out, U[0].p, dim, z[0], sc.p, (int)SCALE, st);
+      for (int64_t i = 0; i + 1 < n; ++i) {
+        step(i, nullptr);
+        vec_axpy_acc(prec, out, U[(i + 1) % 3].p, dim, z[i + 1], sc.p, (int)(SCALE + i + 1), st);
+      }
+    }
+    const int nb2 = vec_dot(QS3X_FP64, out, QS3X_FP64, out, dim, part.p, st);
+    finalize(part.p, nb2, sc.p, 2, 0, 1, 0.0, st);
+    vec_copy_scale(QS3X_FP64, out, QS3X_FP64, out, dim, sc.p, 1, 1.0, st);
+  }
+  QS3X_HIP(hipStreamSynchronize(st));
+  return r;
+}
+
 // FP64 Rayleigh-Ritz on k vectors: slots 0..k-1 of t (FP64) hold the
 // candidate vectors (any scale).  They are orthonormalised (CGS2), W = H Y is
 // formed in slots k..2k-1, G = Y^H W is diagonalised on the host (as the real
@@ -988,7 +1100,27 @@ int qs3x_trlanczos(qs3x_sector *s, int32_t prec, int32_t NOE, int32_t NOK, int32
       theta = rr.theta;
       rr_j0 = 2 * NOK;
       const int S = options().fp64_refine;
-      if (S > 0) {
+      const int m_lz = options().fp64_lanczos;
+      if (m_lz > 0 && NOE == 1) {
+        // FP64 Lanczos from the Rayleigh-Ritz vector until the residual bound
+        // certifies the value (its basis is not stored; the vector, if
+        // wanted, comes from a second pass into slot rr_j0)
+        DevBuf<double> yout(psi ? (size_t)2 * dim : 0);
+        Lz64Result lz = lanczos_refine64(s, t64.slot(rr_j0), m_lz, gap32[0], conv, psi ? yout.p : nullptr, st, t0,
+                                         hv32 + t64.hv_count);
+        t64.hv_count += lz.steps * (psi ? 2 : 1);
+        if (psi)
+          QS3X_HIP(hipMemcpyAsync(t64.slot(rr_j0), yout.p, 16 * (size_t)dim, hipMemcpyDeviceToDevice, st));
+        QS3X_HIP(hipStreamSynchronize(st));
+        theta[0] = lz.theta;
+        resid[0] = lz.resid;
+        if (options().verbose)
+          std::fprintf(stderr, "[qs3x] fp64 lanczos refinement: %.15g residual %.3e bound %.3e after %lld steps%s\n",
+                       lz.theta, lz.resid, lz.resid * lz.resid / std::max(gap32[0], 1e-300), (long long)lz.steps,
+                       lz.certified ? " (certified)" : "");
+        certified = lz.certified;
+        ok = ok || lz.certified;
+      } else if (S > 0) {
         // final FP64 residual correction of the Ritz vectors: Rayleigh-Ritz
         // on their block Krylov space.  Values not accepted above are
         // accepted when the refined pairs pass the residual bound.

@@ -283,3 +283,33 @@ def test_thick_restart_many_restarts_small_basis(prec_name):
     assert st["restarts"] >= 5
     tol = 1e-10 if prec == q3.FP64 else 1e-8
     assert abs(ene[0] - E64) <= tol * abs(E64), (ene[0], E64, st)
+
+
+@pytest.mark.parametrize("L,N,K,fam", [(16, 6, 2, [(1, 1.0, 1.0), (2, 0.4, 0.4)]), (24, 4, 0, [(1, 1.0, 0.8)]),
+                                       (30, 5, 3, [(1, 1.0, 1.0)])])
+def test_fp32_mixed_fp64_lanczos_certificate(L, N, K, fam):
+    """Option fp64_lanczos: the FP32-mixed thick restart ends with plain FP64
+    Lanczos (lanczos.f90:50-92) from its Rayleigh-Ritz vector until the
+    residual bound r^2 / gap certifies E0 (eigen_solver.f90:150-162 is the
+    reference's stop test).  The value and the returned vector are checked
+    against dense diagonalisation of the oracle's H: E0 to 1e-10, the vector's
+    residual |H psi - E psi| <= 1e-5, and the certificate is the one reported."""
+    print(f"[gpu] fp64 lanczos certificate L={L} N={N} K={K}", flush=True)
+    q3 = _q3()
+    om, osec, gm, gsec, rk = _setup(L, N, fam, K=K)
+    H = om.full_hamiltonian(osec)
+    w = np.linalg.eigvalsh(H)
+    q3.set_option("fp64_lanczos", 300)
+    q3.set_option("fp64_refine", 0)
+    try:
+        ene, psi, st = q3.my_trlanczos_routines_complex(gsec, NOE=1, NOK=4, NOM=40, maxitr=400, lnc_ene_conv=1e-12,
+                                                         prec=q3.FP32, seed=3)
+        cert = q3.solver_certificate()
+    finally:
+        q3.set_option("fp64_lanczos", 0)
+        q3.set_option("fp64_refine", 4)
+    assert abs(ene[0] - w[0]) <= 1e-10 * max(1.0, abs(w[0])), (ene[0], w[0])
+    v = psi[:, 0] if psi.ndim == 2 else psi
+    assert abs(np.linalg.norm(v) - 1.0) <= 1e-10
+    assert np.linalg.norm(H @ v - ene[0] * v) <= 1e-5
+    assert cert["accepted"] == 1.0 and cert["bound"] <= 1e-12 * abs(ene[0]) + 1e-300
