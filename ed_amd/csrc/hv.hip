@@ -523,6 +523,71 @@ __global__ __launch_bounds__(kThreads) void k_hv_u1(HvArgs a, int N) {
   }
 }
 
+// Representative of a sorted configuration nn (1-based sites) on an
+// LX x LY x LZ lattice: the colex-minimal translation image, as
+// representative_SQ (state_lists.f90:384-416) finds it by trying every
+// translation (i, j, k), i = 1..LX innermost, keeping the first strictly
+// smaller image.  A minimal image has a down spin in the plane z = 0, one in
+// the plane y = 0 and one in x = 0 (else the translation by -1 along that
+// axis lowers every site index), so every translation that reaches the
+// minimum shifts some spin's x, some spin's y and some spin's z to 0: the
+// N^d distinct such shifts are the only candidates (d the lattice
+// dimension), and among those reaching the minimum the reference keeps the
+// first in its loop order, or none (ell = 0) when nn itself is minimal.
+// Falls back to all LX LY LZ translations when the candidates would be as
+// many.  best: the representative; ei, ei_best as the scan sets them.
+__device__ __forceinline__ void representative_candidates(const int *nn, int N, int LX, int LY, int LZ, int *nd,
+                                                          int *best, int &ei, int &ei_best) {
+  for (int q = 0; q < N; ++q) best[q] = nn[q];
+  ei = 0;
+  ei_best = 0;
+  int cx[kMaxNod], cy[kMaxNod], cz[kMaxNod];
+  int nx = 0, ny = 0, nz = 0;
+  for (int q = 0; q < N; ++q) {
+    const int s0 = nn[q] - 1;
+    const int x = s0 % LX, y = (s0 / LX) % LY, z = s0 / (LX * LY);
+    const int tx = (LX - x) % LX, ty = (LY - y) % LY, tz = (LZ - z) % LZ;
+    bool dx = false, dy = false, dz = false;
+    for (int r = 0; r < nx; ++r) dx |= cx[r] == tx;
+    for (int r = 0; r < ny; ++r) dy |= cy[r] == ty;
+    for (int r = 0; r < nz; ++r) dz |= cz[r] == tz;
+    if (!dx) cx[nx++] = tx;
+    if (!dy) cy[ny++] = ty;
+    if (!dz) cz[nz++] = tz;
+  }
+  const bool full = (int64_t)nx * ny * nz >= (int64_t)LX * LY * LZ;
+  const int kz = full ? LZ : nz, ky = full ? LY : ny, kx = full ? LX : nx;
+  int64_t best_order = -1;  // loop order of the kept translation (-1: nn itself)
+  for (int c3 = 0; c3 < kz; ++c3)
+    for (int c2 = 0; c2 < ky; ++c2)
+      for (int c1 = 0; c1 < kx; ++c1) {
+        // translation by (i, j, k) single shifts, i = 1..LX (i = LX: none)
+        const int i = full ? c1 + 1 : (cx[c1] == 0 ? LX : cx[c1]);
+        const int j = full ? c2 + 1 : (cy[c2] == 0 ? LY : cy[c2]);
+        const int k = full ? c3 + 1 : (cz[c3] == 0 ? LZ : cz[c3]);
+        if (i == LX && j == LY && k == LZ) continue;  // nn itself
+        for (int q = 0; q < N; ++q) nd[q] = shift_site(nn[q], i, j, k, LX, LY, LZ);
+        for (int q = 1; q < N; ++q) {
+          int v = nd[q], z = q - 1;
+          while (z >= 0 && nd[z] > v) {
+            nd[z + 1] = nd[z];
+            --z;
+          }
+          nd[z + 1] = v;
+        }
+        int cmp = 0;  // image against best, colex (largest site first)
+        for (int l = N - 1; l >= 0 && cmp == 0; --l) cmp = nd[l] < best[l] ? -1 : (nd[l] > best[l] ? 1 : 0);
+        const int64_t order = ((int64_t)(k - 1) * LY + (j - 1)) * LX + (i - 1);
+        if (cmp < 0 || (cmp == 0 && best_order >= 0 && order < best_order)) {
+          if (cmp < 0)
+            for (int q = 0; q < N; ++q) best[q] = nd[q];
+          best_order = order;
+          ei = i + (LX + 1) * (j + (LY + 1) * k);
+          ei_best = 1;
+        }
+      }
+}
+
 template <class R, bool REALPH, bool COUNT>
 __global__ __launch_bounds__(kThreads) void k_hv_general(HvArgs a, int N) {
   using C = typename CT<R>::T;
@@ -561,30 +626,8 @@ __global__ __launch_bounds__(kThreads) void k_hv_general(HvArgs a, int N) {
           --pos;
         }
         nn[pos] = t;
-        for (int q = 0; q < N; ++q) best[q] = nn[q];
         int ei = 0, ei_best = 0;
-        for (int k = 1; k <= a.LZ; ++k)
-          for (int j = 1; j <= a.LY; ++j)
-            for (int i = 1; i <= a.LX; ++i) {
-              for (int q = 0; q < N; ++q) nd[q] = shift_site(nn[q], i, j, k, a.LX, a.LY, a.LZ);
-              for (int q = 1; q < N; ++q) {
-                int v = nd[q], z = q - 1;
-                while (z >= 0 && nd[z] > v) {
-                  nd[z + 1] = nd[z];
-                  --z;
-                }
-                nd[z + 1] = v;
-              }
-              for (int l = N - 1; l >= 0; --l) {
-                if (nd[l] > best[l]) break;
-                if (nd[l] < best[l]) {
-                  ei = i + (a.LX + 1) * (j + (a.LY + 1) * k);
-                  ei_best = 1;
-                  for (int q = 0; q <= l; ++q) best[q] = nd[q];
-                  break;
-                }
-              }
-            }
+        representative_candidates(nn, N, a.LX, a.LY, a.LZ, nd, best, ei, ei_best);
         int64_t rank = best[0];
         for (int i2 = 2; i2 <= N; ++i2) rank += a.binom[(int64_t)(best[i2 - 1] - 1) * a.kdim + i2];
         int64_t id = -1;

@@ -951,42 +951,24 @@ __global__ __launch_bounds__(kThreads) void k_hv_stencil4(HvArgs a, StencilArgs 
       const int b = b0 - 1 + j;
       const int g1 = av - 1, g2 = b - av, g3 = c - b;
       const int x0 = j * kSegW + ai + 1;  // window index of (a, b) in a slot
+      // The 8 neighbours are summed without tests: a hop that does not exist
+      // (g1, g2 or g3 = 1: two down spins on one site) lands on a window
+      // point that holds no row -- a = 1, a = b or b = c lie outside every
+      // line's range of a -- and the window holds 0 there.  Only the
+      // diagonal counts the antiparallel bonds.
       C x = r0[x0];
-      R sx = 0, sy = 0, arx = 0, ary = 0, alx, aly;
-      int nanti = 2;  // spin c to the right and the anchor to site L: W > 1
-      C y = rp[x0];   // (a, b, c + 1)
-      sx += y.x;
-      sy += y.y;
-      y = rp[x0 + kSegW + 1];  // anchor to site L: (a + 1, b + 1, c + 1)
-      alx = y.x;
-      aly = y.y;
-      if (g2 > 1) {
-        y = r0[x0 + 1];  // (a + 1, b, c)
-        sx += y.x;
-        sy += y.y;
-        y = r0[x0 - kSegW];  // (a, b - 1, c)
-        sx += y.x;
-        sy += y.y;
-        nanti += 2;
-      }
-      if (g1 > 1) {
-        y = r0[x0 - 1];  // (a - 1, b, c)
-        sx += y.x;
-        sy += y.y;
-        y = rm[x0 - kSegW - 1];  // anchor to site 2: (a - 1, b - 1, c - 1)
-        arx = y.x;
-        ary = y.y;
-        nanti += 2;
-      }
-      if (g3 > 1) {
-        y = r0[x0 + kSegW];  // (a, b + 1, c)
-        sx += y.x;
-        sy += y.y;
-        y = rm[x0];  // (a, b, c - 1)
-        sx += y.x;
-        sy += y.y;
-        nanti += 2;
-      }
+      const C yc1 = rp[x0];              // (a, b, c + 1): spin c to the right (W > 1)
+      const C yal = rp[x0 + kSegW + 1];  // anchor to site L: (a + 1, b + 1, c + 1)
+      const C ya1 = r0[x0 + 1];          // (a + 1, b, c)         g2 > 1
+      const C yb0 = r0[x0 - kSegW];      // (a, b - 1, c)         g2 > 1
+      const C ya0 = r0[x0 - 1];          // (a - 1, b, c)         g1 > 1
+      const C yar = rm[x0 - kSegW - 1];  // anchor to site 2: (a - 1, b - 1, c - 1), g1 > 1
+      const C yb1 = r0[x0 + kSegW];      // (a, b + 1, c)         g3 > 1
+      const C yc0 = rm[x0];              // (a, b, c - 1)         g3 > 1
+      R sx = (yc1.x + ya1.x) + (yb0.x + ya0.x) + (yb1.x + yc0.x);
+      R sy = (yc1.y + ya1.y) + (yb0.y + ya0.y) + (yb1.y + yc0.y);
+      const R alx = yal.x, aly = yal.y, arx = yar.x, ary = yar.y;
+      const int nanti = 2 + 2 * ((g1 > 1) + (g2 > 1) + (g3 > 1));
       if constexpr (REALPH) {
         sx += arx + alx;
         sy += ary + aly;
