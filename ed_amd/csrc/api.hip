@@ -110,15 +110,29 @@ int qs3x_set_option(const char *name, double value) {
       QS3X_REQUIRE(value == 0 || value == 1, "hv_stencil must be 0 or 1");
       options().hv_stencil = (int)value;
     } else if (n == "hv_stencil_lines") {
-      QS3X_REQUIRE(value == 0 || value == 2 || value == 4 || value == 6 || value == 8,
-                   "hv_stencil_lines must be 0 (per precision), 2, 4, 6 or 8");
+      QS3X_REQUIRE(value == 0 || (value >= 2 && value <= 14 && (int)value % 2 == 0),
+                   "hv_stencil_lines must be 0 (per precision) or even, 2 .. 14 (10 .. 14: column form with one "
+                   "position per lane; the slab-window form takes at most 8)");
       options().hv_stencil_lines = (int)value;
     } else if (n == "hv_stencil_slabs") {
       QS3X_REQUIRE(value >= 0 && value <= 1024, "hv_stencil_slabs must be 0..1024");
       options().hv_stencil_slabs = (int)value;
     } else if (n == "hv_stencil_depth") {
-      QS3X_REQUIRE(value == 0 || value == 2 || value == 3, "hv_stencil_depth must be 0 (per precision), 2 or 3");
+      QS3X_REQUIRE(value == 0 || value == 2 || value == 3 || value == 4,
+                   "hv_stencil_depth must be 0 (per precision), 2, 3 (slab-window form) or 4 (pair form; the "
+                   "slab-window form takes 3, the column form 2)");
       options().hv_stencil_depth = (int)value;
+    } else if (n == "hv_stencil_kernel") {
+      QS3X_REQUIRE(value == 0 || value == 1 || value == 2,
+                   "hv_stencil_kernel must be 0 (slab window), 1 (column) or 2 (pair: FP32, else column)");
+      options().hv_stencil_kernel = (int)value;
+    } else if (n == "hv_stencil_span") {
+      QS3X_REQUIRE(value == 0 || value == 1 || value == 2,
+                   "hv_stencil_span must be 0 (per precision), 1 or 2 (2: FP32, up to 8 lines per tile)");
+      options().hv_stencil_span = (int)value;
+    } else if (n == "hv_stencil_order") {
+      QS3X_REQUIRE(value == 0 || value == 1, "hv_stencil_order must be 0 (tile-major) or 1 (n4-major)");
+      options().hv_stencil_order = (int)value;
     } else if (n == "hv_stencil_side") {
       QS3X_REQUIRE(value == 0 || value == 1, "hv_stencil_side must be 0 or 1");
       options().hv_stencil_side = (int)value;

@@ -62,9 +62,19 @@ struct Options {
   // Gap-lattice stencil H.v (nearest-neighbour chains, N_down = 4, uniform
   // bonds): 1 = use it where it applies (default), 0 = never.
   int hv_stencil = 1;
-  int hv_stencil_lines = 0;   // n3-lines per tile of the stencil kernel (0: per precision; 2, 4, 6 or 8)
+  int hv_stencil_lines = 0;   // n3-lines per tile of the stencil kernel (0: per precision; even, 2 .. 14)
   int hv_stencil_slabs = 0;   // n4-slabs per stencil task (0: per precision)
-  int hv_stencil_depth = 0;   // slab windows in flight per block (0: per precision; 2 or 3)
+  int hv_stencil_depth = 0;   // slab windows in flight per block (0: 2; 3 slab-window form, 4 pair form)
+  // stencil kernel form: 0 = slab window (threads take the rows of each slab
+  // in turn), 1 = column (a thread keeps its window positions for a task's
+  // sweep), 2 = pair (column form with two adjacent positions per lane and
+  // 16-byte accesses; FP32, the column form in FP64)
+  int hv_stencil_kernel = 2;
+  int hv_stencil_span = 0;  // column form: window positions per lane, 1, 2 or 4 (0: per precision)
+  // single-vector stencil task order: 0 tile-major (each XCD a run of tiles,
+  // every tile swept over its slabs), 1 n4-major (chunks of slabs in turn;
+  // 2-5 % faster at L = 1000)
+  int hv_stencil_order = 1;
   // stencil: key-search rows on a side stream, overlapping the stencil launch
   // (1), or after it on the caller's stream (0, the default: the overlap
   // measured no gain at L = 1000 -- the stencil fills the CUs -- and slowed

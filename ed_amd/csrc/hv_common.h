@@ -196,7 +196,30 @@ __device__ __forceinline__ typename CT<R>::T fused_v(const FusedCoef<R> &f, type
 
 using KFn = void (*)(HvArgs);
 
+// Gap-lattice stencil H.v (hv_nn.hip: slab-window form; hv_col.hip: column
+// form).  A task is a tile of kSegA values of n2 by B lines (n3) swept over
+// the slabs n4 = c0 .. c1 - 1; a slab's window holds B + 2 lines of kSegW
+// positions (a halo of one each side) in one slot of a kRing-slot LDS ring.
+constexpr int kSegA = 62;
+constexpr int kSegW = 64;
+constexpr int kRing = 4;
+struct StencilArgs {
+  const int4 *lines;    // [(L + 1) * c + b]: first row, lowest a, highest a + 1 (lowest = highest + 1: empty)
+  const int4 *tasks;    // a0, b0, c0, c1 in bits 0..15 of .w (c1 <= c0: nothing to do)
+  int ntasks;
+  int B;                // lines per tile
+};
+using SFn = void (*)(HvArgs, StencilArgs);
+
 }  // namespace hvk
+
+// Column-form stencil kernel (hv_col.hip) for B lines per tile and P window
+// positions per lane (1, 2 or 4); null if not instantiated.
+hvk::SFn stencil_col_kernel(int prec, bool realph, bool fused, int B, int P);
+// Pair-form stencil kernel (hv_pair.hip; FP32, two adjacent window positions
+// per lane, tiles 124 values of n2 wide) for B lines per tile and D slab
+// windows in flight (2 or 4); null if not instantiated.
+hvk::SFn stencil_pair_kernel(bool realph, bool fused, int B, int D);
 
 // Nearest-neighbour chain kernels (hv_nn.hip): null when no specialisation
 // exists for this N / precision.

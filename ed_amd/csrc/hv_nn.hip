@@ -824,17 +824,9 @@ __global__ __launch_bounds__(kThreads) void __attribute__((amdgpu_waves_per_eu(5
 // times instead of once per hop that reaches it, and no configuration word or
 // index is read.  Rows that are not clear go through the per-wavefront queue
 // and the key search of slow_row, as in the kernels above.
-constexpr int kSegA = 62;   // a-values per tile
-constexpr int kSegW = 64;   // window positions per line (halo of one each side)
-constexpr int kRing = 4;    // LDS slots
-constexpr int kMaxLines = 8;
-
-struct StencilArgs {
-  const int4 *lines;    // [(L + 1) * c + b]: first row, lowest a, highest a + 1 (lowest = highest + 1: empty)
-  const int4 *tasks;    // a0, b0, c0, c1 in bits 0..15 of .w (c1 <= c0: nothing to do)
-  int ntasks;
-  int B;                // lines per tile
-};
+// kSegA a-values per tile, kSegW window positions per line (a halo of one
+// each side), kRing LDS slots: hv_common.h
+constexpr int kMaxLines = 8;  // lines per tile of this (slab-window) form
 
 // FUSED: the fused Lanczos step of HvArgs -- the window holds v = c1 P - c2 v'
 // formed from two loaded vectors when it is stored to LDS, and a row writes v
@@ -1202,7 +1194,6 @@ size_t chain_nn_lds_bytes(int N, int L, int nbt, int prec) {
 // ---------------------------------------------------------------- stencil
 namespace {
 
-using SFn = void (*)(HvArgs, StencilArgs);
 using SlowFn = void (*)(HvArgs, const int32_t *, int, const int32_t *);
 
 template <int BT, int D>
@@ -1225,8 +1216,11 @@ SFn stencil_kernel_d(int prec, bool realph, bool fused, int B) {
     default: return stencil_kernel_b<8, D>(prec, realph, fused);
   }
 }
-// lines per tile: 2, 4, 6 or 8; windows in flight: 2 or 3
-SFn stencil_kernel(int prec, bool realph, bool fused, int B, int depth) {
+// Slab-window form: lines per tile 2, 4, 6 or 8, windows in flight 2 or 3.
+// Column form (hv_col.hip): P window positions per lane (1, 2 or 4), lines
+// per tile 2, 4, 6, 8 (and 12, 14 with P = 1), two windows in flight.
+SFn stencil_kernel(int prec, bool realph, bool fused, int B, int depth, bool col, int P) {
+  if (col) return stencil_col_kernel(prec, realph, fused, B, P);
   return depth >= 3 ? stencil_kernel_d<3>(prec, realph, fused, B) : stencil_kernel_d<2>(prec, realph, fused, B);
 }
 template <bool FU>
@@ -1244,20 +1238,26 @@ SlowFn slow_kernel(int prec, bool realph, bool key32, bool fused = false) {
 }
 
 // Band width and task length: options hv_stencil_lines / hv_stencil_slabs,
-// or by default the fastest measured at L = 1000 by the autotuner: 8 lines
-// in both precisions, 8 slabs per task in FP32 (0.215 ms) and 16 in FP64
-// (0.362 ms, against 0.379 ms with 4 lines).
+// or by default the fastest measured at L = 1000, N_down = 4 (n4-major task
+// order): 10 lines in both precisions (the slab-window form takes 8), 16
+// slabs per task in FP32 (pair form: H.v 0.197 ms, fused Lanczos step 0.404
+// ms) and 8 in FP64 (column form: H.v 0.348 ms, fused step 0.711 ms).
 int stencil_lines(int prec) {
-  if (options().hv_stencil_lines > 0) return std::min(options().hv_stencil_lines, kMaxLines);
-  return 8;
+  if (options().hv_stencil_lines > 0) return options().hv_stencil_lines;
+  return 10;
 }
 int stencil_depth(int prec, bool fused) {
   if (options().hv_stencil_depth > 0) return options().hv_stencil_depth;
   return 2;
 }
+// Window positions per lane of the column form: option hv_stencil_span, or 1.
+int stencil_span(int prec) {
+  if (options().hv_stencil_span > 0) return options().hv_stencil_span;
+  return 1;
+}
 int stencil_slabs(int prec) {
   if (options().hv_stencil_slabs > 0) return options().hv_stencil_slabs;
-  return prec == QS3X_FP64 ? 16 : 8;
+  return prec == QS3X_FP64 ? 8 : 16;
 }
 
 }  // namespace
@@ -1327,17 +1327,74 @@ std::vector<int4> stencil_tasks(const std::vector<int4> &lines, int L, int segA,
   return tasks;
 }
 
+// Tasks of a single-vector stencil sweep in n4-major order: the slabs are cut
+// into chunks [k S, (k + 1) S) at the same places for every tile, and the
+// chunks are taken in turn, all tiles of chunk k before any of chunk k + 1.
+// The blocks of the persistent grid, which take tasks in list order, then
+// work on a few consecutive chunks at any time -- a front a few tens of MB
+// wide moving through the vectors, where tile-major order has them spread
+// over the whole of both vectors.  Within a chunk the tiles are dealt to the
+// 8 XCDs as contiguous runs in (n3, n2) order (task 8 m + x on XCD x), so
+// tiles that share halo lines share an L2.
+std::vector<int4> stencil_tasks_cmajor(const std::vector<int4> &lines, int L, int segA, int B, int slabs) {
+  const int W1 = L + 1;
+  const int na = (L + segA - 1) / segA + 1, nb = (L + B - 1) / B + 1;
+  std::vector<int> cmin((size_t)na * nb, 0x7fffffff), cmax((size_t)na * nb, -1);
+  for (int c = 0; c <= L; ++c)
+    for (int b = 3; b < c; ++b) {
+      const int4 e = lines[(size_t)c * W1 + b];
+      if (e.z <= e.y) continue;
+      const int tb = (b - 3) / B;
+      for (int ta = (e.y - 2) / segA; ta <= (e.z - 1 - 2) / segA; ++ta) {
+        const size_t t = (size_t)tb * na + ta;
+        cmin[t] = std::min(cmin[t], c);
+        cmax[t] = std::max(cmax[t], c);
+      }
+    }
+  const int nx = 8;
+  std::vector<int4> tasks;
+  for (int k = 0; k * slabs <= L; ++k) {
+    const int lo = k * slabs, hi = lo + slabs;
+    std::vector<int4> ch;  // this chunk's tasks, tiles in (b, a) order
+    for (int tb = 0; tb < nb; ++tb)
+      for (int ta = 0; ta < na; ++ta) {
+        const size_t t = (size_t)tb * na + ta;
+        if (cmax[t] < cmin[t] || cmax[t] < lo || cmin[t] >= hi) continue;
+        const int c0 = std::max(lo, cmin[t]), c1 = std::min(hi, cmax[t] + 1);
+        ch.push_back(make_int4(2 + ta * segA, 3 + tb * B, c0, c1));
+      }
+    if (ch.empty()) continue;
+    const size_t per = (ch.size() + nx - 1) / nx;  // tasks per XCD in this chunk
+    for (size_t m = 0; m < per; ++m)
+      for (int x = 0; x < nx; ++x) {
+        const size_t i = (size_t)x * per + m;
+        tasks.push_back(i < ch.size() ? ch[i] : make_int4(0, 0, 0, 0));
+      }
+  }
+  return tasks;
+}
+
 namespace {
 
 // Plan for one precision of the single-vector stencil (groups = 1).
-void stencil_plan(int prec, qs3x_sector::StencilPlan &pl, const std::vector<int4> &lines, int L, int B, int slabs) {
-  std::vector<int4> tasks = stencil_tasks(lines, L, kSegA, B, slabs, 1);
+// form: 0 slab window, 1 column, 2 pair; P: window positions per lane of the
+// column form (tiles 64 P - 2 wide; the slab-window form has P = 1); the pair
+// form's tiles are 124 wide in windows of 128.
+void stencil_plan(int prec, qs3x_sector::StencilPlan &pl, const std::vector<int4> &lines, int L, int B, int slabs,
+                  int P, int form) {
+  const int segA = form == 2 ? 124 : kSegW * P - 2;
+  const int segW = form == 2 ? 128 : kSegW * P;
+  std::vector<int4> tasks = options().hv_stencil_order == 1 ? stencil_tasks_cmajor(lines, L, segA, B, slabs)
+                                                             : stencil_tasks(lines, L, segA, B, slabs, 1);
   const size_t es = prec == QS3X_FP64 ? 16 : 8;
   pl.nlines = 0;
   for (const int4 &e : lines) pl.nlines += e.z > e.y ? 1 : 0;
   pl.B = B;
   pl.slabs = slabs;
-  pl.lds = (size_t)kRing * (B + 2) * kSegW * es + (size_t)(slabs + 2) * (B + 2) * sizeof(int4);
+  pl.P = P;
+  pl.form = form;
+  pl.order = options().hv_stencil_order;
+  pl.lds = (size_t)kRing * (B + 2) * segW * es + (size_t)(slabs + 2) * (B + 2) * sizeof(int4);
   pl.ntasks = (int)tasks.size();
   pl.tasks.alloc(std::max<size_t>(tasks.size(), 1));
   if (!tasks.empty()) pl.tasks.upload(tasks.data(), tasks.size());
@@ -1405,25 +1462,41 @@ static bool launch_stencil_impl(qs3x_sector *s, int prec, const HvArgs &a, hipSt
   // tile height, task slabs and windows in flight: the sector's autotuned
   // values, else the options, else the per-precision defaults
   const qs3x_sector::HvTune &tn = s->tune[prec == QS3X_FP64 ? 0 : 1];
-  const int want_B = (tn.set && tn.st_lines > 0) ? tn.st_lines : stencil_lines(prec);
+  // form: 0 slab window, 1 column, 2 pair (FP32 sectors under 2^28 rows; else
+  // the column form)
+  int form = (tn.set && tn.st_col >= 0) ? tn.st_col : options().hv_stencil_kernel;
+  if (form == 2 && (prec == QS3X_FP64 || s->dim >= ((int64_t)1 << 28))) form = 1;
+  const bool col = form >= 1;
+  int want_B = (tn.set && tn.st_lines > 0) ? tn.st_lines : stencil_lines(prec);
+  if (form == 0) want_B = std::min(want_B, kMaxLines);
+  if (form == 2) want_B = std::min(want_B, 10);
+  int want_P = form == 1 ? ((tn.set && tn.st_span > 0) ? tn.st_span : stencil_span(prec)) : 1;
+  if (want_P > 1 && (want_B > 8 || prec == QS3X_FP64)) want_P = 1;  // wide tiles: FP32, up to 8 lines
   const int want_slabs = (tn.set && tn.st_slabs > 0) ? tn.st_slabs : stencil_slabs(prec);
   const int depth = (tn.set && tn.st_depth > 0) ? tn.st_depth : stencil_depth(prec, fused);
-  if (pl.tried && (pl.slabs != want_slabs || pl.B != want_B)) pl.tried = false;  // another tile
+  if (pl.tried && (pl.slabs != want_slabs || pl.B != want_B || pl.P != want_P || pl.form != form ||
+                   pl.order != options().hv_stencil_order))
+    pl.tried = false;  // another tile, kernel form or task order
   if (!pl.tried) {
     pl.tried = true;
     pl.ok = false;
     std::vector<int4> lines;
     stencil_tables(s, &lines);
-    stencil_plan(prec, pl, lines, L, want_B, want_slabs);
+    stencil_plan(prec, pl, lines, L, want_B, want_slabs, want_P, form);
     // the plan's tables went up on the library stream; the launch below may
     // be on the caller's stream, which nothing orders after it
     QS3X_HIP(hipStreamSynchronize(stream()));
     pl.ok = true;
   }
   if (!pl.ok) return false;
+  QS3X_REQUIRE(pl.lds <= 156 * 1024, "stencil H.v: the tile's LDS ring (" + std::to_string(pl.lds) +
+                                         " bytes) exceeds 156 KB; use fewer lines per tile or a smaller span");
   int kb = 1;
   while ((1 << kb) <= L) ++kb;
-  SFn f = stencil_kernel(prec, s->real_phase, fused, pl.B, depth);
+  SFn f = pl.form == 2 ? stencil_pair_kernel(s->real_phase, fused, pl.B, depth >= 4 ? 4 : 2)
+                       : stencil_kernel(prec, s->real_phase, fused, pl.B, depth, col, pl.P);
+  if (!f && pl.form == 2) f = stencil_pair_kernel(s->real_phase, fused, pl.B, 2);
+  QS3X_REQUIRE(f != nullptr, "stencil H.v: no kernel for this tile height and span");
   SlowFn fs = slow_kernel(prec, s->real_phase, 3 * kb <= 32, fused);
   // partial sums per block: 2 (complex <v, Hv>) or 4 in the fused step
   const size_t np = fused ? 4 : 2;
@@ -1465,7 +1538,9 @@ static bool launch_stencil_impl(qs3x_sector *s, int prec, const HvArgs &a, hipSt
   // The slow rows (a latency-bound tail of about 1 % of the rows) run after
   // the stencil on st, or with option hv_stencil_side on a side stream forked
   // from and joined back into st; the two write disjoint rows and partial sums.
-  if (slow_grid > 0 && !options().hv_stencil_side) {
+  // (the pair form writes every row of its tiles and leaves the rows that are
+  // not clear to the slow-row kernel to overwrite: never on a side stream)
+  if (slow_grid > 0 && (!options().hv_stencil_side || pl.form == 2)) {
     HvArgs b = a;
     if (b.dot_partials) b.dot_partials += np * (size_t)pgrid;
     hipLaunchKernelGGL(f, dim3(pgrid), dim3(kThreads), pl.lds, st, a, sa);

@@ -139,6 +139,8 @@ struct qs3x_sector {
     int grid_q = 4;  // quarters of the resident grid launched (4: all of it)
     int stencil = 1;  // kernel 0 on a stencil sector: 1 the stencil, 0 the nearest-neighbour kernel
     int st_lines = 0, st_slabs = 0, st_depth = 0;  // stencil tile height, task slabs, windows in flight (0: options)
+    int st_col = -1;  // stencil kernel form: 0 slab window, 1 column, 2 pair (-1: option hv_stencil_kernel)
+    int st_span = 0;  // column form: window positions per lane (0: option hv_stencil_span)
     int chunk = -1;   // nearest-neighbour kernels: log2 of the interleaved XCD chunk (-1: option hv_chunk)
     int pairs = -1;   // FP32 nearest-neighbour kernel: 0 one row per lane, -1 the default rule (two where it applies)
   } tune[2];
@@ -152,7 +154,7 @@ struct qs3x_sector {
   // precision, the task list of its launch plan.
   struct StencilPlan {
     bool tried = false, ok = false;
-    int B = 0, slabs = 0, ntasks = 0, grid = 0, grid_fused = 0;
+    int B = 0, slabs = 0, P = 1, form = 0, order = 0, ntasks = 0, grid = 0, grid_fused = 0;  // P: window positions per lane
     const void *grid_for = nullptr, *grid_fused_for = nullptr;  // the kernels the grids were sized for
     int64_t nlines = 0;  // non-empty (n3, n4) lines
     size_t lds = 0;

@@ -88,12 +88,24 @@ int qs3x_set_device(int32_t device);
  *                   (n2, n3) swept over n4 through LDS, every hop of a clear
  *                   row read from there, the other rows in a second launch
  *                   through the key search); 0: the nearest-neighbour chain
- *                   kernel.  "hv_stencil_lines" (1..8; 0, the default: 8 in
- *                   FP32, 4 in FP64) and "hv_stencil_slabs" (0, the default:
- *                   8 in FP32, 16 in FP64) set its tile height and the
- *                   n4-slabs per task; "hv_stencil_side" 1 runs its
- *                   key-search rows on a side stream overlapping the stencil
- *                   (default 0: after it).
+ *                   kernel.  "hv_stencil_kernel" picks how a block's threads
+ *                   take a tile's rows: 0 slab window (the rows of each slab
+ *                   dealt out in turn), 1 column (a thread keeps its window
+ *                   positions for the whole sweep), 2 (default) pair (column
+ *                   form with two adjacent positions per lane and 16-byte
+ *                   accesses, FP32 sectors under 2^28 rows; else column).
+ *                   All three give the same bits.  "hv_stencil_lines"
+ *                   (even, 2..14; 0, the default: 10; the slab-window form
+ *                   takes at most 8, the pair form 10) and
+ *                   "hv_stencil_slabs" (0, the default: 16 in FP32, 8 in
+ *                   FP64) set its tile height and the n4-slabs per task;
+ *                   "hv_stencil_span" (column form, FP32: 1 or 2 window
+ *                   positions 64 apart per lane); "hv_stencil_depth" (2, 3
+ *                   slab-window form, 4 pair form: slab windows in flight);
+ *                   "hv_stencil_order" 1 (default) takes the tasks n4-major,
+ *                   0 tile-major; "hv_stencil_side" 1 runs its key-search
+ *                   rows on a side stream overlapping the stencil (default 0:
+ *                   after it; the pair form always after it).
  *   "dense_solver"  eigensolver of qs3x_full_diag: 0 (default) QR iteration
  *                   (rocSOLVER zheev, the algorithm of LAPACK zheev that the
  *                   reference calls); 1 divide and conquer (zheevd).
