@@ -285,6 +285,16 @@ def load_traffic(kernel, launch=None):
     return t
 
 
+def stencil_kernel_name(prec, form, fused=False):
+    """The stencil kernel of a launch: k_hv_stencil4 (slab-window form),
+    k_hv_stencil_col (column form) or k_hv_stencil_pair (pair form, FP32)."""
+    r = "double" if prec == 0 else "float"
+    f = ", fused" if fused else ""
+    if form == 2:
+        return f"k_hv_stencil_pair<real-phase{f}>"
+    return f"{'k_hv_stencil_col' if form == 1 else 'k_hv_stencil4'}<{r}, real-phase{f}>"
+
+
 def hv_bytes(sec, prec, sv):
     """Algorithmic bytes of one H.v launch, a lower bound: every array the
     launch must read or write counted once -- v_in and v_out (s_v bytes per
@@ -300,8 +310,9 @@ def hv_bytes(sec, prec, sv):
     if fam == 5:
         info = sec.hv_stencil_info(prec)
         b = D * 2 * sv + info["lines"] * 16 + info["slow_rows"] * 12
-        name = f"k_hv_stencil4<{'double' if prec == 0 else 'float'}, real-phase> + k_hv_slow_rows ({tag})"
-        cfg = {"family": "stencil", "tile_lines": info["tile_lines"], "slabs": info["slabs"],
+        name = f"{stencil_kernel_name(prec, info['form'])} + k_hv_slow_rows ({tag})"
+        cfg = {"family": "stencil", "form": ["slab window", "column", "pair"][info["form"]],
+               "tile_lines": info["tile_lines"], "slabs": info["slabs"],
                "workgroups": info["grid"], "slow_rows": info["slow_rows"], "lines": info["lines"], "lds": lds}
     else:
         import math
@@ -443,7 +454,8 @@ def main():
         torch.cuda.synchronize()
         fam = sec.hv_last_launch()[0]
         lz[name] = {"ms_per_step": (time.perf_counter() - t0) * 1e3 / n, "steps": n,
-                    "path": "fused step (k_hv_stencil4 FUSED + k_finalize_fused)" if fam == 6 else
+                    "path": (f"fused step ({stencil_kernel_name(prec, sec.hv_stencil_info(prec)['form'], True)}"
+                             " + k_hv_slow_rows fused + k_finalize_fused)") if fam == 6 else
                             "H.v + k_update3 (+ finalize)"}
         log(f"plain Lanczos {name}: {lz[name]['ms_per_step']:.4f} ms per step over {n} steps, {lz[name]['path']}")
 

@@ -463,6 +463,8 @@ int qs3x_hv_stencil_info(const qs3x_sector *s, int32_t prec, int64_t *info) {
     info[3] = pl.B;
     info[4] = pl.slabs;
     info[5] = pl.grid;
+    info[6] = pl.form;
+    info[7] = pl.P;
   });
 }
 

@@ -266,10 +266,12 @@ class Sector:
 
     def hv_stencil_info(self, prec=FP32):
         """Plan of the stencil H.v (after its first launch): dict of slow_rows,
-        lines, tasks, tile_lines, slabs, grid."""
-        info = np.zeros(6, dtype=np.int64)
+        lines, tasks, tile_lines, slabs, grid, form (0 slab window, 1 column,
+        2 pair) and span (window positions per lane of the column form)."""
+        info = np.zeros(8, dtype=np.int64)
         _check(_lib.qs3x_hv_stencil_info(self._h, int(prec), _p(info)))
-        return dict(zip(["slow_rows", "lines", "tasks", "tile_lines", "slabs", "grid"], info.tolist()))
+        return dict(zip(["slow_rows", "lines", "tasks", "tile_lines", "slabs", "grid", "form", "span"],
+                        info.tolist()))
 
     def time_hv(self, prec=FP64, n_rep=10):
         ms = np.zeros(1)

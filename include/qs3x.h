@@ -359,9 +359,10 @@ int qs3x_continued_fraction(int32_t nlev, const double *alpha, const double *bet
  * lds_bytes: its dynamic LDS. */
 int qs3x_hv_last_launch(const qs3x_sector *s, int32_t *family, int64_t *lds_bytes);
 /* The stencil H . v's plan for this sector and precision, after its first
- * launch (no counterpart in the reference): info (6 int64): rows left to the
+ * launch (no counterpart in the reference): info (8 int64): rows left to the
  * key search, non-empty (n3, n4) lines, tasks, tile height in lines, n4-slabs
- * per task, workgroups. */
+ * per task, workgroups, kernel form (0 slab window, 1 column, 2 pair), window
+ * positions per lane of the column form. */
 int qs3x_hv_stencil_info(const qs3x_sector *s, int32_t prec, int64_t *info);
 /* Times n_rep device H . v applications (events on the library stream) and
  * returns the mean milliseconds per application. */
