@@ -396,9 +396,9 @@ def main():
         log("autotuning the H.v launch (kernel family, then its tile / schedule parameters) ...")
         for name, prec in (("fp32", q3.FP32), ("fp64", q3.FP64)):
             best, bms, table = sec.hv_autotune(prec, n_rep=5)
-            tune[name] = dict(best, ms=bms, candidates=[[int(x) for x in r[:10]] + [round(float(r[10]), 4)]
+            tune[name] = dict(best, ms=bms, candidates=[[int(x) for x in r[:11]] + [round(float(r[11]), 4)]
                                                         for r in table])
-            log(f"autotune {name}: {best}: {bms:.4f} ms (default config {float(table[0][10]):.4f} ms, "
+            log(f"autotune {name}: {best}: {bms:.4f} ms (default config {float(table[0][11]):.4f} ms, "
                 f"{len(table)} candidates)")
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream(dev)

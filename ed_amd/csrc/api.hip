@@ -545,11 +545,11 @@ int qs3x_hv_autotune(qs3x_sector *s, int32_t prec, int32_t n_rep, int32_t *best,
         }
       }
       if (results && nres < max_results) {
-        double *r = results + 11 * (size_t)nres;
-        const double row[11] = {(double)fam, (double)c.kernel, (double)c.stencil, (double)c.st_lines,
+        double *r = results + 12 * (size_t)nres;
+        const double row[12] = {(double)fam, (double)c.kernel, (double)c.stencil, (double)c.st_lines,
                                 (double)c.st_slabs, (double)c.st_depth, (double)c.pairs, (double)c.chunk,
-                                (double)c.xcd, (double)c.grid_q, ms};
-        for (int k = 0; k < 11; ++k) r[k] = row[k];
+                                (double)c.xcd, (double)c.grid_q, (double)c.st_col, ms};
+        for (int k = 0; k < 12; ++k) r[k] = row[k];
       }
       ++nres;
       return fam;
@@ -579,16 +579,24 @@ int qs3x_hv_autotune(qs3x_sector *s, int32_t prec, int32_t n_rep, int32_t *best,
     // Stage 2: the winning family's own parameters.
     const qs3x_sector::HvTune w = best_c;
     if (best_family == 5) {
-      // stencil: tile height, task slabs, slab windows in flight
-      for (int B : {2, 4, 6, 8})
-        for (int sl : {4, 8, 16, 32})
-          for (int d : {2, 3}) {
-            qs3x_sector::HvTune c = w;
-            c.st_lines = B;
-            c.st_slabs = sl;
-            c.st_depth = d;
-            run(c, false);
-          }
+      // stencil: kernel form (the precision's default, pair in FP32 and
+      // column in FP64, then the slab-window form), tile height, task slabs,
+      // slab windows in flight
+      const int fdef = prec == QS3X_FP64 ? 1 : 2;
+      for (int form : {fdef, 0})
+        for (int B : {4, 6, 8, 10, 14}) {
+          if ((form == 0 && B > 8) || (form == 2 && B > 10) || (form != 0 && B < 6)) continue;
+          for (int sl : {8, 16, 32})
+            for (int d : {2, 3, 4}) {
+              if ((form == 0 && d == 4) || (form == 1 && d != 2) || (form == 2 && d == 3)) continue;
+              qs3x_sector::HvTune c = w;
+              c.st_col = form;
+              c.st_lines = B;
+              c.st_slabs = sl;
+              c.st_depth = d;
+              run(c, false);
+            }
+        }
     } else if (best_family == 0) {
       // nearest-neighbour kernel: rows per lane (FP32), interleaved chunk,
       // schedule, grid
@@ -618,9 +626,9 @@ int qs3x_hv_autotune(qs3x_sector *s, int32_t prec, int32_t n_rep, int32_t *best,
     (void)hipEventDestroy(e1);
     tn = best_c;
     if (best) {
-      const int v[10] = {best_family, best_c.kernel, best_c.stencil, best_c.st_lines, best_c.st_slabs,
-                         best_c.st_depth, best_c.pairs, best_c.chunk, best_c.xcd, best_c.grid_q};
-      for (int k = 0; k < 10; ++k) best[k] = v[k];
+      const int v[11] = {best_family, best_c.kernel, best_c.stencil, best_c.st_lines, best_c.st_slabs,
+                         best_c.st_depth, best_c.pairs, best_c.chunk, best_c.xcd, best_c.grid_q, best_c.st_col};
+      for (int k = 0; k < 11; ++k) best[k] = v[k];
     }
     if (best_ms) *best_ms = best_t;
     if (n_results) *n_results = std::min(nres, max_results);

@@ -282,13 +282,14 @@ class Sector:
         """qs3x_hv_autotune: times the kernel families, then the winner's own
         parameters, and keeps the fastest for this sector and precision.
         Returns (best dict, best ms, array of rows (family, kernel, stencil,
-        lines, slabs, depth, pairs, chunk, xcd, quarters, ms))."""
-        best = np.zeros(10, dtype=np.int32)
+        lines, slabs, depth, pairs, chunk, xcd, quarters, form, ms))."""
+        best = np.zeros(11, dtype=np.int32)
         ms = np.zeros(1)
-        res = np.zeros((128, 11))
+        res = np.zeros((128, 12))
         n = np.zeros(1, dtype=np.int32)
         _check(_lib.qs3x_hv_autotune(self._h, int(prec), int(n_rep), _p(best), _p(ms), 128, _p(n), _p(res)))
-        keys = ("family", "kernel", "stencil", "lines", "slabs", "depth", "pairs", "chunk", "xcd", "grid_quarters")
+        keys = ("family", "kernel", "stencil", "lines", "slabs", "depth", "pairs", "chunk", "xcd", "grid_quarters",
+                "form")
         return dict(zip(keys, (int(x) for x in best))), float(ms[0]), res[:int(n[0])].copy()
 
     def hv_tune_clear(self):

@@ -371,18 +371,21 @@ int qs3x_time_hv(qs3x_sector *s, int32_t prec, int32_t n_rep, double *ms);
  * Times n_rep applications on a random vector of, first, every kernel family
  * that applies in its default configuration (stencil, nearest-neighbour
  * chain, batched-load chain, per-hop chain, general lattice), then the
- * winning family's own parameters: for the stencil its tile height (2, 4, 6,
- * 8 lines), task length (4, 8, 16, 32 slabs) and slab windows in flight (2,
- * 3); for the nearest-neighbour kernel rows per lane (FP32: two where they
+ * winning family's own parameters: for the stencil its kernel form (the
+ * precision's default -- pair in FP32, column in FP64 -- and the slab-window
+ * form), tile height (4 .. 14 lines, as each form allows), task length (8,
+ * 16, 32 slabs) and slab windows in flight (2, 3 slab window; 2, 4 pair); for
+ * the nearest-neighbour kernel rows per lane (FP32: two where they
  * apply, or one), interleaved XCD chunk (2^0, 2^3, 2^5, 2^7 tiles), tile
  * schedule (per-XCD or round robin) and grid (4 or 2 quarters of the
  * resident grid); for the others the schedule and grid (4..1 quarters).  It
  * keeps the fastest (a later candidate must beat the best so far by 2 %) for
  * this sector and precision; every later H . v of the sector (solvers
- * included) uses it.  best (10 ints, may be NULL): family (as
+ * included) uses it.  best (11 ints, may be NULL): family (as
  * qs3x_hv_last_launch), kernel, stencil, lines, slabs, depth, pairs, chunk,
- * xcd, grid quarters; best_ms its time; results (11 doubles per candidate, up
- * to max_results): the same 10 fields and ms, where ms < 0 is the single
+ * xcd, grid quarters, stencil form (-1: the option's); best_ms its time;
+ * results (12 doubles per candidate, up to max_results): the same 11 fields
+ * and ms, where ms < 0 is the single
  * timed launch of a candidate over 3x slower than the best so far.  Every
  * candidate computes the same H . v (each kernel and parameter is checked
  * against the oracle by the test suite). */

@@ -152,7 +152,7 @@ def test_hv_autotuned_parity(case):
     ref = om.hv(osec, v)
     best64, ms64, res64 = gsec.hv_autotune(q3.FP64, n_rep=2)
     assert len(res64) >= 2 and ms64 > 0 and best64["xcd"] in (1, 8) and best64["grid_quarters"] in (1, 2, 3, 4)
-    assert ms64 <= min(r[10] for r in res64 if r[10] > 0) / 0.98
+    assert ms64 <= min(r[11] for r in res64 if r[11] > 0) / 0.98
     # stage 1 timed each family once, stage 2 the winner's parameters
     fams = [int(r[0]) for r in res64]
     assert best64["family"] in fams and len(set(fams)) == len([f for f in dict.fromkeys(fams)])
@@ -174,8 +174,8 @@ def test_hv_autotuned_parity(case):
 @pytest.mark.parametrize("prec_name", ["fp32", "fp64"])
 def test_hv_autotune_stencil_sector(prec_name):
     """On a stencil sector (nearest-neighbour chain, N_down = 4) the tuner's
-    second stage searches the stencil's tile height, task slabs and windows
-    in flight (or the nearest-neighbour kernel's rows per lane, chunk,
+    second stage searches the stencil's kernel form, tile height, task slabs
+    and windows in flight (or the nearest-neighbour kernel's rows per lane, chunk,
     schedule and grid); the chosen configuration is the one later H.v use,
     and it still equals the oracle's H.v (FP64 1e-12, FP32 2e-6)."""
     print(f"[gpu] hv autotune stencil sector {prec_name}", flush=True)
@@ -194,7 +194,10 @@ def test_hv_autotune_stencil_sector(prec_name):
     stage2 = res[n1:]
     assert len(stage2) >= 2 and all(int(r[0]) == best["family"] for r in stage2)
     if best["family"] == 5:
-        assert {int(r[3]) for r in stage2} == {2, 4, 6, 8} and {int(r[5]) for r in stage2} == {2, 3}
+        # kernel forms (default and slab window), heights, windows in flight
+        fdef = 1 if prec == q3.FP64 else 2
+        assert {int(r[10]) for r in stage2} == {fdef, 0}
+        assert {6, 8, 10} <= {int(r[3]) for r in stage2} and {2, 3} <= {int(r[5]) for r in stage2}
     rng = np.random.default_rng(K)
     v = rng.uniform(-1, 1, s.dim) + 1j * rng.uniform(-1, 1, s.dim)
     osec = om.build_sector(4, rk)
@@ -216,6 +219,7 @@ def test_hv_autotune_stencil_sector(prec_name):
         # the plan of that launch is the tuned tile
         info = s.hv_stencil_info(prec)
         assert info["tile_lines"] == best["lines"] and info["slabs"] == best["slabs"]
+        assert info["form"] == best["form"]
     s.hv_tune_clear()
 
 
