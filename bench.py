@@ -346,9 +346,10 @@ def main():
     ap.add_argument("--reorth", type=int, default=2,
                     help="thick-restart reorthogonalisation: 0 full, 1 locked+local, 2 locked at restart only")
     ap.add_argument("--cgs", type=int, default=1, help="Gram-Schmidt passes per thick-restart step")
-    ap.add_argument("--fp64-lanczos", type=int, default=0,
+    ap.add_argument("--fp64-lanczos", type=int, default=600,
                     help="FP32-mixed thick restart: at most this many FP64 Lanczos steps from the Rayleigh-Ritz "
-                         "vector until the residual bound certifies E0 (0: the block-Krylov refinement instead)")
+                         "vector until the residual bound certifies E0 (0: the block-Krylov refinement instead; "
+                         "at L=1000 the bound certifies after about 305 steps)")
     ap.add_argument("--fp32-floor", type=float, default=1e-7)
     ap.add_argument("--check-every", type=int, default=5,
                     help="thick restart: convergence check every n steps inside cycles (0: at restarts)")

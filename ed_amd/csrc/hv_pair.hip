@@ -107,11 +107,10 @@ __global__ __launch_bounds__(kThreads) void k_hv_stencil_pair(HvArgs a, StencilA
   const int W1 = L + 1;
   f4v fp[D][kF], fw[D][kF];
   f4v xr[kRing][kF], wv[kRing][kF];
-  auto entry = [&](int s_rel, int j) __attribute__((always_inline)) -> int4 {
-    const int4 e = s_tl[s_rel * NL + j];
-    return make_int4(__builtin_amdgcn_readfirstlane(e.x), __builtin_amdgcn_readfirstlane(e.y),
-                     __builtin_amdgcn_readfirstlane(e.z), 0);
-  };
+  // line entry of window line j of slab s (s_rel = s - c0 + 1); the
+  // descriptors span the whole vector, so the entry only enters lane offsets
+  // and stays in vector registers (scalar ones would spill)
+  auto entry = [&](int s_rel, int j) __attribute__((always_inline)) -> int4 { return s_tl[s_rel * NL + j]; };
   // slab s (s_rel = s - c0 + 1) into a set, with per line the flags that
   // the store applies once the loads have arrived: bit 0 / 1 the pair's first
   // / second element lies in its line; bit 2 that one element is row 0 (bit
