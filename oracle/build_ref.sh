@@ -88,3 +88,18 @@ if [ -f "$SRCM" ]; then
   $FC -O2 -module-dir "$OUT/mkinput" "$SRCM" -o "$OUT/mk_input_list.exe"
   echo "built $OUT/mk_input_list.exe"
 fi
+
+# The reference's own driver (source/main.f90), unmodified, compiled against
+# this build's drop-in modules (ed_amd/fortran: input_param, state_lists,
+# ham2vec, lanczos, eigen_solver, get_expectation_values over libqs3x.so)
+# instead of its CPU modules: the drop-in check that
+# tests/test_fortran_driver.py runs on the GPU against the engine's own driver.
+SRCMAIN="$SRC/main.f90"
+SHIMS="$HERE/../ed_amd/fortran/_build"
+if [ -f "$SRCMAIN" ] && [ -f "$SHIMS/qs3x_modules.o" ]; then
+  mkdir -p "$OUT/main"
+  $FC -O2 -fdefault-integer-8 -module-dir "$OUT/main" -I "$SHIMS" -c "$SRCMAIN" -o "$OUT/main/main.o"
+  $FC -o "$OUT/qs3_main_on_qs3x.exe" "$OUT/main/main.o" "$SHIMS/qs3x_modules.o" "$SHIMS/qs3x_iso_c.o" \
+    -L"$HERE/../ed_amd" -lqs3x -Wl,-rpath,'$ORIGIN/../../ed_amd'
+  echo "built $OUT/qs3_main_on_qs3x.exe"
+fi

@@ -3,7 +3,9 @@
 CPU: the reference's own driver (source/main.f90) compiles and links against
 the shim modules unchanged (skipped where the reference sources are absent).
 GPU: the engine's driver runs input decks in the reference's formats and its
-output files agree with the oracle."""
+output files agree with the oracle; the reference's own driver, built against
+the shim modules by oracle/build_ref.sh, runs the same decks and writes the
+same files and values."""
 import os
 import shutil
 import subprocess
@@ -71,8 +73,11 @@ NOE = {noe}, NOK = {nok}, NOM = {nom}, maxitr = 500, lnc_ene_conv = 1.0d-13, i_v
         f.write(deck)
 
 
-def _run(d, expect_ok=True):
-    exe = os.path.join(FDIR, "qs3x_main.exe")
+REF_DRIVER = os.path.join(ROOT, "oracle", "_ref", "qs3_main_on_qs3x.exe")
+
+
+def _run(d, expect_ok=True, exe=None):
+    exe = exe or os.path.join(FDIR, "qs3x_main.exe")
     assert os.path.exists(exe), "run ed_amd/fortran/build_fortran.sh"
     with open(os.path.join(d, "input.dat")) as fin:
         r = subprocess.run([exe], stdin=fin, cwd=d, capture_output=True, text=True, timeout=100)
@@ -206,3 +211,60 @@ def test_driver_refuses_incommensurate_q(tmp_path):
     r = _run(d, expect_ok=False)
     assert r.returncode != 0
     assert "not a multiple of 2 pi" in (r.stdout + r.stderr)
+
+
+DECKS = {
+    # name: (NOS, LX, LY, N_down, K, bonds, ALG, extra deck settings)
+    "chain_lanczos_dsf": (12, 12, 1, 4, (0, 0), orc.chain_bonds(12, [(1, 1.0, 1.0)]), 1,
+                          dict(cal_dsf=1, spsmsz=2, itr_dsf=25, q=(2 * np.pi * 3 / 12, 0.0, 0.0), rfield=0.1, nov=1)),
+    "torus_thick_restart": (16, 4, 4, 3, (1, 0), orc.square_bonds(4, 4, 1.0), 2, dict(nov=1)),
+    "chain_j1j2_dense": (10, 10, 1, 3, (2, 0), orc.chain_bonds(10, [(1, 1.0, 0.8), (2, 0.4, 0.4)]), 3, dict(nov=2)),
+    "ferromagnet": (10, 10, 1, 0, (0, 0), orc.chain_bonds(10, [(1, 1.0, 0.8)]), 1, dict()),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(REF_DRIVER), reason="reference driver not built (oracle/build_ref.sh)")
+@pytest.mark.parametrize("deck", list(DECKS))
+def test_reference_driver_runs_on_the_engine(tmp_path, deck):
+    """The reference's own main.f90, compiled unchanged against the drop-in
+    modules, runs a deck (ALG 1 with the DSF recurrence, 2, 3, and the
+    N_down = 0 case) and writes the files the engine's driver writes, with the
+    same values: energies to 1e-10, everything else to 1e-8 of each file's
+    largest entry."""
+    print(f"[gpu] reference main.f90 on the engine: {deck}", flush=True)
+    NOS, LX, LY, nod, K, bonds, alg, extra = DECKS[deck]
+    outs = {}
+    for tag, exe in (("engine", None), ("reference", REF_DRIVER)):
+        d = str(tmp_path / tag)
+        os.makedirs(d)
+        _write_deck(d, NOS, LX, LY, nod, K, bonds, alg=alg, **extra)
+        _run(d, exe=exe)
+        outs[tag] = d
+    fe = sorted(os.listdir(os.path.join(outs["engine"], "out")))
+    fr = sorted(os.listdir(os.path.join(outs["reference"], "out")))
+    assert fe == fr and len(fe) >= 1, (fe, fr)
+    def tokens(path):
+        with open(path) as f:
+            return [line.split() for line in f if line.strip()]
+
+    def num(t):
+        try:
+            return float(t.replace("D", "E"))
+        except ValueError:
+            return None
+
+    for name in fe:
+        ta = tokens(os.path.join(outs["engine"], "out", name))
+        tb = tokens(os.path.join(outs["reference"], "out", name))
+        assert [len(r) for r in ta] == [len(r) for r in tb], name
+        xa = [num(t) for r in ta for t in r]
+        xb = [num(t) for r in tb for t in r]
+        big = max([abs(x) for x in xa if x is not None] + [1.0])
+        tol = 1e-10 if name == "energy.dat" else 1e-8 * big
+        for ua, ub, sa, sb in zip(xa, xb, [t for r in ta for t in r], [t for r in tb for t in r]):
+            if ua is None or ub is None:
+                assert sa == sb, (name, sa, sb)  # text (headers, separators)
+            else:
+                assert abs(ua - ub) <= tol, (name, ua, ub)
+

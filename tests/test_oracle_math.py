@@ -271,3 +271,47 @@ def test_u1_hv_equals_dense_block(L, N, fam):
     assert sorted(perm.tolist()) == list(range(D))  # the ranks enumerate every configuration once
     Hu = np.stack([m.hv_u1(N, np.eye(D)[:, j].astype(complex)) for j in range(D)], axis=1)
     assert np.allclose(Hu, Hd[np.ix_(perm, perm)], atol=1e-14)
+
+
+def test_incremental_gap_keys_of_chain_hops():
+    """The group H.v updates the packed backward-gap keys of a row for a hop
+    that moves down spin m by d = +-1 without wrapping (the target empty, so
+    the spins keep their order): g[m] -= d and g[m - 1] += d, so key j moves by
+    d (unit(j, m - 1) - unit(j, m)), a gap's unit being 2^(kb (N - 1 - t)) for
+    its field t = (j - gap) mod N in key j and 0 for the gap the key leaves
+    out (t = 0).  Checked against the direct build of the hopped row's keys."""
+    import random
+
+    def keys(n, L, N, kb):
+        g = [n[q + 1] - n[q] for q in range(N - 1)] + [n[0] + L - n[N - 1]]
+        out = []
+        for j in range(N):
+            k = 0
+            for tt in range(1, N):
+                k = (k << kb) | g[(j - tt + N) % N]
+            out.append(k)
+        return out
+
+    def unit(j, gap, N, kb):
+        t = (j - gap + N) % N
+        return (1 << (kb * (N - 1 - t))) if t else 0
+
+    rnd = random.Random(7)
+    n_checked = 0
+    for L, N in ((40, 7), (24, 5), (1000, 4), (33, 8), (12, 2), (9, 3)):
+        kb = L.bit_length()
+        for _ in range(300):
+            n = [1] + sorted(rnd.sample(range(2, L + 1), N - 1))
+            kb0 = keys(n, L, N, kb)
+            for m in range(N):
+                for d in (-1, 1):
+                    t = n[m] + d
+                    if t < 1 or t > L or t in n:
+                        continue
+                    nn = n[:]
+                    nn[m] = t
+                    k1 = keys(nn, L, N, kb)
+                    for j in range(N):
+                        assert kb0[j] + d * (unit(j, (m - 1 + N) % N, N, kb) - unit(j, m, N, kb)) == k1[j]
+                        n_checked += 1
+    assert n_checked > 50000
