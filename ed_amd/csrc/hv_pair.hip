@@ -24,9 +24,9 @@
 // loaded from the neighbouring row and set to 0, and one outside the vector
 // reads 0 in the hardware's range check.  A pair is stored with one 16-byte
 // store when both of its rows belong to this tile and line, otherwise its one
-// such row with an 8-byte store; both stores are issued by every lane (with an
-// offset past the vector where there is nothing to write), so the number of
-// memory operations between a window's loads and the wait for them is fixed.
+// such row with an 8-byte store.  The 16-byte stores are issued by every lane
+// (with an offset past the vector where there is nothing to write); the 8-byte
+// ones only by wavefronts that hold a line's first or last row.
 // Rows that are not clear are written too and then overwritten by the
 // slow-row kernel, which runs after this one on the same stream.
 #include "hv_col.h"
@@ -255,19 +255,21 @@ __global__ __launch_bounds__(kThreads) void k_hv_stencil_pair(HvArgs a, StencilA
           }
         }
       }
-      // 16-byte store where both rows are written, else one 8-byte store of
-      // the single row, issued by every lane
+      // 16-byte store where both rows are written (issued by every lane),
+      // else one 8-byte store of the single row
       const int rr = e.x - e.y + av0;
       const bool both = wr[0] && wr[1], one = wr[0] != wr[1];
       const unsigned off4 = row_off(rr, both);
       const unsigned off2 = row_off(rr + (wr[1] ? 1 : 0), one);
       const C o2 = wr[1] ? make_float2(o.z, o.w) : make_float2(o.x, o.y);
       st4(ro, off4, o);
-      buf_st(ro, off2, o2);
-      if constexpr (FUSED) {
-        const C x2 = wr[1] ? make_float2(xo4.z, xo4.w) : make_float2(xo4.x, xo4.y);
-        st4(rx, off4, xo4);
-        buf_st(rx, off2, x2);
+      if constexpr (FUSED) st4(rx, off4, xo4);
+      if (one) {  // a line's first or last row in the tile: few lanes, most wavefronts none
+        buf_st(ro, off2, o2);
+        if constexpr (FUSED) {
+          const C x2 = wr[1] ? make_float2(xo4.z, xo4.w) : make_float2(xo4.x, xo4.y);
+          buf_st(rx, off2, x2);
+        }
       }
     }
   };
