@@ -141,6 +141,7 @@ def dsf_workload(args):
     block_tune = {}
     if args.tune:
         q3.set_option("block_autotune", 1)
+    q3.set_option("block_fused", args.block_fused)
     for prec, name in ((q3.FP32, "fp32"), (q3.FP64, "fp64")):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -193,7 +194,8 @@ def dsf_workload(args):
            "value": t32, "unit": "s", "higher_is_better": False, "dtype": "complex-fp32 (fp64 coefficients)",
            "data": "synthetic", "config": {"workload": "periodic NN Heisenberg chain L=1000, ground state N_down=3 "
                                                        "k=0, S^-_q into N_down=4 k=q (D~41.4M each), 64 q",
-                                           "steps": steps, "eta": 0.02, "omega_points": 2001},
+                                           "steps": steps, "eta": 0.02, "omega_points": 2001,
+                                           "block_step": "fused" if args.block_fused else "separate"},
            "seconds_fp32_batch": t32, "seconds_fp64_batch": t64, "ms_per_block_step_fp32": st32["ms_per_block_step"],
            "ms_per_block_step_fp64": st64["ms_per_block_step"], "seconds_per_sector_fp32_extrapolated": per_sector,
            "per_sector_sample_seconds": tps, "speedup_batch_vs_per_sector": per_sector / t32,
@@ -358,6 +360,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true", help="only run a 1 GiB device copy (counter scaling)")
     ap.add_argument("--dsf", action="store_true", help="run the batched 64-q DSF workload instead of the headline")
     ap.add_argument("--dsf-steps", type=int, default=200)
+    ap.add_argument("--block-fused", type=int, default=1, choices=(0, 1),
+                    help="batched DSF: 1 the fused block step (default), 0 block stencil H.v then block update")
     ap.add_argument("--sectors", action="store_true", help="run the 280-sector L=40 workload instead of the headline")
     ap.add_argument("--sectors-conv", type=float, default=1e-10)
     args = ap.parse_args()

@@ -87,7 +87,29 @@ struct BlockHvArgs {
   int nxcd;               // L2 domains for the chunk schedule (tile_range)
   const int32_t *rows;    // non-null: apply only these rows (ascending), nrows of them
   int64_t nrows;
+  // Fused Lanczos step (the FUSED kernels): x holds P_{i-1} (the first step:
+  // the start vectors), vprev v_{i-1} (null on the first step); a row writes
+  // v_i = c1 P_{i-1} - c2 v_{i-1} to xout and P_i = H v_i - c3 v_{i-1} to y,
+  // with per q c1 = sc[fis], c2 = sc[fia] sc[fis], c3 = sc[fib]; the partial
+  // sums are [grid][kQ][4]: Re<v,Hv>, |P|^2, Re<v,P>, |v|^2.
+  const void *vprev;
+  void *xout;
+  int fia, fis, fib;
 };
+
+// per-q coefficients of the fused step (the single-vector fused_coef per q)
+template <class R>
+__device__ __forceinline__ void fused_coef_q(const BlockHvArgs &a, int q, R &c1, R &c2, R &c3) {
+  const bool hasv = a.vprev != nullptr && a.fib >= 0;
+  c1 = (R)a.sc[(size_t)a.fis * kQ + q];
+  c2 = hasv ? (R)(a.sc[(size_t)a.fia * kQ + q] * a.sc[(size_t)a.fis * kQ + q]) : (R)0;
+  c3 = hasv ? (R)a.sc[(size_t)a.fib * kQ + q] : (R)0;
+}
+// v = c1 p - c2 w (w absent: c1 p), the rounding order of fused_v
+template <class R>
+__device__ __forceinline__ R fused_e(R c1, R c2, R p, R w, bool hasv) {
+  return hasv ? fz_fma(c1, p, -(c2 * w)) : c1 * p;
+}
 
 // C(x, q) for 1 <= q <= N - 1 from the LDS table.
 __device__ __forceinline__ int bnm(const int32_t *s_binom, int L, int q, int x) {
@@ -216,8 +238,10 @@ __device__ __forceinline__ typename CT<R>::T cload(const void *p, int64_t i) {
   return ((const typename CT<R>::T *)p)[i];
 }
 
-// y = H_q (s_q x) for the kQ interleaved vectors (see the file comment).
-template <int N, class R, class KeyT>
+// y = H_q (s_q x) for the kQ interleaved vectors (see the file comment);
+// FUSED: the fused Lanczos step of BlockHvArgs (every element gathered is
+// formed as v_i from P_{i-1} and v_{i-1}).
+template <int N, class R, class KeyT, bool FUSED>
 __global__ __launch_bounds__(kBThreads) void k_hv_block(BlockHvArgs a) {
   using C = typename CT<R>::T;
   constexpr int H = 2 * N;
@@ -251,7 +275,21 @@ __global__ __launch_bounds__(kBThreads) void k_hv_block(BlockHvArgs a) {
   }
   __syncthreads();
 
-  const R scale = a.iscale >= 0 ? (R)a.sc[(size_t)a.iscale * kQ + lane] : (R)1;
+  const R scale = (!FUSED && a.iscale >= 0) ? (R)a.sc[(size_t)a.iscale * kQ + lane] : (R)1;
+  R fc1 = (R)1, fc2 = (R)0, fc3 = (R)0;
+  const bool hasv = FUSED && a.vprev != nullptr && a.fib >= 0;
+  if constexpr (FUSED) fused_coef_q<R>(a, lane, fc1, fc2, fc3);
+  // element i of this lane's vector: x, or (FUSED) v_i formed from P_{i-1} and v_{i-1}
+  auto xload = [&](int64_t i) __attribute__((always_inline)) -> C {
+    C p = cload<R>(a.x, i);
+    if constexpr (FUSED) {
+      const C w = hasv ? cload<R>(a.vprev, i) : C{(R)0, (R)0};
+      p.x = fused_e<R>(fc1, fc2, p.x, w.x, hasv);
+      p.y = fused_e<R>(fc1, fc2, p.y, w.y, hasv);
+    }
+    return p;
+  };
+  double acc4[4] = {0.0, 0.0, 0.0, 0.0};  // FUSED: Re<v,Hv>, |P|^2, Re<v,P>, |v|^2
   // phases of the closed-form anchor hops (ell = 1 and L - 1) of this lane's q
   const C e1 = cload<R>(a.eph, (size_t)1 * kQ + lane);
   const C eL1 = cload<R>(a.eph, (size_t)(L - 1) * kQ + lane);
@@ -317,14 +355,14 @@ __global__ __launch_bounds__(kBThreads) void k_hv_block(BlockHvArgs a) {
       for (int k = 0; k < RR; ++k) {
         const int r = min(r0 + k, nr - 1);
         const int64_t rw = w_row[r];
-        xin[k] = cload<R>(a.x, rw * kQ + lane);
+        xin[k] = xload(rw * kQ + lane);
 #pragma unroll
         for (int h = 0; h < H; ++h) {
           id[k][h] = w_id[r * H + h];
           aux[k][h] = w_aux[r * H + h];
         }
 #pragma unroll
-        for (int h = 0; h < H; ++h) y[k][h] = cload<R>(a.x, (int64_t)(id[k][h] >= 0 ? id[k][h] : rw) * kQ + lane);
+        for (int h = 0; h < H; ++h) y[k][h] = xload((int64_t)(id[k][h] >= 0 ? id[k][h] : rw) * kQ + lane);
       }
 #pragma unroll
       for (int k = 0; k < RR; ++k) {
@@ -360,26 +398,45 @@ __global__ __launch_bounds__(kBThreads) void k_hv_block(BlockHvArgs a) {
         C out;
         out.x = mk * (dg * x.x + accx * scale);
         out.y = mk * (dg * x.y + accy * scale);
-        st_nt(&((C *)a.y)[(int64_t)w_row[r] * kQ + lane], out);  // written once per step
-        dre += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
-        dim_ += (double)x.x * (double)out.y - (double)x.y * (double)out.x;
+        const int64_t io = (int64_t)w_row[r] * kQ + lane;
+        if constexpr (FUSED) {
+          const C wo = hasv ? cload<R>(a.vprev, io) : C{(R)0, (R)0};
+          C pn;
+          pn.x = out.x - fc3 * wo.x;
+          pn.y = out.y - fc3 * wo.y;
+          st_nt(&((C *)a.y)[io], pn);
+          st_nt(&((C *)a.xout)[io], x);
+          acc4[0] += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
+          acc4[1] += (double)pn.x * (double)pn.x + (double)pn.y * (double)pn.y;
+          acc4[2] += (double)x.x * (double)pn.x + (double)x.y * (double)pn.y;
+          acc4[3] += (double)x.x * (double)x.x + (double)x.y * (double)x.y;
+        } else {
+          st_nt(&((C *)a.y)[io], out);  // written once per step
+          dre += (double)x.x * (double)out.x + (double)x.y * (double)out.y;
+          dim_ += (double)x.x * (double)out.y - (double)x.y * (double)out.x;
+        }
       }
     }
     __builtin_amdgcn_wave_barrier();
   }
   // per-q block partials: sum the 4 wavefronts in fixed order
-  __shared__ double red[2][kBWaves][kQ];
-  red[0][wid][lane] = dre;
-  red[1][wid][lane] = dim_;
+  constexpr int NS = FUSED ? 4 : 2;
+  __shared__ double red[NS][kBWaves][kQ];
+  if constexpr (FUSED) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) red[k][wid][lane] = acc4[k];
+  } else {
+    red[0][wid][lane] = dre;
+    red[1][wid][lane] = dim_;
+  }
   __syncthreads();
   if (a.dot_partials && wid == 0) {
-    double sr = 0, si = 0;
-    for (int w = 0; w < kBWaves; ++w) {
-      sr += red[0][w][lane];
-      si += red[1][w][lane];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      double t = 0;
+      for (int w = 0; w < kBWaves; ++w) t += red[k][w][lane];
+      a.dot_partials[NS * ((size_t)blockIdx.x * kQ + lane) + k] = t;
     }
-    a.dot_partials[2 * ((size_t)blockIdx.x * kQ + lane)] = sr;
-    a.dot_partials[2 * ((size_t)blockIdx.x * kQ + lane) + 1] = si;
   }
 }
 
@@ -395,10 +452,11 @@ size_t hv_block_lds(int L, int nbt, int prec) {
 
 using BFn = void (*)(BlockHvArgs);
 
-template <int N>
+template <int N, bool FUSED = false>
 BFn pick_block(int prec, bool key32) {
-  if (prec == QS3X_FP64) return key32 ? k_hv_block<N, double, uint32_t> : k_hv_block<N, double, uint64_t>;
-  return key32 ? k_hv_block<N, float, uint32_t> : k_hv_block<N, float, uint64_t>;
+  if (prec == QS3X_FP64)
+    return key32 ? k_hv_block<N, double, uint32_t, FUSED> : k_hv_block<N, double, uint64_t, FUSED>;
+  return key32 ? k_hv_block<N, float, uint32_t, FUSED> : k_hv_block<N, float, uint64_t, FUSED>;
 }
 
 BFn block_kernel(int N, int prec, bool key32, size_t *lds, int L, int nbt) {
@@ -411,6 +469,14 @@ BFn block_kernel(int N, int prec, bool key32, size_t *lds, int L, int nbt) {
 #undef BK
     default: return nullptr;
   }
+}
+
+// The fused per-row block kernel: N_down = 4 only (the rows the block
+// stencil leaves, the one place the fused step runs).
+BFn block_kernel_fused(int N, int prec, bool key32, size_t *lds, int L, int nbt) {
+  if (N != 4) return nullptr;
+  *lds = hv_block_lds<4>(L, nbt, prec);
+  return pick_block<4, true>(prec, key32);
 }
 
 // ---------------------------------------------- block gap-lattice stencil
@@ -458,7 +524,13 @@ constexpr size_t block_stencil_lds(int slabs) {
   return (size_t)kSRing * (B + 2) * kSW * 128 + (size_t)(slabs + 2) * (B + 2) * sizeof(int4);
 }
 
-template <class R, int B, int NT>
+// FUSED: the fused Lanczos step of BlockHvArgs.  The window holds v_i,
+// formed from the loaded P_{i-1} and v_{i-1} as each window point goes into
+// the ring; a thread's fetch slots are then its own output points first (in
+// the order compute takes them) and the halo after them, so the v_{i-1} of
+// its own rows stays in registers (two sets, slabs of even and odd c - c0)
+// from the window's store until the slab is computed.
+template <class R, int B, int NT, bool FUSED>
 __global__ __launch_bounds__(NT) void k_hv_block_stencil(BlockHvArgs a, BlockStencilArgs sa) {
   using V = typename Chunk16<R>::T;
   constexpr int NQ = Chunk16<R>::nq;
@@ -470,57 +542,100 @@ __global__ __launch_bounds__(NT) void k_hv_block_stencil(BlockHvArgs a, BlockSte
   constexpr int PR = NT / 8;              // window points per round (8 chunks each)
   constexpr int PER = (NWIN + PR - 1) / PR;  // window points per thread
   constexpr int NOUT8 = B * kSA * 8;     // output chunks per slab
+  constexpr int MI = NOUT8 / NT;         // output points per thread and slab
+  constexpr int NS = FUSED ? 4 : 2;      // partial sums per q
   static_assert(NOUT8 % NT == 0, "whole rounds of output chunks");
-  static_assert(kSRing * SLOT * 16 >= NT * NQ * 2 * 8, "the partial-sum area must fit in the ring");
+  static_assert(kSRing * SLOT * 16 >= NT * NQ * NS * 8, "the partial-sum area must fit in the ring");
   extern __shared__ __align__(16) unsigned char smem[];
   V *s_ring = (V *)smem;
   int4 *s_tl = (int4 *)(smem + (size_t)kSRing * SLOT * sizeof(V));
-  __shared__ R s_qt[6][kQ];         // per q: scale, row mask, explist_q(1) re, im, explist_q(L - 1) re, im
-  __shared__ double s_acc[2][kQ];   // per q: the block's sum conj(x) y
+  // per q: scale, row mask, explist_q(1) re, im, explist_q(L - 1) re, im; FUSED: c1, c2, c3
+  __shared__ R s_qt[FUSED ? 9 : 6][kQ];
+  __shared__ double s_acc[NS][kQ];  // per q: the block's partial sums
   const int L = a.L;
   const int tid = threadIdx.x, sub = tid & 7, ps = tid >> 3;
+  const bool hasv = FUSED && a.vprev != nullptr && a.fib >= 0;
   for (int q = tid; q < kQ; q += NT) {
-    s_qt[0][q] = a.iscale >= 0 ? (R)a.sc[(size_t)a.iscale * kQ + q] : (R)1;
+    s_qt[0][q] = (!FUSED && a.iscale >= 0) ? (R)a.sc[(size_t)a.iscale * kQ + q] : (R)1;
     s_qt[1][q] = (R)a.rowmask[1 * kQ + q];
     const R *e = (const R *)a.eph;
     s_qt[2][q] = e[2 * ((size_t)1 * kQ + q)];
     s_qt[3][q] = e[2 * ((size_t)1 * kQ + q) + 1];
     s_qt[4][q] = e[2 * ((size_t)(L - 1) * kQ + q)];
     s_qt[5][q] = e[2 * ((size_t)(L - 1) * kQ + q) + 1];
-    s_acc[0][q] = 0.0;
-    s_acc[1][q] = 0.0;
+    if constexpr (FUSED) fused_coef_q<R>(a, q, s_qt[6][q], s_qt[7][q], s_qt[8][q]);
+#pragma unroll
+    for (int k = 0; k < NS; ++k) s_acc[k][q] = 0.0;
   }
   const R jxy2 = (R)a.bt[0], jz4 = (R)a.bt[1], shift = (R)a.shift;
   const V *__restrict__ x = (const V *)a.x;
+  const V *__restrict__ vp = (const V *)a.vprev;
   V *__restrict__ y = (V *)a.y;
+  V *__restrict__ xo = (V *)a.xout;
   const V zero = (V)(R)0;
-  V preX[PER], preY[PER];
+  // fetch slot m of this thread: its window point (NWIN: none)
+  auto wpoint = [&](int m) __attribute__((always_inline)) -> int {
+    if constexpr (!FUSED) {
+      return ps + PR * m;
+    } else {
+      if (m < MI) {
+        const int pos = ps + PR * m;
+        return (1 + pos / kSA) * kSW + pos % kSA + 1;
+      }
+      const int h = ps + PR * (m - MI);  // halo: lines 0 and NL - 1, then the end columns of lines 1 .. B
+      if (h < kSW) return h;
+      if (h < 2 * kSW) return (NL - 1) * kSW + (h - kSW);
+      const int h2 = h - 2 * kSW;
+      return h2 < 2 * B ? (1 + h2 / 2) * kSW + ((h2 & 1) ? kSW - 1 : 0) : NWIN;
+    }
+  };
+  V preX[PER], preY[PER], wX[PER], wY[PER];
+  V wown[2][FUSED ? MI : 1];  // FUSED: v_{i-1} of the thread's own points, slabs of even / odd c - c0
   // window points of slab te for vector group g
-  auto fetch = [&](V (&pre)[PER], int a0, int g, const int4 *te) __attribute__((always_inline)) {
+  auto fetch = [&](V (&pre)[PER], V (&w)[PER], int a0, int g, const int4 *te) __attribute__((always_inline)) {
 #pragma unroll
     for (int m = 0; m < PER; ++m) {
-      const int p = ps + PR * m;
-      V v = zero;
+      const int p = wpoint(m);
+      V v = zero, u = zero;
       if (p < NWIN) {
         const int j = p / kSW, i = p - j * kSW;
         const int4 e = te[j];
         const int av = a0 - 1 + i;
-        if (av >= e.y && av < e.z) v = x[(int64_t)(e.x + (av - e.y)) * RV + g * 8 + sub];
+        if (av >= e.y && av < e.z) {
+          const int64_t idx = (int64_t)(e.x + (av - e.y)) * RV + g * 8 + sub;
+          v = x[idx];
+          if (FUSED && hasv) u = vp[idx];
+        }
       }
       pre[m] = v;
-    }
-  };
-  auto store = [&](const V (&pre)[PER], int sl) __attribute__((always_inline)) {
-#pragma unroll
-    for (int m = 0; m < PER; ++m) {
-      const int p = ps + PR * m;
-      if (p < NWIN) s_ring[(size_t)sl * SLOT + p * 8 + sub] = pre[m];
+      if constexpr (FUSED) w[m] = u;
     }
   };
   // this thread's wave vectors of group g
-  R qs[NQ], qm[NQ], e1r[NQ], e1i[NQ], eLr[NQ], eLi[NQ];
-  double dre[NQ], dim_[NQ];
-  auto compute = [&](int a0, int b0, int g, int c, const int4 *te0) __attribute__((always_inline)) {
+  R qs[NQ], qm[NQ], e1r[NQ], e1i[NQ], eLr[NQ], eLi[NQ], fc1[NQ], fc2[NQ], fc3[NQ];
+  double acc[NQ][NS];
+  // par: the own-point set the slab's v_{i-1} goes to (-1: none)
+  auto store = [&](const V (&pre)[PER], const V (&w)[PER], int sl, int par) __attribute__((always_inline)) {
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+      const int p = wpoint(m);
+      V v = pre[m];
+      if constexpr (FUSED) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+          v[2 * k] = fused_e<R>(fc1[k], fc2[k], v[2 * k], w[m][2 * k], hasv);
+          v[2 * k + 1] = fused_e<R>(fc1[k], fc2[k], v[2 * k + 1], w[m][2 * k + 1], hasv);
+        }
+        if (m < MI) {
+          if (par == 0) wown[0][m] = w[m];
+          if (par == 1) wown[1][m] = w[m];
+        }
+      }
+      if (p < NWIN) s_ring[(size_t)sl * SLOT + p * 8 + sub] = v;
+    }
+  };
+  auto compute = [&](int a0, int b0, int g, int c, const int4 *te0, const V (&wo)[FUSED ? MI : 1])
+                     __attribute__((always_inline)) {
     const V *r0 = s_ring + (size_t)(c % kSRing) * SLOT;
     const V *rm = s_ring + (size_t)((c + kSRing - 1) % kSRing) * SLOT;
     const V *rp = s_ring + (size_t)((c + 1) % kSRing) * SLOT;
@@ -556,22 +671,41 @@ __global__ __launch_bounds__(NT) void k_hv_block_stencil(BlockHvArgs a, BlockSte
         }
         const V xv = r0[x0];
         const R dg = shift - (R)2 * jz4 * (R)nanti;
-        V out;
+        V out, pn;
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
           const R sx = sm[2 * k] + e1r[k] * al[2 * k] - e1i[k] * al[2 * k + 1] + eLr[k] * ar[2 * k] -
                        eLi[k] * ar[2 * k + 1];
           const R sy = sm[2 * k + 1] + e1r[k] * al[2 * k + 1] + e1i[k] * al[2 * k] + eLr[k] * ar[2 * k + 1] +
                        eLi[k] * ar[2 * k];
-          const R xr = xv[2 * k] * qs[k], xi = xv[2 * k + 1] * qs[k];
-          const R orr = qm[k] * (dg * xr + jxy2 * qs[k] * sx);
-          const R oi = qm[k] * (dg * xi + jxy2 * qs[k] * sy);
+          // FUSED: the window already holds v_i (scale 1)
+          const R sq = FUSED ? (R)1 : qs[k];
+          const R xr = xv[2 * k] * sq, xi = xv[2 * k + 1] * sq;
+          const R orr = qm[k] * (dg * xr + jxy2 * sq * sx);
+          const R oi = qm[k] * (dg * xi + jxy2 * sq * sy);
           out[2 * k] = orr;
           out[2 * k + 1] = oi;
-          dre[k] += (double)xr * (double)orr + (double)xi * (double)oi;
-          dim_[k] += (double)xr * (double)oi - (double)xi * (double)orr;
+          if constexpr (FUSED) {
+            const V vo = wo[pb / NT];
+            const R pr = orr - fc3[k] * vo[2 * k], pi = oi - fc3[k] * vo[2 * k + 1];
+            pn[2 * k] = pr;
+            pn[2 * k + 1] = pi;
+            acc[k][0] += (double)xr * (double)orr + (double)xi * (double)oi;
+            acc[k][1] += (double)pr * (double)pr + (double)pi * (double)pi;
+            acc[k][2] += (double)xr * (double)pr + (double)xi * (double)pi;
+            acc[k][3] += (double)xr * (double)xr + (double)xi * (double)xi;
+          } else {
+            acc[k][0] += (double)xr * (double)orr + (double)xi * (double)oi;
+            acc[k][1] += (double)xr * (double)oi - (double)xi * (double)orr;
+          }
         }
-        __builtin_nontemporal_store(out, &y[(int64_t)r * RV + g * 8 + sub]);
+        const int64_t io = (int64_t)r * RV + g * 8 + sub;
+        if constexpr (FUSED) {
+          __builtin_nontemporal_store(pn, &y[io]);
+          __builtin_nontemporal_store(xv, &xo[io]);
+        } else {
+          __builtin_nontemporal_store(out, &y[io]);
+        }
       }
     }
   };
@@ -583,14 +717,19 @@ __global__ __launch_bounds__(NT) void k_hv_block_stencil(BlockHvArgs a, BlockSte
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
       const int q = g * GQ + sub * NQ + k;
-      qs[k] = s_qt[0][q];
+      qs[k] = FUSED ? (R)1 : s_qt[0][q];
       qm[k] = s_qt[1][q];
       e1r[k] = s_qt[2][q];
       e1i[k] = s_qt[3][q];
       eLr[k] = s_qt[4][q];
       eLi[k] = s_qt[5][q];
-      dre[k] = 0.0;
-      dim_[k] = 0.0;
+      if constexpr (FUSED) {
+        fc1[k] = s_qt[6][q];
+        fc2[k] = s_qt[7][q];
+        fc3[k] = s_qt[8][q];
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2) acc[k][s2] = 0.0;
     }
     // line entries of slabs c0 - 1 .. c1 (entry (cs, j) at s_tl[(cs - c0 + 1) NL + j])
     const int nent = (c1 - c0 + 2) * NL;
@@ -603,54 +742,51 @@ __global__ __launch_bounds__(NT) void k_hv_block_stencil(BlockHvArgs a, BlockSte
     __syncthreads();
     auto te = [&](int cs) { return s_tl + (cs - c0 + 1) * NL; };
     {
-      V preZ[PER];
-      fetch(preX, a0, g, te(c0 - 1));
-      fetch(preY, a0, g, te(c0));
-      fetch(preZ, a0, g, te(c0 + 1));
-      store(preX, (c0 + kSRing - 1) % kSRing);
-      store(preY, c0 % kSRing);
-      store(preZ, (c0 + 1) % kSRing);
+      V preZ[PER], wZ[PER];
+      fetch(preX, wX, a0, g, te(c0 - 1));
+      fetch(preY, wY, a0, g, te(c0));
+      fetch(preZ, wZ, a0, g, te(c0 + 1));
+      store(preX, wX, (c0 + kSRing - 1) % kSRing, -1);
+      store(preY, wY, c0 % kSRing, 0);
+      store(preZ, wZ, (c0 + 1) % kSRing, 1);
     }
-    if (c0 + 2 <= c1) fetch(preX, a0, g, te(c0 + 2));
+    if (c0 + 2 <= c1) fetch(preX, wX, a0, g, te(c0 + 2));
     __syncthreads();
     for (int c = c0; c < c1; c += 2) {
-      if (c + 3 <= c1) fetch(preY, a0, g, te(c + 3));
-      compute(a0, b0, g, c, te(c));
-      if (c + 2 <= c1) store(preX, (c + 2) % kSRing);
+      if (c + 3 <= c1) fetch(preY, wY, a0, g, te(c + 3));
+      compute(a0, b0, g, c, te(c), wown[0]);
+      if (c + 2 <= c1) store(preX, wX, (c + 2) % kSRing, 0);
       __syncthreads();
       if (c + 1 < c1) {
-        if (c + 4 <= c1) fetch(preX, a0, g, te(c + 4));
-        compute(a0, b0, g, c + 1, te(c + 1));
-        if (c + 3 <= c1) store(preY, (c + 3) % kSRing);
+        if (c + 4 <= c1) fetch(preX, wX, a0, g, te(c + 4));
+        compute(a0, b0, g, c + 1, te(c + 1), wown[1]);
+        if (c + 3 <= c1) store(preY, wY, (c + 3) % kSRing, 1);
         __syncthreads();
       }
     }
     // the task's partial sums into the block's per-q totals, in a fixed order
     // (the ring is free until the next task's prologue)
-    double *s_red = (double *)smem;  // [NT][NQ][2]
+    double *s_red = (double *)smem;  // [NT][NQ][NS]
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      s_red[(tid * NQ + k) * 2] = dre[k];
-      s_red[(tid * NQ + k) * 2 + 1] = dim_[k];
-    }
+    for (int k = 0; k < NQ; ++k)
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2) s_red[(tid * NQ + k) * NS + s2] = acc[k][s2];
     __syncthreads();
     if (tid < GQ) {
       const int sb = tid / NQ, k = tid % NQ;
-      double sr = 0.0, si = 0.0;
-      for (int p = 0; p < NT / 8; ++p) {
-        sr += s_red[((p * 8 + sb) * NQ + k) * 2];
-        si += s_red[((p * 8 + sb) * NQ + k) * 2 + 1];
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2) {
+        double sr = 0.0;
+        for (int p = 0; p < NT / 8; ++p) sr += s_red[((p * 8 + sb) * NQ + k) * NS + s2];
+        s_acc[s2][g * GQ + tid] += sr;
       }
-      s_acc[0][g * GQ + tid] += sr;
-      s_acc[1][g * GQ + tid] += si;
     }
     __syncthreads();
   }
   if (a.dot_partials) {
-    for (int q = tid; q < kQ; q += NT) {
-      a.dot_partials[2 * ((size_t)blockIdx.x * kQ + q)] = s_acc[0][q];
-      a.dot_partials[2 * ((size_t)blockIdx.x * kQ + q) + 1] = s_acc[1][q];
-    }
+    for (int q = tid; q < kQ; q += NT)
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2) a.dot_partials[NS * ((size_t)blockIdx.x * kQ + q) + s2] = s_acc[s2][q];
   }
 }
 
@@ -659,11 +795,11 @@ using BSFn = void (*)(BlockHvArgs, BlockStencilArgs);
 // Block stencil kernel for B lines per tile and NT threads per block (256:
 // B even, 2..14; 512: B = 4, 8, 12), or nullptr.
 template <class R>
-BSFn block_stencil_kernel_r(int B, int NT, size_t *lds, int slabs) {
-#define BSK(b, nt)                                  \
-  if (B == b && NT == nt) {                         \
-    *lds = block_stencil_lds<R, b>(slabs);          \
-    return k_hv_block_stencil<R, b, nt>;            \
+BSFn block_stencil_kernel_r(int B, int NT, size_t *lds, int slabs, bool fused) {
+#define BSK(b, nt)                                                                            \
+  if (B == b && NT == nt) {                                                                   \
+    *lds = block_stencil_lds<R, b>(slabs);                                                    \
+    return fused ? k_hv_block_stencil<R, b, nt, true> : k_hv_block_stencil<R, b, nt, false>; \
   }
   BSK(2, 256) BSK(4, 256) BSK(6, 256) BSK(8, 256) BSK(10, 256) BSK(12, 256) BSK(14, 256)
   BSK(4, 512) BSK(8, 512) BSK(12, 512)
@@ -671,9 +807,9 @@ BSFn block_stencil_kernel_r(int B, int NT, size_t *lds, int slabs) {
   return nullptr;
 }
 
-BSFn block_stencil_kernel(int prec, int B, int NT, size_t *lds, int slabs) {
-  return prec == QS3X_FP64 ? block_stencil_kernel_r<double>(B, NT, lds, slabs)
-                           : block_stencil_kernel_r<float>(B, NT, lds, slabs);
+BSFn block_stencil_kernel(int prec, int B, int NT, size_t *lds, int slabs, bool fused = false) {
+  return prec == QS3X_FP64 ? block_stencil_kernel_r<double>(B, NT, lds, slabs, fused)
+                           : block_stencil_kernel_r<float>(B, NT, lds, slabs, fused);
 }
 
 // --------------------------------------------------------- block update
@@ -772,6 +908,38 @@ __global__ __launch_bounds__(kBThreads) void k_finalize_q(const double *part, in
       sc[(size_t)i0 * kQ + q] = b;
       if (i1 >= 0) sc[(size_t)i1 * kQ + q] = b > 0 ? 1.0 / b : 0.0;
     }
+  }
+}
+
+// Per-q finalize of the fused step (one block per q): from the four sums,
+// alpha = Re<v,Hv>, beta^2 = |P|^2 - 2 alpha Re<v,P> + alpha^2 |v|^2 (the
+// norm of P - alpha v); sc[ia] = alpha, sc[io] = alpha + shift, sc[ib] = beta,
+// sc[is] = 1 / beta (0 if beta = 0).
+__global__ __launch_bounds__(kBThreads) void k_finalize_q4(const double *part, int nb, double *sc, int ia, int io,
+                                                           int ib, int is, double shift) {
+  const int q = blockIdx.x;
+  double t[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nb; b += kBThreads)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t[k] += part[4 * ((size_t)b * kQ + q) + k];
+  __shared__ double red[4][kBThreads];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) red[k][threadIdx.x] = t[k];
+  __syncthreads();
+  for (int s = kBThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double al = red[0][0], pp = red[1][0], vp = red[2][0], vv = red[3][0];
+    const double b2 = pp - 2.0 * al * vp + al * al * vv;
+    const double b = sqrt(b2 > 0 ? b2 : 0.0);
+    sc[(size_t)ia * kQ + q] = al;
+    sc[(size_t)io * kQ + q] = al + shift;
+    sc[(size_t)ib * kQ + q] = b;
+    sc[(size_t)is * kQ + q] = b > 0 ? 1.0 / b : 0.0;
   }
 }
 
@@ -1030,15 +1198,21 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
     d_eqr.upload(eqr.data(), eqr.size());
     d_psi.upload(psi, (size_t)2 * gs->dim);
     const size_t es = prec == QS3X_FP64 ? 16 : 8;
-    DevBuf<unsigned char> U[3];
-    for (auto &u : U) u.alloc(es * (size_t)dim * kQ);
+    // Fused step (option block_fused, block-stencil sectors): buffers P, v
+    // and their successors; otherwise u_{i-1}, u_i, w.
+    const bool want_fused = options().block_fused && options().block_stencil && NT == 4 &&
+                            stencil_applies(&tgt);
+    DevBuf<unsigned char> U[4];
+    for (int k = 0; k < (want_fused ? 4 : 3); ++k) U[k].alloc(es * (size_t)dim * kQ);
     const int steps = (int)std::min<int64_t>(itr_dsf, dim);
     const int NS = steps + 2;
     // per-q scalar slots [slot][kQ]: arec [NS] | aout [NS] | beta [NS] | scale [NS+1] | norm1, tmp
     const int A_REC = 0, A_OUT = NS, BETA = 2 * NS, SCALE = 3 * NS, NORM1 = 4 * NS + 2, TMP = 4 * NS + 3;
     DevBuf<double> sc((size_t)(4 * NS + 8) * kQ);
     const int gridS = cus() * 2, gridU = cus() * 8;
-    DevBuf<double> part((size_t)2 * kQ * std::max(gridU, gridS * 4));
+    // per-block partial sums: 2 per q (4 for the fused step), at most 2
+    // blocks per CU for a stencil launch plus the per-row kernel's grid
+    DevBuf<double> part((size_t)4 * kQ * std::max(gridU, gridS * 4));
     QS3X_HIP(hipStreamSynchronize(st));
     const double t1 = now_s();
     // f_q = S^-_q psi; norm1_q = |f_q|^2 (FP64); scale[0] = 1 / sqrt(norm1)
@@ -1098,9 +1272,9 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
     // Clear rows through the block stencil where it applies, the others
     // (the sector's non-clear rows, about 1 % at L = 1000) through the
     // per-row block kernel; their partial sums follow the stencil's.
-    BSFn hvs = nullptr;
+    BSFn hvs = nullptr, hvsF = nullptr;
     size_t lds_s = 0;
-    int gridSt = 0, gridSl = 0, ntasks = 0, ntS = 256;
+    int gridSt = 0, gridSl = 0, ntasks = 0, ntS = 256, gridStF = 0;
     DevBuf<int4> st_tasks;
     BlockStencilArgs sarg{};
     std::vector<int4> st_lines;
@@ -1121,23 +1295,38 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
       if (!tasks.empty()) st_tasks.upload(tasks.data(), tasks.size());
       if (lds_s > 64 * 1024)
         QS3X_HIP(hipFuncSetAttribute((const void *)hvs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_s));
-      int bpc = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, hvs, ntS, lds_s) != hipSuccess || bpc < 1) bpc = 1;
       // a multiple of the 8 XCDs: block g takes tasks 8 m + g % 8, its XCD's tiles
-      gridSt = (int)std::min<int64_t>((int64_t)cus() * bpc, std::max(ntasks, 8));
-      gridSt = std::max(8, gridSt / 8 * 8);
+      auto grid_of = [&](BSFn k) {
+        int bpc = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, k, ntS, lds_s) != hipSuccess || bpc < 1) bpc = 1;
+        const int gr = (int)std::min<int64_t>((int64_t)cus() * bpc, std::max(ntasks, 8));
+        return std::max(8, gr / 8 * 8);
+      };
+      gridSt = grid_of(hvs);
+      if (want_fused) {
+        size_t lds_f = 0;
+        hvsF = block_stencil_kernel(prec, B, ntS, &lds_f, slabs, true);
+        QS3X_REQUIRE(hvsF && lds_f == lds_s, "qs3x_dsf_sminus_batch: no fused block stencil kernel");
+        if (lds_s > 64 * 1024)
+          QS3X_HIP(hipFuncSetAttribute((const void *)hvsF, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_s));
+        gridStF = grid_of(hvsF);
+      }
       gridSl = tgt.nslow > 0 ? (int)std::min<int64_t>((int64_t)cus() * nbpc,
                                                       ((tgt.nslow + 63) / 64 + kBWaves - 1) / kBWaves)
                              : 0;
       sarg = BlockStencilArgs{tgt.d_lines.p, st_tasks.p, ntasks};
       QS3X_REQUIRE((int64_t)(gridSt + gridSl) * kQ * 2 <= (int64_t)part.n,
                    "qs3x_dsf_sminus_batch: partial buffer (stencil)");
+      QS3X_REQUIRE((int64_t)(gridStF + gridSl) * kQ * 4 <= (int64_t)part.n,
+                   "qs3x_dsf_sminus_batch: partial buffer (fused stencil)");
     };
     if (use_stencil) {
       // defaults: the fastest measured at L = 1000, N_down = 4 in both
       // precisions (12 lines, 512 threads, 32 slabs: FP32 9.9 ms, FP64 20.5 ms
-      // per launch, against 12.3 and 25.2 ms with 6 lines, 256 threads, 8 slabs)
-      setup_stencil(options().block_stencil_lines > 0 ? options().block_stencil_lines : 12,
+      // per launch, against 12.3 and 25.2 ms with 6 lines, 256 threads, 8
+      // slabs); the fused step: 8 lines (12 spill registers at 512 threads;
+      // FP32 20.4 ms, FP64 43.7 ms per step against 22.8 and 48.4 with 4)
+      setup_stencil(options().block_stencil_lines > 0 ? options().block_stencil_lines : (want_fused ? 8 : 12),
                     options().block_stencil_threads > 0 ? options().block_stencil_threads : 512,
                     options().block_stencil_slabs > 0 ? options().block_stencil_slabs : 32);
     }
@@ -1148,7 +1337,9 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
     QS3X_HIP(hipEventCreate(&e1));
     // Option block_autotune: time one block stencil launch per tile shape
     // (lines, threads, slabs) on the S^- vectors and keep the fastest for the
-    // recurrence (qs3x_block_tune_last reports the table).
+    // recurrence (qs3x_block_tune_last reports the table).  With the fused
+    // step the fused kernel is timed, reading the S^- vectors as both P and v
+    // and writing two scratch buffers.
     block_tune_table().clear();
     if (use_stencil && options().block_autotune) {
       static const int kCand[][3] = {{12, 512, 32}, {12, 512, 16}, {12, 512, 64}, {8, 512, 32}, {4, 512, 32},
@@ -1161,10 +1352,21 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
         t.x = U[0].p;
         t.y = U[1].p;
         t.iscale = SCALE;
-        hipLaunchKernelGGL(hvs, dim3(gridSt), dim3(ntS), lds_s, st, t, sarg);  // warm-up
+        BSFn kt = hvs;
+        int gt = gridSt;
+        if (hvsF) {
+          t.iscale = -1;
+          t.shift = 0.0;
+          t.vprev = U[0].p;
+          t.xout = U[2].p;
+          t.fis = t.fia = t.fib = SCALE;
+          kt = hvsF;
+          gt = gridStF;
+        }
+        hipLaunchKernelGGL(kt, dim3(gt), dim3(ntS), lds_s, st, t, sarg);  // warm-up
         QS3X_HIP(hipGetLastError());
         QS3X_HIP(hipEventRecord(e0, st));
-        for (int r = 0; r < 2; ++r) hipLaunchKernelGGL(hvs, dim3(gridSt), dim3(ntS), lds_s, st, t, sarg);
+        for (int r = 0; r < 2; ++r) hipLaunchKernelGGL(kt, dim3(gt), dim3(ntS), lds_s, st, t, sarg);
         QS3X_HIP(hipEventRecord(e1, st));
         QS3X_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -1180,7 +1382,51 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
       block_tune_table().push_back({(double)kCand[bi][0], (double)kCand[bi][1], (double)kCand[bi][2], best_ms});
     }
     float hv_ms_total = 0;
-    for (int i = 0; i < steps; ++i) {
+    const bool fused = hvsF != nullptr;
+    size_t ldsF = 0;
+    BFn hvF = fused ? block_kernel_fused(NT, prec, key32, &ldsF, L, (int)m->bt_jxy2.size()) : nullptr;
+    QS3X_REQUIRE(!fused || (hvF && ldsF == lds), "qs3x_dsf_sminus_batch: no fused per-row block kernel");
+    if (fused && ldsF > 64 * 1024)
+      QS3X_HIP(hipFuncSetAttribute((const void *)hvF, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ldsF));
+    // The fused step (lanczos.f90:50-92 for each q): step i reads P_{i-1}
+    // and v_{i-1} (step 0: the S^- vectors f and scale[0] = 1/|f|), writes
+    // v_i and P_i = (H - e_zz) v_i - beta_{i-1} v_{i-1}; alpha_i and beta_i
+    // from its four per-q sums.  Both precisions apply H - e_zz (beta^2 =
+    // |P|^2 - 2 alpha Re<v,P> + alpha^2 |v|^2 keeps its digits for alpha of
+    // order one), and alpha + e_zz is the recurrence's alpha.  Buffers: even
+    // steps read P from U[0] and v from U[3] and write v to U[1] and P to U[2];
+    // odd steps the other way round.
+    for (int i = 0; fused && i < steps; ++i) {
+      const bool even = (i % 2) == 0;
+      BlockHvArgs f = a;
+      f.shift = 0.0;
+      f.iscale = -1;
+      f.x = even ? U[0].p : U[2].p;
+      f.vprev = i == 0 ? nullptr : (even ? U[3].p : U[1].p);
+      f.xout = even ? U[1].p : U[3].p;
+      f.y = even ? U[2].p : U[0].p;
+      f.fis = SCALE + i;
+      f.fia = i > 0 ? A_REC + i - 1 : -1;
+      f.fib = i > 0 ? BETA + i - 1 : -1;
+      f.dot_partials = part.p;
+      if (i == 1) QS3X_HIP(hipEventRecord(e0, st));
+      hipLaunchKernelGGL(hvsF, dim3(gridStF), dim3(ntS), lds_s, st, f, sarg);
+      QS3X_HIP(hipGetLastError());
+      int nbh = gridStF;
+      if (gridSl > 0) {
+        BlockHvArgs b = f;
+        b.rows = tgt.d_slow.p;
+        b.nrows = tgt.nslow;
+        b.dot_partials = part.p + (size_t)4 * kQ * gridStF;
+        hipLaunchKernelGGL(hvF, dim3(gridSl), dim3(kBThreads), ldsF, st, b);
+        QS3X_HIP(hipGetLastError());
+        nbh += gridSl;
+      }
+      hipLaunchKernelGGL(k_finalize_q4, dim3(kQ), dim3(kBThreads), 0, st, part.p, nbh, sc.p, A_REC + i, A_OUT + i,
+                         BETA + i, SCALE + i + 1, m->ezz - rfield * sztot_new);
+      QS3X_HIP(hipGetLastError());
+    }
+    for (int i = 0; !fused && i < steps; ++i) {
       void *cur = U[i % 3].p, *prev = U[(i + 2) % 3].p, *w = U[(i + 1) % 3].p;
       a.x = cur;
       a.y = w;
@@ -1255,7 +1501,7 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
       stats[3] = steps > 1 ? hv_ms_total / (steps - 1) : 0.0;  // ms per block step (H.v + update), steps 2..
       stats[4] = (double)dim;
       stats[5] = (double)steps;
-      stats[6] = hvs ? 1.0 : 0.0;                        // 1: clear rows through the block stencil
+      stats[6] = hvs ? (fused ? 2.0 : 1.0) : 0.0;        // 1: clear rows through the block stencil, 2: fused step
       stats[7] = hvs ? (double)tgt.nslow : 0.0;          // rows through the per-row block kernel then
     }
   });

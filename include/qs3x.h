@@ -261,8 +261,10 @@ int qs3x_sector_build_all_k(qs3x_model *m, int32_t nod, qs3x_sector **out);
  * stats (8 doubles, may be NULL): [seconds setup, seconds S^-, seconds
  * recurrence, ms per block step, rows of the all-k sector, steps, 1 if the
  * clear rows ran through the block gap-lattice stencil (N_down + 1 = 4,
- * uniform bonds; option block_stencil) else 0, rows through the per-row
- * block kernel in that case]. */
+ * uniform bonds; option block_stencil), 2 if it ran the fused step (option
+ * block_fused: H.v and the three-term update in one pass, alpha and beta
+ * from four per-q sums; four vector buffers instead of three) else 0, rows
+ * through the per-row block kernel in those cases]. */
 int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const double *pos,
                           const double *psi, int32_t itr_dsf, double rfield, int32_t prec,
                           double *alpha, double *beta2, double *norm1, double *stats);

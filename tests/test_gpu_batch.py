@@ -203,16 +203,20 @@ def test_block_stencil_matches_per_row_block_kernel(L, steps, prec_name):
         a0, b0, n0, st0 = q3.dsf_sminus_batch(gs, Ks, pos, psi, steps, prec=prec)
     finally:
         q3.set_option("block_stencil", 1)
-    a1, b1, n1, st1 = q3.dsf_sminus_batch(gs, Ks, pos, psi, steps, prec=prec)
-    # a second tile shape: 6 lines per tile, 256 threads, 8 slabs per task
+    q3.set_option("block_fused", 0)  # the separate H.v and update
     try:
-        q3.set_option("block_stencil_lines", 6)
-        q3.set_option("block_stencil_threads", 256)
-        q3.set_option("block_stencil_slabs", 8)
-        a2, b2, n2, st2 = q3.dsf_sminus_batch(gs, Ks, pos, psi, steps, prec=prec)
+        a1, b1, n1, st1 = q3.dsf_sminus_batch(gs, Ks, pos, psi, steps, prec=prec)
+        # a second tile shape: 6 lines per tile, 256 threads, 8 slabs per task
+        try:
+            q3.set_option("block_stencil_lines", 6)
+            q3.set_option("block_stencil_threads", 256)
+            q3.set_option("block_stencil_slabs", 8)
+            a2, b2, n2, st2 = q3.dsf_sminus_batch(gs, Ks, pos, psi, steps, prec=prec)
+        finally:
+            for o in ("block_stencil_lines", "block_stencil_threads", "block_stencil_slabs"):
+                q3.set_option(o, 0)
     finally:
-        for o in ("block_stencil_lines", "block_stencil_threads", "block_stencil_slabs"):
-            q3.set_option(o, 0)
+        q3.set_option("block_fused", 1)
     assert st0["block_stencil"] == 0 and st1["block_stencil"] == 1 and st2["block_stencil"] == 1
     assert 0 < st1["rows_per_row_kernel"] < st1["dim_all_k"]
     assert np.array_equal(n0, n1)
@@ -248,7 +252,7 @@ def test_block_stencil_autotune():
     finally:
         q3.set_option("block_autotune", 0)
     tab = q3.block_tune_last()
-    assert st1["block_stencil"] == 1 and len(tab) >= 3
+    assert st1["block_stencil"] >= 1 and len(tab) >= 3
     timed, chosen = tab[:-1], tab[-1]
     assert all(r[3] > 0 for r in timed)
     assert chosen[3] <= min(r[3] for r in timed) / 0.98
@@ -259,3 +263,52 @@ def test_block_stencil_autotune():
             continue
         assert np.max(np.abs(a1[j, :6] - a0[j, :6])) <= 1e-11 * max(1.0, np.max(np.abs(a0[j, :6])))
         assert np.max(np.abs(b1[j, :6] - b0[j, :6])) <= 1e-11 * max(1e-300, np.max(np.abs(b0[j, :6])))
+
+
+@pytest.mark.parametrize("L,steps", [(40, 30), (120, 20), (300, 12)])
+@pytest.mark.parametrize("prec_name", ["fp64", "fp32"])
+def test_block_fused_step_matches_separate_kernels(L, steps, prec_name):
+    """The fused block step (option block_fused 1, the default on stencil
+    sectors: the block stencil and the per-row block kernel form v_i from
+    P_{i-1} and v_{i-1} as they read them and write v_i and P_i; alpha and
+    beta per q from four sums) against the block stencil H.v followed by the
+    block update (option block_fused 0), the recurrence of
+    get_expectation_values.f90:481-592 for all 64 q.  FP64 coefficients to
+    1e-10 over the first steps, FP32 to the FP32 bar of this file; also with a
+    second tile shape (6 lines, 256 threads, 8 slabs per task)."""
+    print(f"[gpu] fused block step vs separate kernels L={L} {prec_name}", flush=True)
+    q3 = _q3()
+    prec = q3.FP64 if prec_name == "fp64" else q3.FP32
+    gm = q3.Model(L, L, 1, 1, q3.chain_bonds(L, [(1, 1.0, 0.8)]))
+    gs = q3.Sector.build(gm, 3, (0.0, 0.0, 0.0))
+    E0, psi, _ = q3.lanczos_routines(gs, maxitr=2000, lnc_ene_conv=1e-13, prec=q3.FP64, seed=5)
+    pos = np.array([[i, 0.0, 0.0] for i in range(L)], dtype=float)
+    Ks = [int(round(L * m / 64)) % L for m in range(64)]
+    q3.set_option("block_fused", 0)
+    try:
+        a0, b0, n0, st0 = q3.dsf_sminus_batch(gs, Ks, pos, psi, steps, prec=prec)
+    finally:
+        q3.set_option("block_fused", 1)
+    a1, b1, n1, st1 = q3.dsf_sminus_batch(gs, Ks, pos, psi, steps, prec=prec)
+    try:
+        q3.set_option("block_stencil_lines", 6)
+        q3.set_option("block_stencil_threads", 256)
+        q3.set_option("block_stencil_slabs", 8)
+        a2, b2, n2, st2 = q3.dsf_sminus_batch(gs, Ks, pos, psi, steps, prec=prec)
+    finally:
+        for o in ("block_stencil_lines", "block_stencil_threads", "block_stencil_slabs"):
+            q3.set_option(o, 0)
+    assert st0["block_stencil"] == 1 and st1["block_stencil"] == 2 and st2["block_stencil"] == 2
+    assert np.array_equal(n0, n1) and np.array_equal(n0, n2)
+    k = 6 if prec == q3.FP64 else 4
+    tol_a, tol_b = (1e-10, 1e-10) if prec == q3.FP64 else (1e-6, 1e-4)
+    worst = [0.0, 0.0]
+    for j in range(64):
+        if n0[j] <= 1e-20:
+            continue
+        for a_, b_ in ((a1, b1), (a2, b2)):
+            ea = np.max(np.abs(a_[j, :k] - a0[j, :k])) / max(1.0, np.max(np.abs(a0[j, :k])))
+            eb = np.max(np.abs(b_[j, :k] - b0[j, :k])) / max(1e-300, np.max(np.abs(b0[j, :k])))
+            worst = [max(worst[0], ea), max(worst[1], eb)]
+            assert ea <= tol_a and eb <= tol_b, (j, Ks[j], ea, eb)
+    print(f"[gpu]   worst alpha {worst[0]:.2e} beta^2 {worst[1]:.2e}", flush=True)
