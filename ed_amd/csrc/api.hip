@@ -150,6 +150,9 @@ int qs3x_set_option(const char *name, double value) {
     } else if (n == "block_stencil_threads") {
       QS3X_REQUIRE(value == 0 || value == 256 || value == 512, "block_stencil_threads must be 0, 256 or 512");
       options().block_stencil_threads = (int)value;
+    } else if (n == "group_hop_cache") {
+      QS3X_REQUIRE(value == 0 || value == 1, "group_hop_cache must be 0 or 1");
+      options().group_hop_cache = (int)value;
     } else if (n == "block_fused") {
       QS3X_REQUIRE(value == 0 || value == 1, "block_fused must be 0 or 1");
       options().block_fused = (int)value;

@@ -98,6 +98,7 @@ struct Options {
   int block_stencil_slabs = 0;
   int block_stencil_threads = 0;  // threads per block of the block stencil (0: 512; 256 or 512)
   int block_autotune = 0;         // 1: the DSF batch times the block stencil tile shapes and keeps the fastest
+  int group_hop_cache = 1;        // 1: the group solve's later steps read the hops its first step found
   int block_fused = 1;            // 1: the DSF batch on block-stencil sectors runs the fused step (H.v + update)
   // dense eigensolver of qs3x_full_diag: 0 = QR iteration on the tridiagonal
   // form (rocSOLVER zheev, the reference's LAPACK zheev algorithm; default),

@@ -126,6 +126,14 @@ struct MultiArgs {
   int scale_base;  // slot base of the scale of v1 (slot = scale_base + step)
   const int *step;
   double *part;  // [tiles] complex
+  // Hop cache (null: none): per hop slot h and row of the launch (row
+  // (t - t0) kMT + thread), the target's index (-1: absent) and a code
+  // (0: no hop; else bit 31 | bond type << 16 | stabiliser order << 12 |
+  // ell), written by step 0 and read by every later step instead of the
+  // representative search.
+  int32_t *rec_id;
+  uint32_t *rec_code;
+  int64_t rec_rows;
 };
 
 template <int N, int DEG, class R, class KeyT>
@@ -193,15 +201,34 @@ __global__ __launch_bounds__(kMT) void k_hv_multi(MultiArgs a) {
         }
       }
       R anti = 0;
-      int hc[H], hcnt[H], hei[H];
+      int hc[H], hcnt[H], hei[H], hbt[H];
       R hj[H];
+      int hid[H];
 #pragma unroll
       for (int h = 0; h < H; ++h) {
         hc[h] = -1;
         hcnt[h] = 1;
         hei[h] = 0;
         hj[h] = 0;
+        hbt[h] = -1;
       }
+      const int64_t rrow = (t - a.t0) * kMT + threadIdx.x;
+      if (a.rec_id && step > 0) {
+        // the hops recorded at step 0, in the same slot order (so anti sums
+        // in the same order as the search below)
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+          const uint32_t cd = a.rec_code[(size_t)h * a.rec_rows + rrow];
+          hid[h] = a.rec_id[(size_t)h * a.rec_rows + rrow];
+          if (cd) {
+            const int btp = (int)((cd >> 16) & 0x7fffu), ell = (int)(cd & 0xfffu);
+            anti += s_jz4[btp];
+            hj[h] = s_jxy2[btp];
+            hcnt[h] = (int)((cd >> 12) & 0xfu);
+            hei[h] = ell ? ell + 3 * (L + 1) : 0;
+          }
+        }
+      } else {
 #pragma unroll
       for (int m = 0; m < N; ++m) {
         const int st = n[m];
@@ -284,17 +311,28 @@ __global__ __launch_bounds__(kMT) void k_hv_multi(MultiArgs a) {
           hcnt[h] = count;
           hei[h] = isrep ? 0 : (L + 1 - mx) + 3 * (L + 1);
           hj[h] = s_jxy2[btp];
+          hbt[h] = btp;
         }
       }
       uint2 wd[H];
 #pragma unroll
       for (int h = 0; h < H; ++h) wd[h] = d.bitmap[(uint32_t)(hc[h] >= 0 ? hc[h] : 0) >> 5];
-      int hid[H];
 #pragma unroll
       for (int h = 0; h < H; ++h) {
         const uint32_t b = (uint32_t)hc[h] & 31u;
         const bool in = hc[h] >= 0 && ((wd[h].x >> b) & 1u);
         hid[h] = in ? (int)wd[h].y + __popc(wd[h].x & ((1u << b) - 1u)) : -1;
+      }
+      if (a.rec_id) {
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+          const uint32_t ell = hei[h] ? (uint32_t)(hei[h] - 3 * (L + 1)) : 0u;
+          const uint32_t cd =
+              hbt[h] >= 0 ? (1u << 31) | ((uint32_t)hbt[h] << 16) | ((uint32_t)hcnt[h] << 12) | ell : 0u;
+          a.rec_id[(size_t)h * a.rec_rows + rrow] = hid[h];
+          a.rec_code[(size_t)h * a.rec_rows + rrow] = cd;
+        }
+      }
       }
       C y[H];
 #pragma unroll
@@ -444,6 +482,9 @@ MFn multi_kernel(int N, int prec, int deg, bool key32) {
   }
 }
 
+// Largest hop cache of a group solve (all N_down groups together).
+constexpr size_t kHopCacheMax = (size_t)32 << 30;
+
 int cus() {
   static int c = 0;
   if (!c) {
@@ -547,6 +588,9 @@ int qs3x_lanczos_group(qs3x_sector *const *secs, int32_t nsec, int32_t prec, int
       MFn f;
       size_t lds;
       int grid;
+      int32_t *rid;
+      uint32_t *rcode;
+      int64_t rrows;
     };
     std::vector<Grp> grps;
     int kb = 1;
@@ -555,7 +599,7 @@ int qs3x_lanczos_group(qs3x_sector *const *secs, int32_t nsec, int32_t prec, int
       const int N = secs[order[j]]->nod;
       int k = j;
       while (k < S && secs[order[k]]->nod == N) ++k;
-      Grp g{N, j, k, nullptr, 0, 1};
+      Grp g{N, j, k, nullptr, 0, 1, nullptr, nullptr, 0};
       g.f = multi_kernel(N, prec, m->max_deg, (int64_t)(N - 1) * kb <= 32);
       QS3X_REQUIRE(g.f, "qs3x_lanczos_group: no kernel for this N_down / bond degree");
       const size_t rs = prec == QS3X_FP64 ? 8 : 4;
@@ -569,8 +613,33 @@ int qs3x_lanczos_group(qs3x_sector *const *secs, int32_t nsec, int32_t prec, int
       int nbpc = 0;
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbpc, g.f, kMT, g.lds) != hipSuccess || nbpc < 1) nbpc = 1;
       g.grid = (int)std::max<int64_t>(1, std::min<int64_t>(tile0[k] - tile0[j], (int64_t)cus() * nbpc));
+      g.rrows = (tile0[k] - tile0[j]) * kMT;
       grps.push_back(g);
       j = k;
+    }
+    // Option group_hop_cache: every step after the first reads the hops the
+    // first one found (8 bytes per hop slot and row) instead of searching
+    // for representatives again, when all groups' tables fit in the cap.
+    std::vector<DevBuf<int32_t>> rids;
+    std::vector<DevBuf<uint32_t>> rcodes;
+    size_t hop_cache_bytes = 0;
+    {
+      const int deg_t = m->max_deg <= 2 ? 2 : 4;  // the kernels' hop slots per spin
+      size_t cache = 0;
+      for (const Grp &g : grps) cache += (size_t)8 * g.N * deg_t * (size_t)g.rrows;
+      // the code's fields: ell < 4096, bond type < 32768
+      const bool fits = m->NOS < 4096 && m->bt_jxy2.size() < 32768;
+      if (options().group_hop_cache && fits && cache <= kHopCacheMax) {
+        for (Grp &g : grps) {
+          const size_t n = (size_t)g.N * deg_t * (size_t)g.rrows;
+          rids.emplace_back(n);
+          rcodes.emplace_back(n);
+          g.rid = rids.back().p;
+          g.rcode = rcodes.back().p;
+        }
+        hop_cache_bytes = 0;
+        for (const auto &r : rids) hop_cache_bytes += 8 * r.n;
+      }
     }
     const double shift = prec == QS3X_FP32 ? m->ezz : 0.0;
     // one Lanczos step (slots from the device step counter)
@@ -602,6 +671,9 @@ int qs3x_lanczos_group(qs3x_sector *const *secs, int32_t nsec, int32_t prec, int
         a.scale_base = sl.scale;
         a.step = d_step.p;
         a.part = part.p;
+        a.rec_id = g.rid;
+        a.rec_code = g.rcode;
+        a.rec_rows = g.rrows;
         hipLaunchKernelGGL(g.f, dim3(g.grid), dim3(kMT), g.lds, st, a);
       }
       hipLaunchKernelGGL(k_finalize_multi_sec, dim3(S), dim3(kMT), 0, st, part.p, d_tile0.p, d_act.p, S, sc.p, 1,
@@ -694,7 +766,7 @@ int qs3x_lanczos_group(qs3x_sector *const *secs, int32_t nsec, int32_t prec, int
       stats[3] = (double)launches;
       stats[4] = (double)rows;
       stats[5] = (double)grps.size();
-      stats[6] = 0.0;
+      stats[6] = (double)hop_cache_bytes;  // bytes of the hop cache (0: none)
       stats[7] = 0.0;
     }
   });

@@ -323,8 +323,8 @@ def lanczos_group(sectors, maxitr=2000, minitr=20, itrint=5, lnc_ene_conv=1e-14,
     _check(_lib.qs3x_lanczos_group(hs, len(sectors), int(prec), int(maxitr), int(minitr), int(itrint),
                                    float(lnc_ene_conv), int(seed), 1 if use_graph else 0, _p(ene), _p(its),
                                    _p(stats)))
-    keys = ["seconds", "seconds_setup", "steps", "step_launches", "rows_padded", "groups"]
-    return ene, its, dict(zip(keys, stats[:6].tolist()))
+    keys = ["seconds", "seconds_setup", "steps", "step_launches", "rows_padded", "groups", "hop_cache_bytes"]
+    return ene, its, dict(zip(keys, stats[:7].tolist()))
 
 
 def my_trlanczos_routines_complex(sector, NOE, NOK, NOM, maxitr=1000, lnc_ene_conv=1e-14, prec=FP64,

@@ -285,7 +285,13 @@ int qs3x_block_tune_last(int32_t max_rows, int32_t *n_rows, double *rows);
  * use_graph != 0 replays the step as a captured hipGraph.  ene (nsec):
  * lowest Ritz values; iters (nsec, may be NULL): steps per sector; stats (8,
  * may be NULL): [seconds total, seconds setup, group steps, step launches,
- * padded rows, N_down groups, 0, 0]. */
+ * padded rows, N_down groups, bytes of the hop cache, 0].  Option
+ * group_hop_cache (1, the default; no counterpart in the reference): step 0
+ * records every row's hops (target index and a code of bond type,
+ * stabiliser order and translation; 8 bytes per hop slot) and the later
+ * steps read them instead of searching for representatives again, when the
+ * tables of all N_down groups fit in 32 GiB; the arithmetic, and so every
+ * result, is the same to the bit. */
 int qs3x_lanczos_group(qs3x_sector *const *secs, int32_t nsec, int32_t prec, int64_t maxitr,
                        int64_t minitr, int64_t itrint, double conv, uint64_t seed, int32_t use_graph,
                        double *ene, int64_t *iters, double *stats);

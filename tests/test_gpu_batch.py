@@ -165,6 +165,38 @@ def test_lanczos_group_matches_per_sector(prec_name, graph):
             assert abs(e - ed) <= 1e-9 * max(1.0, abs(ed)), (N, K, e, ed)
 
 
+@pytest.mark.parametrize("L,J2,prec_name,graph", [(24, 0.0, "fp64", True), (24, 0.0, "fp32", False),
+                                                  (18, 0.4, "fp32", True), (18, 0.4, "fp64", False)])
+def test_lanczos_group_hop_cache_bit_identical(L, J2, prec_name, graph):
+    """Option group_hop_cache: the group's first step records every row's
+    hops and the later steps read them instead of searching for
+    representatives again.  The arithmetic is the same, so energies and step
+    counts equal those without the cache to the bit, on nearest-neighbour
+    chains (2 hop slots per spin) and J1-J2 chains (4), in both precisions,
+    with and without the captured step."""
+    print(f"[gpu] lanczos group hop cache L={L} J2={J2} {prec_name} graph={graph}", flush=True)
+    q3 = _q3()
+    prec = q3.FP64 if prec_name == "fp64" else q3.FP32
+    terms = [(1, 1.0, 0.8)] + ([(2, J2, J2)] if J2 else [])
+    gm = q3.Model(L, L, 1, 1, q3.chain_bonds(L, terms))
+    secs = []
+    for N in range(1, 6):
+        for K in range(L):
+            s = q3.Sector.build(gm, N, (q3.wavevector(K, L), 0.0, 0.0))
+            if s.dim:
+                secs.append(s)
+    kw = dict(maxitr=300, minitr=10, itrint=5, lnc_ene_conv=1e-12, prec=prec, seed=7, use_graph=graph)
+    E1, it1, st1 = q3.lanczos_group(secs, **kw)
+    q3.set_option("group_hop_cache", 0)
+    try:
+        E0, it0, st0 = q3.lanczos_group(secs, **kw)
+    finally:
+        q3.set_option("group_hop_cache", 1)
+    assert st1["hop_cache_bytes"] > 0 and st0["hop_cache_bytes"] == 0
+    assert np.array_equal(it1, it0)
+    assert np.array_equal(E1, E0), np.max(np.abs(E1 - E0))
+
+
 def test_lanczos_group_long_chain():
     """A group of long-chain sectors (L = 6000, N_down = 2, k = 0..3; about
     96 KB of LDS tables, beyond the default 64 KB): each E0 equals the
