@@ -1200,8 +1200,14 @@ int qs3x_dsf_sminus_batch(qs3x_sector *gs, int32_t nq, const int32_t *K, const d
     const size_t es = prec == QS3X_FP64 ? 16 : 8;
     // Fused step (option block_fused, block-stencil sectors): buffers P, v
     // and their successors; otherwise u_{i-1}, u_i, w.
-    const bool want_fused = options().block_fused && options().block_stencil && NT == 4 &&
-                            stencil_applies(&tgt);
+    bool want_fused = options().block_fused && options().block_stencil && NT == 4 && stencil_applies(&tgt);
+    if (want_fused) {
+      // the fourth buffer only where it fits beside the other three (1 GiB
+      // spare); otherwise the separate block stencil and update
+      size_t fr = 0, tot = 0;
+      if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr < 4 * es * (size_t)dim * kQ + ((size_t)1 << 30))
+        want_fused = false;
+    }
     DevBuf<unsigned char> U[4];
     for (int k = 0; k < (want_fused ? 4 : 3); ++k) U[k].alloc(es * (size_t)dim * kQ);
     const int steps = (int)std::min<int64_t>(itr_dsf, dim);
