@@ -1057,6 +1057,7 @@ __global__ __launch_bounds__(kThreads) void k_hv_stencil4(HvArgs a, StencilArgs 
 // Rows of an N_down = 4 stencil sector that are not clear (the stencil kernel
 // skips them): a count per chunk of kSlowChunk rows, then the rows in order.
 constexpr int kSlowChunk = 4096;
+constexpr int kSlowGridMax = 2048;  // blocks of the slow-row kernel (with the stencil's, within kMaxBlocks)
 __device__ __forceinline__ bool stencil_slow(uint64_t w, int bits, int L) {
   const uint64_t m = (1ull << bits) - 1ull;
   const int av = (int)((w >> bits) & m), b = (int)((w >> (2 * bits)) & m), c = (int)((w >> (3 * bits)) & m);
@@ -1510,7 +1511,10 @@ static bool launch_stencil_impl(qs3x_sector *s, int prec, const HvArgs &a, hipSt
       done[f] = pl.lds;
     }
   }
-  const int slow_grid = s->nslow > 0 ? (int)std::min<int64_t>((s->nslow + kThreads - 1) / kThreads, 512) : 0;
+  // one row per thread (a latency-bound key search: more rows per thread
+  // serialise their dependent loads), up to kSlowGridMax blocks
+  const int slow_grid =
+      s->nslow > 0 ? (int)std::min<int64_t>((s->nslow + kThreads - 1) / kThreads, kSlowGridMax) : 0;
   int &pgrid = fused ? pl.grid_fused : pl.grid;
   const void *&pfor = fused ? pl.grid_fused_for : pl.grid_for;
   if (pgrid == 0 || pfor != (const void *)f) {
@@ -1531,7 +1535,7 @@ static bool launch_stencil_impl(qs3x_sector *s, int prec, const HvArgs &a, hipSt
     const int by_lds = std::max(1, (int)((160 * 1024) / (pl.lds + at.sharedSizeBytes)));
     const int per_cu = std::min(by_regs, by_lds);
     int grid = std::min<int64_t>(pl.ntasks, (int64_t)cus * per_cu);
-    grid = std::min(grid, kMaxBlocks - 512);
+    grid = std::min(grid, kMaxBlocks - kSlowGridMax);
     pgrid = std::max(8, (grid / 8) * 8);
   }
   StencilArgs sa{s->d_lines.p, pl.tasks.p, pl.ntasks, pl.B};
