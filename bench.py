@@ -538,9 +538,12 @@ def main():
                                          "conv": args.conv_mixed, "check_every": args.check_every},
                  "fp64_plain_settings": {"conv": args.conv_plain, "itrint": 5, "minitr": 20}}
         if args.NOM64 > 0:
-            tr64 = thick(q3.FP64, args.NOM64)
-            log(f"FP64 thick restart (NOM {args.NOM64}): E0={tr64['E0']:.15f}, {tr64}")
-            solve["fp64_thick_restart"] = tr64
+            try:
+                tr64 = thick(q3.FP64, args.NOM64)
+                log(f"FP64 thick restart (NOM {args.NOM64}): E0={tr64['E0']:.15f}, {tr64}")
+                solve["fp64_thick_restart"] = tr64
+            except q3.QS3XError as e:  # e.g. its 252 GB basis not available on this device
+                log(f"FP64 thick restart (NOM {args.NOM64}) skipped: {e}")
         # time to the first Ritz value within 1e-9 of E0, allocation included:
         # plain FP64 Lanczos (the reference's ALG=1 path; its three vectors are
         # allocated inside the timed call) over each FP32-mixed solver
@@ -557,7 +560,7 @@ def main():
         # the same solver in FP64 with the same basis memory, solve times (its
         # basis allocation follows the free of the FP32 one and is reported
         # apart, in fp64_thick_restart)
-        if args.NOM64 > 0 and solve["fp64_thick_restart"]["time_to_1e-9"]:
+        if "fp64_thick_restart" in solve and solve["fp64_thick_restart"]["time_to_1e-9"]:
             d = solve["fp64_thick_restart"]["time_to_1e-9"][0]
             r2 = {}
             if tr32["time_to_1e-9"]:
