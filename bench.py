@@ -8,8 +8,9 @@ vectors, operator H - e_zz, FP64 reductions), with inputs resident in HBM.
 value = H.v applications per second.  The same JSON line also reports the
 FP64 H.v rate; the time to the ground-state energy within 1e-9 of the
 Bethe-ansatz value 242.000099091345514 for plain FP64 Lanczos, plain
-FP32-mixed Lanczos and the FP32-mixed thick restart (allocation included),
-with the thick restart's acceptance certificate; the roofline of the H.v
+FP32-mixed Lanczos, the FP32-mixed thick restart (allocation included),
+with the thick restart's acceptance certificate, and the FP64 thick restart
+with the same basis memory (--NOM64, 380 vectors); the roofline of the H.v
 launch (compulsory bytes over event time); and the CPU baseline: the
 reference's own ham_to_vec_wave_vector loop (compiled from its sources into
 oracle/_ref) timed on a bounded sample of rows with every host core.
