@@ -369,8 +369,8 @@ def main():
     ap.add_argument("--sectors-conv", type=float, default=1e-10)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one MI355X; multi-GPU runs are not part of this build "
-              "(see DESIGN.md for how the H.v would shard).", file=sys.stderr)
+        print("bench.py measures one MI355X; runs on several GPUs are not supported (the H.v would "
+              "shard by n4 slabs, with one halo slab exchanged with each neighbour).", file=sys.stderr)
         sys.exit(2)
 
     if args.calibrate_copy:
@@ -442,7 +442,7 @@ def main():
         res[name] = {"ms": ms_ev, "wall_ms": wall * 1e3 / args.steps, "hv_per_s": args.steps / wall,
                      "warmup_launches": nwarm, "alg_bytes": B, "alg_GBs": B / (ms_ev * 1e-3) / 1e9,
                      "kernel": kname, "launch": kcfg,
-                     # SURVEY's gather model: every hop's target element charged to HBM
+                     # the gather model: every hop's target element charged to HBM
                      "all_gathers_bytes": D * (16 + 2 * sv) + nnz * sv}
         log(f"{name}: {ms_ev:.4f} ms/H.v (events), {wall * 1e3 / args.steps:.4f} ms (wall), "
             f"{res[name]['alg_GBs']:.1f} GB/s of compulsory bytes ({kname})")
