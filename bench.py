@@ -10,7 +10,7 @@ FP64 H.v rate; the time to the ground-state energy within 1e-9 of the
 Bethe-ansatz value 242.000099091345514 for plain FP64 Lanczos, plain
 FP32-mixed Lanczos, the FP32-mixed thick restart (allocation included),
 with the thick restart's acceptance certificate, and the FP64 thick restart
-with the same basis memory (--NOM64, 380 vectors); the roofline of the H.v
+with the same basis memory (--NOM64, 400 vectors); the roofline of the H.v
 launch (compulsory bytes over event time); and the CPU baseline: the
 reference's own ham_to_vec_wave_vector loop (compiled from its sources into
 oracle/_ref) timed on a bounded sample of rows with every host core.
@@ -340,13 +340,14 @@ def main():
     ap.add_argument("--NOK", type=int, default=4)
     # one first cycle holds the whole convergence (the spectrum's low end is a
     # quasi-continuum, level spacing ~1e-4, so small bases restart many
-    # times: NOM 128 takes 1.5 s to 1e-9 against 0.37 s); 761 complex-float
-    # vectors are 252 GB
-    ap.add_argument("--NOM", type=int, default=760,
+    # times: NOM 128 takes 1.5 s to 1e-9 against 0.37 s; NOM 760 restarts
+    # once, 10 steps short of its convergence test); 801 complex-float
+    # vectors are 265 GB
+    ap.add_argument("--NOM", type=int, default=800,
                     help="basis of the FP32-mixed thick restart")
     # the FP64 thick restart with the same basis memory as the FP32-mixed one
-    # (381 complex-double vectors, 252 GB): the same algorithm in FP64
-    ap.add_argument("--NOM64", type=int, default=380,
+    # (401 complex-double vectors, 265 GB): the same algorithm in FP64
+    ap.add_argument("--NOM64", type=int, default=400,
                     help="basis of the FP64 thick-restart leg (0 skips it)")
     ap.add_argument("--reorth", type=int, default=2,
                     help="thick-restart reorthogonalisation: 0 full, 1 locked+local, 2 locked at restart only")
