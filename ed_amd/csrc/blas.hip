@@ -146,13 +146,36 @@ __global__ __launch_bounds__(kVecThreads) void k_dot(const typename CT<R1>::T *x
 // mode 0: sc[i0] = re, sc[i1] = im; mode 1: sc[i0] = re, sc[i1] = re + shift;
 // mode 2: b = sqrt(re): sc[i0] = b, sc[i1] = 1/b (0 if b == 0);
 // mode 3: sc[i0] = sqrt(re)
+// A thread's partial sums, blocks b = thread, thread + blockDim, ... in that
+// order, with the loads of kFinU blocks issued before they are added (the
+// kernel is one block: its time is the latency of its loads).
+constexpr int kFinU = 8;
+template <int NS>
+__device__ __forceinline__ void sum_partials(const double *part, int nb, double (&t)[NS]) {
+#pragma unroll
+  for (int k = 0; k < NS; ++k) t[k] = 0.0;
+  for (int b0 = threadIdx.x; b0 < nb; b0 += kFinU * blockDim.x) {
+    double v[kFinU][NS];
+#pragma unroll
+    for (int u = 0; u < kFinU; ++u) {
+      const int b = b0 + u * blockDim.x;
+#pragma unroll
+      for (int k = 0; k < NS; ++k) v[u][k] = b < nb ? part[NS * b + k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kFinU; ++u) {
+      if (b0 + u * (int)blockDim.x >= nb) break;
+#pragma unroll
+      for (int k = 0; k < NS; ++k) t[k] += v[u][k];
+    }
+  }
+}
+
 __global__ __launch_bounds__(kVecThreads) void k_finalize(const double *part, int nb, double *sc, int mode, int i0,
                                                           int i1, double shift) {
-  double re = 0, im = 0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-    re += part[2 * b];
-    im += part[2 * b + 1];
-  }
+  double t2[2];
+  sum_partials<2>(part, nb, t2);
+  double re = t2[0], im = t2[1];
   __shared__ double ore, oim;
   block_sum2(re, im, &ore, &oim);
   if (threadIdx.x == 0) {
@@ -179,10 +202,8 @@ __global__ __launch_bounds__(kVecThreads) void k_finalize(const double *part, in
 // beta, sc[is] = 1 / beta (0 if beta = 0).
 __global__ __launch_bounds__(kVecThreads) void k_finalize_fused(const double *part, int nb, double *sc, int ia, int io,
                                                                 int ib, int is, double shift) {
-  double t[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int b = threadIdx.x; b < nb; b += blockDim.x)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) t[k] += part[4 * b + k];
+  double t[4];
+  sum_partials<4>(part, nb, t);
   __shared__ double o[4][2];
   block_sum2(t[0], t[1], &o[0][0], &o[1][0]);
   block_sum2(t[2], t[3], &o[2][0], &o[3][0]);
