@@ -508,7 +508,13 @@ def main():
 
         # the thick restart first: its basis is the process's first large
         # allocation, as in a driver that solves once
-        tr32 = thick(q3.FP32, args.NOM, args.conv_mixed)
+        try:
+            tr32 = thick(q3.FP32, args.NOM, args.conv_mixed)
+        except q3.QS3XError as e:  # its basis not available on this device: a smaller one
+            nom = args.NOM * 3 // 4
+            log(f"FP32-mixed thick restart (NOM {args.NOM}) failed ({e}); NOM {nom} instead")
+            args.NOM = nom
+            tr32 = thick(q3.FP32, args.NOM, args.conv_mixed)
         cert = q3.solver_certificate()
         log(f"FP32-mixed thick restart (NOM {args.NOM}): E0={tr32['E0']:.15f}, {tr32}, certificate {cert}")
         torch.cuda.synchronize()
