@@ -230,109 +230,109 @@ __global__ __launch_bounds__(kMT) void k_hv_multi(MultiArgs a) {
         }
       } else {
 #pragma unroll
-      for (int m = 0; m < N; ++m) {
-        const int st = n[m];
-        const int e0 = s_ptr[st], deg = s_ptr[st + 1] - e0;
+        for (int m = 0; m < N; ++m) {
+          const int st = n[m];
+          const int e0 = s_ptr[st], deg = s_ptr[st + 1] - e0;
 #pragma unroll
-        for (int e = 0; e < DEG; ++e) {
-          if (e >= deg) break;
-          const int ent = s_adj[e0 + e];
-          const int tg = ent & 0xffff, btp = ent >> 16;
-          bool in = false;
+          for (int e = 0; e < DEG; ++e) {
+            if (e >= deg) break;
+            const int ent = s_adj[e0 + e];
+            const int tg = ent & 0xffff, btp = ent >> 16;
+            bool in = false;
 #pragma unroll
-          for (int q = 0; q < N; ++q) in |= (n[q] == tg);
-          if (in) continue;
-          anti += s_jz4[btp];
-          int nn[N];
-          KeyT K[N];
-          const int dd = tg - st;
-          if (dd == 1 || dd == -1) {
+            for (int q = 0; q < N; ++q) in |= (n[q] == tg);
+            if (in) continue;
+            anti += s_jz4[btp];
+            int nn[N];
+            KeyT K[N];
+            const int dd = tg - st;
+            if (dd == 1 || dd == -1) {
 #pragma unroll
-            for (int q = 0; q < N; ++q) nn[q] = n[q] + (q == m ? dd : 0);
+              for (int q = 0; q < N; ++q) nn[q] = n[q] + (q == m ? dd : 0);
+#pragma unroll
+              for (int j = 0; j < N; ++j) {
+                const int ta = (j - (m - 1 + N) % N + N) % N, tb = (j - m + N) % N;  // field of g[m - 1], g[m]
+                const KeyT ua = ta ? (KeyT)1 << (kb * (N - 1 - ta)) : (KeyT)0;
+                const KeyT ub = tb ? (KeyT)1 << (kb * (N - 1 - tb)) : (KeyT)0;
+                K[j] = dd > 0 ? (KeyT)(KB[j] + ua - ub) : (KeyT)(KB[j] - ua + ub);
+              }
+            } else {
+              int r[N > 1 ? N - 1 : 1];
+#pragma unroll
+              for (int q = 0; q < N - 1; ++q) r[q] = (q < m) ? n[q] : n[q + 1];
+#pragma unroll
+              for (int q = 0; q < N; ++q) {
+                const bool ltq = (q < N - 1) ? (r[q < N - 1 ? q : 0] < tg) : false;
+                const bool ltp = (q > 0) ? (r[q > 0 ? q - 1 : 0] < tg) : true;
+                nn[q] = ltq ? r[q < N - 1 ? q : 0] : (ltp ? tg : r[q > 0 ? q - 1 : 0]);
+              }
+              int g[N];
+#pragma unroll
+              for (int q = 0; q < N - 1; ++q) g[q] = nn[q + 1] - nn[q];
+              g[N - 1] = nn[0] + L - nn[N - 1];
+#pragma unroll
+              for (int j = 0; j < N; ++j) {
+                KeyT k = 0;
+#pragma unroll
+                for (int tt = 1; tt < N; ++tt) k = (KeyT)((k << kb) | (KeyT)g[(j - tt + N) % N]);
+                K[j] = k;
+              }
+            }
+            KeyT kmax = K[0];
+#pragma unroll
+            for (int j = 1; j < N; ++j) kmax = K[j] > kmax ? K[j] : kmax;
+            int mx = 0, count = 0;
 #pragma unroll
             for (int j = 0; j < N; ++j) {
-              const int ta = (j - (m - 1 + N) % N + N) % N, tb = (j - m + N) % N;  // field of g[m - 1], g[m]
-              const KeyT ua = ta ? (KeyT)1 << (kb * (N - 1 - ta)) : (KeyT)0;
-              const KeyT ub = tb ? (KeyT)1 << (kb * (N - 1 - tb)) : (KeyT)0;
-              K[j] = dd > 0 ? (KeyT)(KB[j] + ua - ub) : (KeyT)(KB[j] - ua + ub);
+              const bool tie = (K[j] == kmax);
+              count += tie ? 1 : 0;
+              mx = (tie && nn[j] > mx) ? nn[j] : mx;
             }
-          } else {
-            int r[N > 1 ? N - 1 : 1];
+            const bool isrep = (nn[0] == 1) && (K[0] == kmax);
+            int c = 0;
+            if (N > 1) {
+              int f[N > 1 ? N - 1 : 1];
+              int sum = 0;
 #pragma unroll
-            for (int q = 0; q < N - 1; ++q) r[q] = (q < m) ? n[q] : n[q + 1];
+              for (int tt = 1; tt < N; ++tt) {
+                f[tt - 1] = (int)((kmax >> (kb * (N - 1 - tt))) & fmask);
+                sum += f[tt - 1];
+              }
+              int P = L - sum;
+              c = s_binom[P - 1];
 #pragma unroll
-            for (int q = 0; q < N; ++q) {
-              const bool ltq = (q < N - 1) ? (r[q < N - 1 ? q : 0] < tg) : false;
-              const bool ltp = (q > 0) ? (r[q > 0 ? q - 1 : 0] < tg) : true;
-              nn[q] = ltq ? r[q < N - 1 ? q : 0] : (ltp ? tg : r[q > 0 ? q - 1 : 0]);
+              for (int mm = 2; mm < N; ++mm) {
+                P += f[N - mm];
+                c += s_binom[(mm - 1) * (L - 1) + (P - 1)];
+              }
             }
-            int g[N];
-#pragma unroll
-            for (int q = 0; q < N - 1; ++q) g[q] = nn[q + 1] - nn[q];
-            g[N - 1] = nn[0] + L - nn[N - 1];
-#pragma unroll
-            for (int j = 0; j < N; ++j) {
-              KeyT k = 0;
-#pragma unroll
-              for (int tt = 1; tt < N; ++tt) k = (KeyT)((k << kb) | (KeyT)g[(j - tt + N) % N]);
-              K[j] = k;
-            }
+            const int h = m * DEG + e;
+            hc[h] = c;
+            hcnt[h] = count;
+            hei[h] = isrep ? 0 : (L + 1 - mx) + 3 * (L + 1);
+            hj[h] = s_jxy2[btp];
+            hbt[h] = btp;
           }
-          KeyT kmax = K[0];
-#pragma unroll
-          for (int j = 1; j < N; ++j) kmax = K[j] > kmax ? K[j] : kmax;
-          int mx = 0, count = 0;
-#pragma unroll
-          for (int j = 0; j < N; ++j) {
-            const bool tie = (K[j] == kmax);
-            count += tie ? 1 : 0;
-            mx = (tie && nn[j] > mx) ? nn[j] : mx;
-          }
-          const bool isrep = (nn[0] == 1) && (K[0] == kmax);
-          int c = 0;
-          if (N > 1) {
-            int f[N > 1 ? N - 1 : 1];
-            int sum = 0;
-#pragma unroll
-            for (int tt = 1; tt < N; ++tt) {
-              f[tt - 1] = (int)((kmax >> (kb * (N - 1 - tt))) & fmask);
-              sum += f[tt - 1];
-            }
-            int P = L - sum;
-            c = s_binom[P - 1];
-#pragma unroll
-            for (int mm = 2; mm < N; ++mm) {
-              P += f[N - mm];
-              c += s_binom[(mm - 1) * (L - 1) + (P - 1)];
-            }
-          }
-          const int h = m * DEG + e;
-          hc[h] = c;
-          hcnt[h] = count;
-          hei[h] = isrep ? 0 : (L + 1 - mx) + 3 * (L + 1);
-          hj[h] = s_jxy2[btp];
-          hbt[h] = btp;
         }
-      }
-      uint2 wd[H];
+        uint2 wd[H];
 #pragma unroll
-      for (int h = 0; h < H; ++h) wd[h] = d.bitmap[(uint32_t)(hc[h] >= 0 ? hc[h] : 0) >> 5];
-#pragma unroll
-      for (int h = 0; h < H; ++h) {
-        const uint32_t b = (uint32_t)hc[h] & 31u;
-        const bool in = hc[h] >= 0 && ((wd[h].x >> b) & 1u);
-        hid[h] = in ? (int)wd[h].y + __popc(wd[h].x & ((1u << b) - 1u)) : -1;
-      }
-      if (a.rec_id) {
+        for (int h = 0; h < H; ++h) wd[h] = d.bitmap[(uint32_t)(hc[h] >= 0 ? hc[h] : 0) >> 5];
 #pragma unroll
         for (int h = 0; h < H; ++h) {
-          const uint32_t ell = hei[h] ? (uint32_t)(hei[h] - 3 * (L + 1)) : 0u;
-          const uint32_t cd =
-              hbt[h] >= 0 ? (1u << 31) | ((uint32_t)hbt[h] << 16) | ((uint32_t)hcnt[h] << 12) | ell : 0u;
-          a.rec_id[(size_t)h * a.rec_rows + rrow] = hid[h];
-          a.rec_code[(size_t)h * a.rec_rows + rrow] = cd;
+          const uint32_t b = (uint32_t)hc[h] & 31u;
+          const bool in = hc[h] >= 0 && ((wd[h].x >> b) & 1u);
+          hid[h] = in ? (int)wd[h].y + __popc(wd[h].x & ((1u << b) - 1u)) : -1;
         }
-      }
+        if (a.rec_id) {
+#pragma unroll
+          for (int h = 0; h < H; ++h) {
+            const uint32_t ell = hei[h] ? (uint32_t)(hei[h] - 3 * (L + 1)) : 0u;
+            const uint32_t cd =
+                hbt[h] >= 0 ? (1u << 31) | ((uint32_t)hbt[h] << 16) | ((uint32_t)hcnt[h] << 12) | ell : 0u;
+            a.rec_id[(size_t)h * a.rec_rows + rrow] = hid[h];
+            a.rec_code[(size_t)h * a.rec_rows + rrow] = cd;
+          }
+        }
       }
       C y[H];
 #pragma unroll
